@@ -1,0 +1,579 @@
+// api.cpp -- the C-ABI entry points: the reference's acceleration / octree /
+// integrator / Python-interface functions, implemented on the GPU, plus the
+// additive device-resident entry points of grav_sim_hip_ext.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gs_internal.h"
+#include "octree.h"
+
+namespace gs {
+
+hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *m, int n,
+                                     double G, double eps, bool skip_massless,
+                                     const int *targets, int num_targets, double *out);
+
+static long long g_last_walk_interactions = 0;
+
+ErrorStatus device_acceleration(Context *c, const double *x, const double *m, int n, double G,
+                                const AccelerationParam *p, double *a)
+{
+    const double eps = p->softening_length;
+    switch (p->method) {
+        case ACCELERATION_METHOD_PAIRWISE:
+        case ACCELERATION_METHOD_MASSLESS:
+            GS_HIP(launch_direct_f64(c, x, m, n, G, eps,
+                                     p->method == ACCELERATION_METHOD_MASSLESS, a));
+            return make_success_error_status();
+        case ACCELERATION_METHOD_PAIRWISE_FP32:
+        case ACCELERATION_METHOD_MASSLESS_FP32: {
+            // massless sources carry zero mass and contribute exactly zero
+            const Fp32Config cfg = fp32_config();
+            const size_t pb = direct_f32_partial_bytes(n, cfg, c->cu_count);
+            void *part;
+            GS_TRY(slot(c, SLOT_PARTIAL, pb, &part));
+            GS_HIP(launch_direct_f32(c, x, m, n, G, eps, a, part, pb, cfg));
+            return make_success_error_status();
+        }
+        case ACCELERATION_METHOD_BARNES_HUT: {
+            DeviceTree *t;
+            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, &t));
+            GS_TRY(tree_walk(c, t, x, G, eps, p->opening_angle, a, &g_last_walk_interactions));
+            return make_success_error_status();
+        }
+        case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE:
+        case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
+            return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
+                                        "Acceleration method %d is not available yet",
+                                        p->method);
+        default:
+            return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unknown acceleration method. Got: %d",
+                                        p->method);
+    }
+}
+
+static bool known_method(int method)
+{
+    return (method >= ACCELERATION_METHOD_PAIRWISE && method <= ACCELERATION_METHOD_BARNES_HUT) ||
+           (method >= ACCELERATION_METHOD_PAIRWISE_FP32 &&
+            method <= ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE);
+}
+
+// Host-buffer form used by acceleration(): H2D x and m, compute, D2H a.
+static ErrorStatus host_acceleration(double *a, const System *system,
+                                     const AccelerationParam *p)
+{
+    if (!a || !system || !p)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    if (!known_method(p->method))
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unknown acceleration method. Got: %d",
+                                    p->method);
+    const int n = system->num_particles;
+    if (n <= 0)
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                    "Number of particles must be positive. Got: %d", n);
+    if (!system->x || !system->m)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "System array x or m is NULL");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    void *dx, *dm, *da;
+    GS_TRY(slot(c, SLOT_X, sizeof(double) * 3 * n, &dx));
+    GS_TRY(slot(c, SLOT_M, sizeof(double) * n, &dm));
+    GS_TRY(slot(c, SLOT_A, sizeof(double) * 3 * n, &da));
+    GS_HIP(hipMemcpyAsync(dx, system->x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    GS_HIP(hipMemcpyAsync(dm, system->m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    GS_TRY(device_acceleration(c, static_cast<double *>(dx), static_cast<double *>(dm), n,
+                               system->G, p, static_cast<double *>(da)));
+    GS_HIP(hipMemcpyAsync(a, da, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return make_success_error_status();
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+// ====================================================================== GsSim
+struct GsSim {
+    int n = 0;
+    double G = 1.0;
+    double *x = nullptr, *v = nullptr, *m = nullptr, *a = nullptr;
+    double *xe = nullptr, *ve = nullptr;
+};
+
+extern "C" {
+
+// ---------------------------------------------------------------- acceleration
+ErrorStatus acceleration(double *a, const System *system, const AccelerationParam *p)
+{
+    return host_acceleration(a, system, p);
+}
+
+ErrorStatus acceleration_barnes_hut(double *a, const System *system, const AccelerationParam *p)
+{
+    if (!p) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "Acceleration parameter pointer is NULL");
+    AccelerationParam q = *p;
+    q.method = ACCELERATION_METHOD_BARNES_HUT;
+    return host_acceleration(a, system, &q);
+}
+
+ErrorStatus construct_octree(LinearOctree *octree, const System *system,
+                             const AccelerationParam *p, const double *box_center,
+                             const double box_width)
+{
+    if (!octree) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "Octree pointer is NULL");
+    if (!system) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "System pointer is NULL");
+    if (!p) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "Acceleration parameter pointer is NULL");
+    const int n = system->num_particles;
+    if (n <= 0)
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                    "Number of particles must be positive. Got: %d", n);
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    void *dx, *dm;
+    GS_TRY(slot(c, SLOT_X, sizeof(double) * 3 * n, &dx));
+    GS_TRY(slot(c, SLOT_M, sizeof(double) * n, &dm));
+    GS_HIP(hipMemcpyAsync(dx, system->x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    GS_HIP(hipMemcpyAsync(dm, system->m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    DeviceTree *t;
+    GS_TRY(tree_build(c, static_cast<double *>(dx), static_cast<double *>(dm), n,
+                      p->max_num_particles_per_leaf, box_center, box_width, &t));
+    return tree_to_host(c, t, octree);
+}
+
+ErrorStatus benchmark_acceleration(const System *system, const AccelerationParam *params,
+                                   const int num_params, const int *num_times)
+{
+    if (!system || !params || !num_times)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    const int n = system->num_particles;
+    std::vector<double> ref(static_cast<size_t>(3) * n), a(static_cast<size_t>(3) * n);
+    // the MAE reference is the first parameter set, always evaluated once
+    if (num_params > 0) GS_TRY(acceleration(ref.data(), system, &params[0]));
+    fputs("Benchmarking acceleration...\n", stdout);
+    for (int i = 0; i < num_params; i++) {
+        const int times = num_times[i];
+        if (times <= 0) {
+            printf("Test %d:    Skipped since num_times: %d <= 0\n\n", i, times);
+            continue;
+        }
+        std::vector<double> rt(times);
+        double mae = 0.0;
+        for (int j = 0; j < times; j++) {
+            const double t0 = grav_get_current_time();
+            GS_TRY(acceleration(a.data(), system, &params[i]));
+            rt[j] = grav_get_current_time() - t0;
+            if (i != 0 && j == 0) {
+                for (int k = 0; k < 3 * n; k++) mae += fabs(ref[k] - a[k]);
+                mae /= n;
+            }
+        }
+        printf("Test %d:", i);
+        switch (params[i].method) {
+            case ACCELERATION_METHOD_PAIRWISE: fputs("    Method: Pairwise\n", stdout); break;
+            case ACCELERATION_METHOD_MASSLESS: fputs("    Method: Massless\n", stdout); break;
+            case ACCELERATION_METHOD_BARNES_HUT: fputs("    Method: Barnes-Hut\n", stdout); break;
+            case ACCELERATION_METHOD_PAIRWISE_FP32:
+                fputs("    Method: Pairwise (FP32 split)\n", stdout);
+                break;
+            case ACCELERATION_METHOD_MASSLESS_FP32:
+                fputs("    Method: Massless (FP32 split)\n", stdout);
+                break;
+            default:
+                return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                            "Unknown acceleration method. Got: %d",
+                                            params[i].method);
+        }
+        double mean = 0.0;
+        for (double t : rt) mean += t;
+        mean /= times;
+        double var = 0.0;
+        for (double t : rt) var += (t - mean) * (t - mean);
+        const double sd = times > 1 ? sqrt(var / (times - 1)) : 0.0;
+        printf("    Number of times: %d\n", times);
+        printf("    Avg time: %.3g (+- %.3g) s\n", mean, sd);
+        printf("    MAE: %.3g\n", mae);
+        printf("\n");
+    }
+    return make_success_error_status();
+}
+
+// ---------------------------------------------------------------- energy
+double compute_energy(const System *system)
+{
+    if (!system || system->num_particles <= 0) return 0.0;
+    Context *c;
+    ErrorStatus st = context(&c);
+    if (st.return_code != GRAV_SUCCESS) {
+        print_and_free_traceback(&st);
+        return NAN;
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int n = system->num_particles;
+    void *dx, *dv, *dm, *dp;
+    const int nparts = (n + 255) / 256;
+    if (slot(c, SLOT_X, sizeof(double) * 3 * n, &dx).return_code ||
+        slot(c, SLOT_V, sizeof(double) * 3 * n, &dv).return_code ||
+        slot(c, SLOT_M, sizeof(double) * n, &dm).return_code ||
+        slot(c, SLOT_PARTIAL, sizeof(double) * nparts, &dp).return_code)
+        return NAN;
+    std::vector<double> part(nparts);
+    int np = 0;
+    if (hipMemcpyAsync(dx, system->x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dv, system->v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(dm, system->m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream) ||
+        launch_energy(c, static_cast<double *>(dx), static_cast<double *>(dv),
+                      static_cast<double *>(dm), n, system->G, static_cast<double *>(dp), &np) ||
+        hipMemcpyAsync(part.data(), dp, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream) ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return NAN;
+    double e = 0.0;
+    for (double v : part) e += v;
+    return e;
+}
+
+void compute_energy_python(double *energy, const double G, const double *sol_state,
+                           const int32 num_snapshots, const int32 num_particles)
+{
+    std::vector<double> x(3 * static_cast<size_t>(num_particles)),
+        v(3 * static_cast<size_t>(num_particles)), m(num_particles);
+    for (int s = 0; s < num_snapshots; s++) {
+        const double *snap = sol_state + static_cast<size_t>(s) * num_particles * 7;
+        for (int i = 0; i < num_particles; i++) {
+            m[i] = snap[i * 7 + 0];
+            for (int d = 0; d < 3; d++) {
+                x[3 * i + d] = snap[i * 7 + 1 + d];
+                v[3 * i + d] = snap[i * 7 + 4 + d];
+            }
+        }
+        System sys = get_new_system();
+        sys.num_particles = num_particles;
+        sys.x = x.data();
+        sys.v = v.data();
+        sys.m = m.data();
+        sys.G = G;
+        energy[s] = compute_energy(&sys);
+    }
+}
+
+// ---------------------------------------------------------------- GsSim
+ErrorStatus gs_sim_create(GsSim **out, int n, double G)
+{
+    if (!out) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "out is NULL");
+    if (n <= 0)
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                    "Number of particles must be positive. Got: %d", n);
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GsSim *s = new GsSim();
+    s->n = n;
+    s->G = G;
+    const size_t b3 = sizeof(double) * 3 * n, b1 = sizeof(double) * n;
+    hipError_t e = hipMalloc(&s->x, b3);
+    if (!e) e = hipMalloc(&s->v, b3);
+    if (!e) e = hipMalloc(&s->m, b1);
+    if (!e) e = hipMalloc(&s->a, b3);
+    if (!e) e = hipMalloc(&s->xe, b3);
+    if (!e) e = hipMalloc(&s->ve, b3);
+    if (!e) e = hipMemsetAsync(s->xe, 0, b3, c->stream);
+    if (!e) e = hipMemsetAsync(s->ve, 0, b3, c->stream);
+    if (!e) e = hipMemsetAsync(s->a, 0, b3, c->stream);
+    if (e) {
+        gs_sim_destroy(s);
+        return hip_error(e, "gs_sim_create allocation", __FILE__, __LINE__, __func__);
+    }
+    *out = s;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_destroy(GsSim *s)
+{
+    if (!s) return make_success_error_status();
+    Context *c;
+    if (context(&c).return_code == GRAV_SUCCESS) {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(s->x);
+    (void)hipFree(s->v);
+    (void)hipFree(s->m);
+    (void)hipFree(s->a);
+    (void)hipFree(s->xe);
+    (void)hipFree(s->ve);
+    delete s;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_upload(GsSim *s, const double *x, const double *v, const double *m)
+{
+    if (!s) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "sim is NULL");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const size_t b3 = sizeof(double) * 3 * s->n, b1 = sizeof(double) * s->n;
+    if (x) GS_HIP(hipMemcpyAsync(s->x, x, b3, hipMemcpyHostToDevice, c->stream));
+    if (v) GS_HIP(hipMemcpyAsync(s->v, v, b3, hipMemcpyHostToDevice, c->stream));
+    if (m) GS_HIP(hipMemcpyAsync(s->m, m, b1, hipMemcpyHostToDevice, c->stream));
+    GS_HIP(hipMemsetAsync(s->xe, 0, b3, c->stream));
+    GS_HIP(hipMemsetAsync(s->ve, 0, b3, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_download(GsSim *s, double *x, double *v)
+{
+    if (!s) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "sim is NULL");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const size_t b3 = sizeof(double) * 3 * s->n;
+    if (x) GS_HIP(hipMemcpyAsync(x, s->x, b3, hipMemcpyDeviceToHost, c->stream));
+    if (v) GS_HIP(hipMemcpyAsync(v, s->v, b3, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_acceleration(GsSim *s, const AccelerationParam *p, double *a_host)
+{
+    if (!s || !p) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
+    if (a_host) {
+        GS_HIP(hipMemcpyAsync(a_host, s->a, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost,
+                              c->stream));
+        GS_HIP(hipStreamSynchronize(c->stream));
+    }
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_leapfrog_begin(GsSim *s, const AccelerationParam *p, double dt)
+{
+    if (!s || !p) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
+    GS_HIP(launch_lf_half_kick(c, s->v, s->ve, s->a, s->n, dt));
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_leapfrog_steps(GsSim *s, const AccelerationParam *p, double dt, int64 k)
+{
+    if (!s || !p) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    for (int64 i = 0; i < k; i++) {
+        GS_HIP(launch_lf_drift(c, s->x, s->xe, s->v, s->n, dt));
+        GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
+        GS_HIP(launch_lf_kick(c, s->v, s->ve, s->a, s->n, dt));
+    }
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_leapfrog_end(GsSim *s, double dt)
+{
+    if (!s) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "sim is NULL");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GS_HIP(launch_lf_sync(c, s->v, s->a, s->n, dt));
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_energy(GsSim *s, double *energy)
+{
+    if (!s || !energy) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int nparts = (s->n + 255) / 256;
+    void *dp;
+    GS_TRY(slot(c, SLOT_PARTIAL, sizeof(double) * nparts, &dp));
+    int np = 0;
+    GS_HIP(launch_energy(c, s->x, s->v, s->m, s->n, s->G, static_cast<double *>(dp), &np));
+    std::vector<double> part(np);
+    GS_HIP(hipMemcpyAsync(part.data(), dp, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    double e = 0.0;
+    for (double v : part) e += v;
+    *energy = e;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_sim_accel_error(GsSim *s, const AccelerationParam *p, int num_samples,
+                               double *median, double *p99, double *max)
+{
+    if (!s || !p || !median || !p99 || !max)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int n = s->n;
+    const int ns = (num_samples <= 0 || num_samples >= n) ? n : num_samples;
+    std::vector<int> targets(ns);
+    for (int t = 0; t < ns; t++)
+        targets[t] = static_cast<int>((static_cast<long long>(t) * n) / ns);
+    GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
+    std::vector<double> a(static_cast<size_t>(3) * n), ref(static_cast<size_t>(3) * ns);
+    void *dt, *dr;
+    GS_TRY(slot(c, SLOT_IDS, sizeof(int) * ns, &dt));
+    GS_TRY(slot(c, SLOT_V, sizeof(double) * 3 * ns, &dr));
+    GS_HIP(hipMemcpyAsync(dt, targets.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
+    const bool skip = p->method == ACCELERATION_METHOD_MASSLESS ||
+                      p->method == ACCELERATION_METHOD_MASSLESS_FP32;
+    GS_HIP(launch_direct_f64_targets(c, s->x, s->m, n, s->G, p->softening_length, skip,
+                                     static_cast<int *>(dt), ns, static_cast<double *>(dr)));
+    GS_HIP(hipMemcpyAsync(a.data(), s->a, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipMemcpyAsync(ref.data(), dr, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> err;
+    err.reserve(ns);
+    for (int t = 0; t < ns; t++) {
+        const int i = targets[t];
+        double dn = 0.0, rn = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double df = a[3 * i + d] - ref[3 * t + d];
+            dn += df * df;
+            rn += ref[3 * t + d] * ref[3 * t + d];
+        }
+        if (rn > 0.0) err.push_back(sqrt(dn / rn));
+    }
+    if (err.empty()) {
+        *median = *p99 = *max = 0.0;
+        return make_success_error_status();
+    }
+    std::sort(err.begin(), err.end());
+    const size_t k = err.size();
+    *median = k % 2 ? err[k / 2] : 0.5 * (err[k / 2 - 1] + err[k / 2]);
+    *p99 = err[std::min(k - 1, static_cast<size_t>(std::ceil(0.99 * k)) - 1)];
+    *max = err.back();
+    return make_success_error_status();
+}
+
+// ---------------------------------------------------------------- octree extras
+ErrorStatus gs_octree_sort(const System *system, int64 *sorted_keys, int *sorted_idx,
+                           double *center, double *width)
+{
+    if (!system || !sorted_keys || !sorted_idx)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    const int n = system->num_particles;
+    if (n <= 0)
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                    "Number of particles must be positive. Got: %d", n);
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    void *dx;
+    GS_TRY(slot(c, SLOT_X, sizeof(double) * 3 * n, &dx));
+    GS_HIP(hipMemcpyAsync(dx, system->x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    DeviceTree t;
+    GS_TRY(tree_sort(c, static_cast<double *>(dx), n, nullptr, -1.0, &t));
+    GS_HIP(hipMemcpyAsync(sorted_keys, t.skeys, sizeof(int64) * n, hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipMemcpyAsync(sorted_idx, t.sidx, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    if (center)
+        for (int d = 0; d < 3; d++) center[d] = t.center[d];
+    if (width) *width = t.width;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_last_walk_interactions(long long *count)
+{
+    if (!count) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "count is NULL");
+    *count = g_last_walk_interactions;
+    return make_success_error_status();
+}
+
+// ---------------------------------------------------------------- FP32 autotuner
+ErrorStatus gs_set_fp32_config(int block, int ipt, int split)
+{
+    const int shape = block * 16 + ipt;
+    const int ok[] = {256 * 16 + 1, 256 * 16 + 2, 256 * 16 + 4, 128 * 16 + 2,
+                      128 * 16 + 4, 512 * 16 + 1, 512 * 16 + 2};
+    if (std::find(std::begin(ok), std::end(ok), shape) == std::end(ok) || split < 0)
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unsupported FP32 shape (%d, %d, %d)", block,
+                                    ipt, split);
+    Fp32Config &cfg = fp32_config();
+    cfg.block = block;
+    cfg.ipt = ipt;
+    cfg.jsplit = split;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
+{
+    if (n <= 0) return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "n must be positive");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    // synthetic input: a deterministic unit cube lattice jittered by an LCG
+    std::vector<double> hx(static_cast<size_t>(3) * n), hm(n, 1.0 / n);
+    unsigned long long st = 12345;
+    for (auto &v : hx) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        v = static_cast<double>(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    GsSim *s;
+    GS_TRY(gs_sim_create(&s, n, 1.0));
+    ErrorStatus es = gs_sim_upload(s, hx.data(), nullptr, hm.data());
+    if (es.return_code) {
+        gs_sim_destroy(s);
+        return es;
+    }
+    const int shapes[][2] = {{256, 1}, {256, 2}, {256, 4}, {128, 2}, {128, 4}, {512, 1}, {512, 2}};
+    const int splits[] = {0, 1, 2, 4, 8, 16, 32};
+    const Fp32Config saved = fp32_config();
+    Fp32Config best = saved;
+    float best_ms = 1e30f;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    for (auto &sh : shapes)
+        for (int sp : splits) {
+            Fp32Config cfg;
+            cfg.block = sh[0];
+            cfg.ipt = sh[1];
+            cfg.jsplit = sp;
+            const size_t pb = direct_f32_partial_bytes(n, cfg, c->cu_count);
+            void *part;
+            ErrorStatus e2 = slot(c, SLOT_PARTIAL, pb, &part);
+            if (e2.return_code) {
+                free_traceback(&e2);
+                continue;
+            }
+            if (launch_direct_f32(c, s->x, s->m, n, 1.0, 1e-3, s->a, part, pb, cfg)) continue;
+            (void)hipEventRecord(e0, c->stream);
+            for (int r = 0; r < 3; r++)
+                (void)launch_direct_f32(c, s->x, s->m, n, 1.0, 1e-3, s->a, part, pb, cfg);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cfg;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    gs_sim_destroy(s);
+    fp32_config() = best;
+    if (block) *block = best.block;
+    if (ipt) *ipt = best.ipt;
+    if (split) *split = best.jsplit;
+    return make_success_error_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// gs_internal.h -- internal declarations shared by the library's host and
+// device sources.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "grav_sim_hip.h"
+#include "grav_sim_hip_ext.h"
+
+namespace gs {
+
+// ---------------------------------------------------------------- errors
+// Turn a failing HIP call into the library's ErrorStatus convention:
+// allocation failures -> GRAV_MEMORY_ERROR, everything else -> GRAV_FAILURE.
+ErrorStatus hip_error(hipError_t e, const char *what, const char *file, int line,
+                      const char *func);
+
+#define GS_HIP(call)                                                                \
+    do {                                                                            \
+        hipError_t gs_e_ = (call);                                                  \
+        if (gs_e_ != hipSuccess)                                                    \
+            return ::gs::hip_error(gs_e_, #call, __FILE__, __LINE__, __func__);     \
+    } while (0)
+
+#define GS_TRY(call)                                                                \
+    do {                                                                            \
+        ErrorStatus gs_s_ = WRAP_TRACEBACK(call);                                   \
+        if (gs_s_.return_code != GRAV_SUCCESS)                                      \
+            return gs_s_;                                                           \
+    } while (0)
+
+// ---------------------------------------------------------------- device context
+// One context per process (device 0 unless GS_DEVICE is set): a non-blocking
+// stream at default priority, grow-only device buffers addressed by slot, and
+// the kernel-timing registry.  Calls that use the context hold its mutex.
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum Slot : int {
+    // host-entry staging
+    SLOT_X = 0, SLOT_V, SLOT_M, SLOT_A, SLOT_IDS,
+    // direct-sum partials
+    SLOT_PARTIAL,
+    // octree
+    SLOT_BBOX, SLOT_KEYS, SLOT_KEYS_ALT, SLOT_IDX, SLOT_IDX_ALT, SLOT_HIST, SLOT_HIST_SCAN,
+    SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_SORTED_P, SLOT_MISC,
+    // learned correction
+    SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD,
+    SLOT_COUNT
+};
+
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    int clock_khz = 0;
+    char arch[64] = {0};
+    char name[256] = {0};
+    std::recursive_mutex mu;
+    Buffer slots[SLOT_COUNT];
+    // pinned host staging for small readbacks
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+
+    // timing
+    bool timing = false;
+    struct Rec {
+        int kernel;
+        hipEvent_t start, stop;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::string> kernel_names;
+    std::vector<double> kernel_ms;
+    std::vector<long long> kernel_calls;
+};
+
+// Returns the initialised context or an error (no HIP device, init failure).
+ErrorStatus context(Context **out);
+
+// Grow-only device buffer for a slot.
+ErrorStatus slot(Context *c, Slot s, size_t bytes, void **out);
+ErrorStatus pinned(Context *c, size_t bytes, void **out);
+
+// Kernel timing: wrap a launch as
+//   int t = timing_begin(c, "name");  launch...;  timing_end(c, t);
+int timing_begin(Context *c, const char *kernel);
+void timing_end(Context *c, int rec);
+ErrorStatus timing_flush(Context *c);
+
+// ---------------------------------------------------------------- kernels (launchers)
+// Every launcher enqueues on c->stream and returns hipSuccess or the launch error.
+
+// FP64 direct sum, reference summation order (bitwise).  skip_massless: sources
+// with m == 0 are skipped (massless method).
+hipError_t launch_direct_f64(Context *c, const double *x, const double *m, int n, double G,
+                             double eps, bool skip_massless, double *a);
+
+// FP32 tile-relative direct sum with tile-level Kahan compensation.
+struct Fp32Config {
+    int block = 256;     // threads per workgroup
+    int ipt = 2;         // targets per thread
+    int jsplit = 0;      // source chunks (0 = automatic)
+};
+hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n, double G,
+                             double eps, double *a, void *partial_buf, size_t partial_bytes,
+                             const Fp32Config &cfg);
+size_t direct_f32_partial_bytes(int n, const Fp32Config &cfg, int cu_count);
+Fp32Config &fp32_config();
+
+// FP64 energy (src/utils.c:27-59 formula; reduction order differs).
+hipError_t launch_energy(Context *c, const double *x, const double *v, const double *m, int n,
+                         double G, double *partials, int *num_partials);
+
+// Leapfrog updates (src/integrator.c:973-1094 op order).
+hipError_t launch_lf_half_kick(Context *c, double *v, double *ve, const double *a, int n,
+                               double dt);
+hipError_t launch_lf_drift(Context *c, double *x, double *xe, const double *v, int n, double dt);
+hipError_t launch_lf_kick(Context *c, double *v, double *ve, const double *a, int n, double dt);
+hipError_t launch_lf_sync(Context *c, double *v, const double *a, int n, double dt);
+
+// ---------------------------------------------------------------- octree
+struct DeviceTree;  // defined in octree.hip
+
+// Build the tree of (x, m) (device arrays, AoS) into the context's buffers.
+// box_center/box_width: NULL/<=0 -> bounding box.
+ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
+                       const double *box_center, double box_width, DeviceTree **out);
+// Walk; writes a (device, AoS, original order).
+ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, double eps,
+                      double theta, double *a, long long *interactions);
+// Host copy of the tree in the reference struct layout.
+ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out);
+
+// Device-resident acceleration of (x, m) into a (all device pointers).
+ErrorStatus device_acceleration(Context *c, const double *x, const double *m, int n, double G,
+                                const AccelerationParam *p, double *a);
+
+}  // namespace gs

@@ -1,0 +1,45 @@
+// octree.h -- device octree records shared by the tree kernels and hosts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gs {
+
+// One node, 64 bytes, read by the walk with wave-uniform (scalar) loads.
+struct NodeRec {
+    double cx, cy, cz;  // centre of mass
+    double mass;
+    long long prefix;   // Morton prefix at `depth` (3*depth bits)
+    int depth;
+    int count;          // particles in the node
+    int first;          // first particle, sorted position
+    int child0;         // first child (children are contiguous), -1 for leaves
+    int nchild;         // number of children, 0 for leaves
+    int pad;
+};
+static_assert(sizeof(NodeRec) == 64, "NodeRec must stay 64 bytes");
+
+struct DeviceTree {
+    int n = 0;
+    int leaf_cap = 1;
+    int num_nodes = 0;
+    int num_levels = 0;
+    int node_cap = 0;
+    double center[3] = {0, 0, 0};
+    double width = 0.0;
+    int level_begin[MORTON_MAX_LEVEL + 2] = {0};
+    int level_end[MORTON_MAX_LEVEL + 2] = {0};
+    double *box = nullptr;          // device: centre[3], width
+    long long *skeys = nullptr;     // device: sorted keys
+    int *sidx = nullptr;            // device: sorted original indices
+    double4 *P = nullptr;           // device: (x, y, z, m) in sorted order
+    NodeRec *nodes = nullptr;       // device: level-ordered nodes
+    double4 *mom = nullptr;         // device: (M, sum m x, sum m y, sum m z)
+    unsigned *tmp = nullptr;        // device: per-parent child counts
+    unsigned *scan_tmp = nullptr;   // device: scan scratch
+};
+
+ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_center,
+                      double box_width, DeviceTree *t);
+
+}  // namespace gs

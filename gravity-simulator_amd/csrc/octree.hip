@@ -1,0 +1,737 @@
+// octree.hip -- Barnes-Hut linear octree on gfx950.
+//
+// Pipeline (src/linear_octree.c:113-962, src/acceleration_barnes_hut.c:78-248):
+//   k_bbox_*       min/max reduction of x (FP64, order independent -> exact)
+//   k_morton       63-bit keys with the reference's FP64 quantisation
+//                  ((x-c)/w + 0.5, *2^21, truncate, mask) and magic-number
+//                  bit spread; contraction off -> bit-exact keys
+//   radix sort     stable LSD, 8-bit digits, 8 passes (histogram, scan,
+//                  wave-ranked scatter): the same permutation as the
+//                  reference's stable 9-bit LSD sort
+//   build          level-synchronous: children of the internal nodes of one
+//                  depth are found by binary search on the sorted keys,
+//                  counted, scanned and emitted contiguously (level order)
+//   multipoles     bottom-up per level; leaves sum their particles in sorted
+//                  order, internal nodes sum their children (FP64)
+//   k_walk         one wavefront walks for 64 Morton-consecutive targets with
+//                  a shared explicit stack of (node, lane mask): each lane
+//                  applies the reference opening test to its own particle;
+//                  lanes that accept a node add its monopole, the others
+//                  descend.  Geometry and sums in FP64, 1/r^3 through FP32
+//                  rsqrt.
+#include <cstring>
+
+#include "gs_internal.h"
+#include "octree.h"
+#include "scan.h"
+
+#pragma clang fp contract(off)
+
+namespace gs {
+
+constexpr int MAXL = MORTON_MAX_LEVEL;
+
+// ------------------------------------------------------------------ bbox
+constexpr int BB_BLOCK = 256;
+
+__global__ __launch_bounds__(BB_BLOCK) void k_bbox_partial(const double *__restrict__ x, int n,
+                                                           double *__restrict__ part)
+{
+    __shared__ double red[6][BB_BLOCK];
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = x[d];
+        hi[d] = x[d];
+    }
+    for (int i = blockIdx.x * BB_BLOCK + threadIdx.x; i < n; i += gridDim.x * BB_BLOCK)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], x[3 * i + d]);
+            hi[d] = fmax(hi[d], x[3 * i + d]);
+        }
+    for (int d = 0; d < 3; d++) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int s = BB_BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; d++) {
+                red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+                red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[0..2] = centre, out[3] = width (src/linear_octree.c:138-145)
+__global__ void k_bbox_final(const double *__restrict__ part, int nparts, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = part[d];
+        hi[d] = part[3 + d];
+    }
+    for (int p = 1; p < nparts; p++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], part[6 * p + d]);
+            hi[d] = fmax(hi[d], part[6 * p + 3 + d]);
+        }
+    for (int d = 0; d < 3; d++) out[d] = (hi[d] + lo[d]) / 2.0;
+    out[3] = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+}
+
+// ------------------------------------------------------------------ Morton
+__device__ __forceinline__ long long spread21(long long v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffLL;
+    v = (v | v << 16) & 0x1f0000ff0000ffLL;
+    v = (v | v << 8) & 0x100f00f00f00f00fLL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3LL;
+    v = (v | v << 2) & 0x1249249249249249LL;
+    return v;
+}
+
+__device__ __forceinline__ long long quantise(double xv, double c, double w)
+{
+    const double u = (xv - c) / w + 0.5;
+    const double s = u * 2097152.0;
+    // C's double -> int64 conversion truncates toward zero; out-of-range and
+    // NaN inputs (zero-width box) map to 0 here instead of being undefined
+    if (!(s > -9.2e18 && s < 9.2e18)) return 0;
+    return static_cast<long long>(s);
+}
+
+__global__ void k_morton(const double *__restrict__ x, int n, const double *__restrict__ box,
+                         long long *__restrict__ keys, int *__restrict__ idx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double w = box[3];
+    const long long nx = spread21(quantise(x[3 * i + 0], box[0], w));
+    const long long ny = spread21(quantise(x[3 * i + 1], box[1], w));
+    const long long nz = spread21(quantise(x[3 * i + 2], box[2], w));
+    keys[i] = nx | (ny << 1) | (nz << 2);
+    idx[i] = i;
+}
+
+// ------------------------------------------------------------------ radix sort
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_ROUNDS = 8;                          // keys per lane
+constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;         // 2048 keys per tile
+constexpr int RS_BITS = 8;
+constexpr int RS_BINS = 1 << RS_BITS;
+constexpr int RS_PASSES = 8;                          // 64 bits (bit 63 is always 0)
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const long long *__restrict__ keys, int n,
+                                                      int shift, unsigned *__restrict__ hist,
+                                                      int ntiles)
+{
+    __shared__ unsigned h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_ROUNDS; k++) {
+        const int q = base + k * RS_BLOCK + threadIdx.x;
+        if (q < n) atomicAdd(&h[(keys[q] >> shift) & (RS_BINS - 1)], 1u);
+    }
+    __syncthreads();
+    // digit-major layout: all tiles of digit 0, then digit 1, ...
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: wave w owns tile positions [w*512, (w+1)*512), visited in
+// 8 rounds of 64 consecutive keys; within a round, peers with equal digits
+// are found with 8 ballots and ranked by lane order.
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__restrict__ kin,
+                                                         const int *__restrict__ iin, int n,
+                                                         int shift,
+                                                         const unsigned *__restrict__ offs,
+                                                         int ntiles,
+                                                         long long *__restrict__ kout,
+                                                         int *__restrict__ iout)
+{
+    __shared__ unsigned wcount[RS_WAVES][RS_BINS];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int b = threadIdx.x; b < RS_WAVES * RS_BINS; b += RS_BLOCK)
+        (&wcount[0][0])[b] = 0;
+    __syncthreads();
+
+    volatile unsigned *cnt = wcount[wave];
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const int base = blockIdx.x * RS_TILE + wave * (RS_ROUNDS * 64);
+    long long key[RS_ROUNDS];
+    int val[RS_ROUNDS];
+    unsigned rank[RS_ROUNDS];
+    int dig[RS_ROUNDS];
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int q = base + r * 64 + lane;
+        const bool ok = q < n;
+        key[r] = ok ? kin[q] : 0;
+        val[r] = ok ? iin[q] : 0;
+        const int d = ok ? static_cast<int>((key[r] >> shift) & (RS_BINS - 1)) : -1;
+        dig[r] = d;
+        unsigned long long peers = __ballot(ok);
+        for (int b = 0; b < RS_BITS; b++) {
+            const bool bit = ok && ((d >> b) & 1);
+            const unsigned long long bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        if (!ok) peers = 0;
+        const unsigned before = ok ? cnt[d] : 0u;
+        rank[r] = before + static_cast<unsigned>(__popcll(peers & lt_mask));
+        __builtin_amdgcn_wave_barrier();
+        // the lowest lane of each peer group advances the wave's count
+        if (ok && (peers & lt_mask) == 0) cnt[d] = before + static_cast<unsigned>(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        const int d = dig[r];
+        if (d < 0) continue;
+        unsigned off = offs[d * ntiles + blockIdx.x];
+        for (int w = 0; w < wave; w++) off += wcount[w][d];
+        const unsigned dst = off + rank[r];
+        if (dst < static_cast<unsigned>(n)) {
+            kout[dst] = key[r];
+            iout[dst] = val[r];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ gather
+__global__ void k_gather_particles(const double *__restrict__ x, const double *__restrict__ m,
+                                   const int *__restrict__ sidx, int n,
+                                   double4 *__restrict__ P)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = sidx[s];
+    P[s] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], m[i]);
+}
+
+// ------------------------------------------------------------------ build
+// Child octant ranges of a node of depth d (children at depth d+1) by binary
+// search on its sorted key range (src/linear_octree.c:342-393).
+__device__ void octant_bounds(const long long *__restrict__ keys, int first, int count,
+                              int child_depth, int bounds[9])
+{
+    const int shift = 3 * (MAXL - child_depth);
+    int lo = first;
+    const int end = first + count;
+    bounds[0] = first;
+    for (int o = 0; o < 8; o++) {
+        // first position in [lo, end) whose octant digit is > o
+        int a = lo, b = end;
+        while (a < b) {
+            const int mid = (a + b) >> 1;
+            const int oct = static_cast<int>((keys[mid] >> shift) & 7);
+            if (oct <= o) a = mid + 1;
+            else b = mid;
+        }
+        bounds[o + 1] = a;
+        lo = a;
+    }
+}
+
+__device__ __forceinline__ bool node_internal(const NodeRec &nd, int leaf_cap)
+{
+    return nd.depth == 0 || (nd.count > leaf_cap && nd.depth < MAXL);
+}
+
+__global__ void k_count_children(NodeRec *__restrict__ nodes, int lb, int le, int leaf_cap,
+                                 const long long *__restrict__ keys, unsigned *__restrict__ cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= le - lb) return;
+    NodeRec nd = nodes[lb + t];
+    unsigned c = 0;
+    if (node_internal(nd, leaf_cap)) {
+        int b[9];
+        octant_bounds(keys, nd.first, nd.count, nd.depth + 1, b);
+        for (int o = 0; o < 8; o++) c += b[o + 1] > b[o] ? 1u : 0u;
+    }
+    cnt[t] = c;
+}
+
+__global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int leaf_cap,
+                                const long long *__restrict__ keys,
+                                const unsigned *__restrict__ offs, int node_cap)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= le - lb) return;
+    NodeRec &nd = nodes[lb + t];
+    if (!node_internal(nd, leaf_cap)) {
+        nd.nchild = 0;
+        nd.child0 = -1;
+        return;
+    }
+    int b[9];
+    octant_bounds(keys, nd.first, nd.count, nd.depth + 1, b);
+    const int c0 = le + static_cast<int>(offs[t]);
+    int c = c0;
+    for (int o = 0; o < 8; o++) {
+        if (b[o + 1] <= b[o]) continue;
+        NodeRec ch;
+        ch.depth = nd.depth + 1;
+        ch.prefix = nd.prefix * 8 + o;
+        ch.first = b[o];
+        ch.count = b[o + 1] - b[o];
+        ch.nchild = 0;
+        ch.child0 = -1;
+        ch.cx = ch.cy = ch.cz = 0.0;
+        ch.mass = 0.0;
+        if (c < node_cap) nodes[c] = ch;
+        c++;
+    }
+    nd.child0 = c0;
+    nd.nchild = c - c0;
+}
+
+// moments: mom[node] = (sum m, sum m x, sum m y, sum m z)
+__global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
+                            const double4 *__restrict__ P, double4 *__restrict__ mom)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= le - lb) return;
+    const int node = lb + t;
+    NodeRec &nd = nodes[node];
+    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+    if (nd.nchild == 0) {
+        for (int s = nd.first; s < nd.first + nd.count; s++) {
+            const double4 p = P[s];
+            M += p.w;
+            X += p.w * p.x;
+            Y += p.w * p.y;
+            Z += p.w * p.z;
+        }
+    } else {
+        for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
+            const double4 q = mom[c];
+            M += q.x;
+            X += q.y;
+            Y += q.z;
+            Z += q.w;
+        }
+    }
+    mom[node] = make_double4(M, X, Y, Z);
+    nd.mass = M;
+    if (M != 0.0) {
+        nd.cx = X / M;
+        nd.cy = Y / M;
+        nd.cz = Z / M;
+    } else {
+        // massless node: place its (zero) monopole on its first particle
+        const double4 p = P[nd.first];
+        nd.cx = p.x;
+        nd.cy = p.y;
+        nd.cz = p.z;
+    }
+}
+
+// ------------------------------------------------------------------ walk
+constexpr int WK_BLOCK = 256;
+constexpr int WK_WAVES = WK_BLOCK / 64;
+constexpr int WK_STACK = 192;  // >= 7 pending siblings x 21 levels + 8
+
+struct WalkArgs {
+    double w2;  // squared box width
+    double theta2;
+    double eps2;
+    double G;
+};
+
+__device__ __forceinline__ void pp_force(double xi0, double xi1, double xi2, double px, double py,
+                                         double pz, double pm, double eps2, double &a0,
+                                         double &a1, double &a2)
+{
+    const double R0 = xi0 - px, R1 = xi1 - py, R2 = xi2 - pz;
+    const double r2 = R0 * R0 + R1 * R1 + R2 * R2 + eps2;
+    const float ri = __builtin_amdgcn_rsqf(static_cast<float>(r2));
+    const double w = static_cast<double>(static_cast<float>(pm) * (ri * ri * ri));
+    a0 = __builtin_fma(-w, R0, a0);
+    a1 = __builtin_fma(-w, R1, a1);
+    a2 = __builtin_fma(-w, R2, a2);
+}
+
+__global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ nodes,
+                                                   const double4 *__restrict__ P,
+                                                   const long long *__restrict__ skeys,
+                                                   const int *__restrict__ sidx, int n,
+                                                   WalkArgs args, double *__restrict__ a,
+                                                   unsigned long long *__restrict__ n_int)
+{
+    __shared__ int stk_node[WK_WAVES][WK_STACK];
+    __shared__ unsigned long long stk_mask[WK_WAVES][WK_STACK];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int s = (blockIdx.x * WK_WAVES + wave) * 64 + lane;
+    const bool valid = s < n;
+    const unsigned long long all = __ballot(valid);
+    if (all == 0) return;
+
+    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0;
+    long long key = 0;
+    if (valid) {
+        const double4 p = P[s];
+        xi0 = p.x;
+        xi1 = p.y;
+        xi2 = p.z;
+        key = skeys[s];
+    }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    unsigned cnt = 0;
+    const unsigned long long me = 1ull << lane;
+    int *snode = stk_node[wave];
+    unsigned long long *smask = stk_mask[wave];
+
+    int sp = 0;
+    {
+        const int c0 = nodes[0].child0, nc = nodes[0].nchild;
+        for (int c = nc - 1; c >= 0; c--) {
+            if (lane == 0) {
+                snode[sp] = c0 + c;
+                smask[sp] = all;
+            }
+            sp++;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        sp--;
+        const int node = __builtin_amdgcn_readfirstlane(snode[sp]);
+        const unsigned long long M =
+            (static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(
+                 static_cast<int>(smask[sp] >> 32)))
+             << 32) |
+            static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(smask[sp])));
+        __builtin_amdgcn_wave_barrier();
+        const NodeRec nd = nodes[node];
+        const bool act = (M & me) != 0;
+        if (nd.nchild == 0 && nd.count == 1) {
+            // single-particle leaf: its monopole is the particle itself
+            if (act && nd.first != s) {
+                const double4 p = P[nd.first];
+                pp_force(xi0, xi1, xi2, p.x, p.y, p.z, p.w, args.eps2, a0, a1, a2);
+                cnt++;
+            }
+            continue;
+        }
+        bool open = false;
+        if (act) {
+            const bool included = (key >> (3 * (MAXL - nd.depth))) == nd.prefix;
+            const double R0 = xi0 - nd.cx, R1 = xi1 - nd.cy, R2 = xi2 - nd.cz;
+            const double d2 = R0 * R0 + R1 * R1 + R2 * R2;
+            // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
+            const double l2 = __builtin_amdgcn_ldexp(args.w2, -2 * nd.depth);
+            if (!included && l2 < args.theta2 * d2) {
+                const float ri = __builtin_amdgcn_rsqf(static_cast<float>(d2 + args.eps2));
+                const double w = static_cast<double>(static_cast<float>(nd.mass) * (ri * ri * ri));
+                a0 = __builtin_fma(-w, R0, a0);
+                a1 = __builtin_fma(-w, R1, a1);
+                a2 = __builtin_fma(-w, R2, a2);
+                cnt++;
+            } else {
+                open = true;
+            }
+        }
+        const unsigned long long O = __ballot(open);
+        if (O == 0) continue;
+        if (nd.nchild == 0) {
+            const bool mine = (O & me) != 0;
+            for (int t = nd.first; t < nd.first + nd.count; t++) {
+                const double4 p = P[t];
+                if (mine && t != s) {
+                    pp_force(xi0, xi1, xi2, p.x, p.y, p.z, p.w, args.eps2, a0, a1, a2);
+                    cnt++;
+                }
+            }
+        } else if (sp + nd.nchild <= WK_STACK) {
+            for (int c = nd.nchild - 1; c >= 0; c--) {
+                if (lane == 0) {
+                    snode[sp] = nd.child0 + c;
+                    smask[sp] = O;
+                }
+                sp++;
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else if (lane == 0 && n_int) {
+            // cannot happen for depth <= 21 (at most 7 pending siblings per
+            // level); flagged through the counter instead of overflowing
+            atomicAdd(n_int, 1ull << 62);
+        }
+    }
+    if (valid) {
+        const int i = sidx[s];
+        a[3 * i + 0] = args.G * a0;
+        a[3 * i + 1] = args.G * a1;
+        a[3 * i + 2] = args.G * a2;
+    }
+    // interaction count: one atomic per wave
+    unsigned long long tot = cnt;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane == 0 && n_int) atomicAdd(n_int, tot);
+}
+
+// ------------------------------------------------------------------ host side
+static DeviceTree g_tree;
+
+static ErrorStatus ensure_nodes(Context *c, DeviceTree *t, int need)
+{
+    if (need <= t->node_cap) return make_success_error_status();
+    int cap = t->node_cap > 0 ? t->node_cap : 1024;
+    while (cap < need) cap *= 2;
+    GS_HIP(hipStreamSynchronize(c->stream));
+    NodeRec *nn = nullptr;
+    GS_HIP(hipMalloc(reinterpret_cast<void **>(&nn), sizeof(NodeRec) * cap));
+    if (t->nodes) {
+        GS_HIP(hipMemcpy(nn, t->nodes, sizeof(NodeRec) * t->node_cap, hipMemcpyDeviceToDevice));
+        GS_HIP(hipFree(t->nodes));
+    }
+    t->nodes = nn;
+    double4 *nm = nullptr;
+    GS_HIP(hipMalloc(reinterpret_cast<void **>(&nm), sizeof(double4) * cap));
+    if (t->mom) GS_HIP(hipFree(t->mom));
+    t->mom = nm;
+    unsigned *tc = nullptr;
+    GS_HIP(hipMalloc(reinterpret_cast<void **>(&tc), sizeof(unsigned) * (cap + 1)));
+    if (t->tmp) GS_HIP(hipFree(t->tmp));
+    t->tmp = tc;
+    void *st = nullptr;
+    GS_HIP(hipMalloc(&st, scan_temp_bytes(cap) + 64));
+    if (t->scan_tmp) GS_HIP(hipFree(t->scan_tmp));
+    t->scan_tmp = static_cast<unsigned *>(st);
+    t->node_cap = cap;
+    return make_success_error_status();
+}
+
+ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_center,
+                      double box_width, DeviceTree *t)
+{
+    if (n <= 0) return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "Number of particles must be positive");
+    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    if (static_cast<long long>(ntiles) * RS_BINS > scan_max_elements())
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Too many particles for the radix sort: %d", n);
+    void *p;
+    GS_TRY(slot(c, SLOT_BBOX, 64 * sizeof(double) + 6 * 1024 * sizeof(double), &p));
+    double *box = static_cast<double *>(p);
+    double *bpart = box + 64;
+    GS_TRY(slot(c, SLOT_KEYS, sizeof(long long) * n, &p));
+    long long *k0 = static_cast<long long *>(p);
+    GS_TRY(slot(c, SLOT_KEYS_ALT, sizeof(long long) * n, &p));
+    long long *k1 = static_cast<long long *>(p);
+    GS_TRY(slot(c, SLOT_IDX, sizeof(int) * n, &p));
+    int *i0 = static_cast<int *>(p);
+    GS_TRY(slot(c, SLOT_IDX_ALT, sizeof(int) * n, &p));
+    int *i1 = static_cast<int *>(p);
+    const size_t hn = static_cast<size_t>(ntiles) * RS_BINS;
+    GS_TRY(slot(c, SLOT_HIST, sizeof(unsigned) * hn, &p));
+    unsigned *hist = static_cast<unsigned *>(p);
+    GS_TRY(slot(c, SLOT_HIST_SCAN, scan_temp_bytes(hn) + 64, &p));
+    unsigned *stmp = static_cast<unsigned *>(p);
+
+    const int rec = timing_begin(c, "tree_sort");
+    if (box_center && box_width > 0.0) {
+        double hb[4] = {box_center[0], box_center[1], box_center[2], box_width};
+        GS_HIP(hipMemcpyAsync(box, hb, sizeof(hb), hipMemcpyHostToDevice, c->stream));
+    } else {
+        int nb = (n + BB_BLOCK - 1) / BB_BLOCK;
+        if (nb > 1024) nb = 1024;
+        hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(BB_BLOCK), 0, c->stream, x, n, bpart);
+        hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, bpart, nb, box);
+    }
+    hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, n, box, k0,
+                       i0);
+    GS_HIP(hipGetLastError());
+    for (int pass = 0; pass < RS_PASSES; pass++) {
+        const int shift = pass * RS_BITS;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, n, shift,
+                           hist, ntiles);
+        GS_HIP(scan_exclusive(c->stream, hist, hist, static_cast<long long>(hn), stmp, nullptr));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, i0, n,
+                           shift, hist, ntiles, k1, i1);
+        GS_HIP(hipGetLastError());
+        long long *tk = k0;
+        k0 = k1;
+        k1 = tk;
+        int *ti = i0;
+        i0 = i1;
+        i1 = ti;
+    }
+    timing_end(c, rec);
+    // 8 passes: the sorted data is back in the primary slots
+    t->n = n;
+    t->skeys = k0;
+    t->sidx = i0;
+    t->box = box;
+    double hb[4];
+    GS_HIP(hipMemcpyAsync(hb, box, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; d++) t->center[d] = hb[d];
+    t->width = hb[3];
+    return make_success_error_status();
+}
+
+ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
+                       const double *box_center, double box_width, DeviceTree **out)
+{
+    DeviceTree *t = &g_tree;
+    GS_TRY(tree_sort(c, x, n, box_center, box_width, t));
+    t->leaf_cap = leaf_cap;
+    void *p;
+    GS_TRY(slot(c, SLOT_SORTED_P, sizeof(double4) * n, &p));
+    t->P = static_cast<double4 *>(p);
+    hipLaunchKernelGGL(k_gather_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, m,
+                       t->sidx, n, t->P);
+    GS_HIP(hipGetLastError());
+
+    const int rec = timing_begin(c, "tree_build");
+    GS_TRY(ensure_nodes(c, t, 2 * n + 64));
+    NodeRec root;
+    memset(&root, 0, sizeof(root));
+    root.depth = 0;
+    root.prefix = 0;
+    root.first = 0;
+    root.count = n;
+    root.child0 = -1;
+    GS_HIP(hipMemcpyAsync(t->nodes, &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
+    t->level_begin[0] = 0;
+    t->level_end[0] = 1;
+    int levels = 1;
+    unsigned *total_dev = nullptr;
+    GS_TRY(slot(c, SLOT_MISC, 256, &p));
+    total_dev = static_cast<unsigned *>(p);
+    void *hp;
+    GS_TRY(pinned(c, 256, &hp));
+    unsigned *total_host = static_cast<unsigned *>(hp);
+    for (int d = 1; d <= MAXL; d++) {
+        const int lb = t->level_begin[d - 1], le = t->level_end[d - 1];
+        const int np = le - lb;
+        if (np <= 0) break;
+        const int grid = (np + 255) / 256;
+        hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c->stream, t->nodes, lb,
+                           le, leaf_cap, t->skeys, t->tmp);
+        GS_HIP(scan_exclusive(c->stream, t->tmp, t->tmp, np, t->scan_tmp, total_dev));
+        GS_HIP(hipMemcpyAsync(total_host, total_dev, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              c->stream));
+        GS_HIP(hipStreamSynchronize(c->stream));
+        const int nc = static_cast<int>(*total_host);
+        if (le + nc > t->node_cap) {
+            // grow (keeps the node array) and recount into the new scratch
+            GS_TRY(ensure_nodes(c, t, le + nc));
+            hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c->stream, t->nodes,
+                               lb, le, leaf_cap, t->skeys, t->tmp);
+            GS_HIP(scan_exclusive(c->stream, t->tmp, t->tmp, np, t->scan_tmp, total_dev));
+        }
+        hipLaunchKernelGGL(k_emit_children, dim3(grid), dim3(256), 0, c->stream, t->nodes, lb, le,
+                           leaf_cap, t->skeys, t->tmp, t->node_cap);
+        GS_HIP(hipGetLastError());
+        t->level_begin[d] = le;
+        t->level_end[d] = le + nc;
+        levels = d + 1;
+        if (nc == 0) break;
+    }
+    // the deepest emitted level has no children (depth 21 is always a leaf)
+    {
+        const int d = levels - 1;
+        const int lb = t->level_begin[d], le = t->level_end[d];
+        if (le > lb && d == MAXL) {
+            hipLaunchKernelGGL(k_emit_children, dim3((le - lb + 255) / 256), dim3(256), 0,
+                               c->stream, t->nodes, lb, le, leaf_cap, t->skeys, t->tmp,
+                               t->node_cap);
+        }
+    }
+    t->num_levels = levels;
+    t->num_nodes = t->level_end[levels - 1];
+    for (int d = levels - 1; d >= 0; d--) {
+        const int lb = t->level_begin[d], le = t->level_end[d];
+        if (le <= lb) continue;
+        hipLaunchKernelGGL(k_multipole, dim3((le - lb + 255) / 256), dim3(256), 0, c->stream,
+                           t->nodes, lb, le, t->P, t->mom);
+    }
+    GS_HIP(hipGetLastError());
+    timing_end(c, rec);
+    *out = t;
+    return make_success_error_status();
+}
+
+ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, double eps,
+                      double theta, double *a, long long *interactions)
+{
+    (void)x;
+    WalkArgs args;
+    args.w2 = t->width * t->width;
+    args.theta2 = theta * theta;
+    args.eps2 = eps * eps;
+    args.G = G;
+    void *p;
+    GS_TRY(slot(c, SLOT_MISC, 256, &p));
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
+    GS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+    const int waves = (t->n + 63) / 64;
+    const int grid = (waves + WK_WAVES - 1) / WK_WAVES;
+    const int rec = timing_begin(c, "bh_walk");
+    hipLaunchKernelGGL(k_walk, dim3(grid), dim3(WK_BLOCK), 0, c->stream, t->nodes, t->P, t->skeys,
+                       t->sidx, t->n, args, a, cnt);
+    timing_end(c, rec);
+    GS_HIP(hipGetLastError());
+    if (interactions) {
+        unsigned long long h = 0;
+        GS_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        GS_HIP(hipStreamSynchronize(c->stream));
+        *interactions = static_cast<long long>(h);
+    }
+    return make_success_error_status();
+}
+
+ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out)
+{
+    const int n = t->n, nn = t->num_nodes;
+    std::vector<NodeRec> h(nn);
+    GS_HIP(hipMemcpyAsync(h.data(), t->nodes, sizeof(NodeRec) * nn, hipMemcpyDeviceToHost,
+                          c->stream));
+    *out = get_new_linear_octree();
+    out->box_width = t->width;
+    out->num_internal_nodes = nn;
+    out->particle_morton_indices_deepest_level = static_cast<int64 *>(malloc(sizeof(int64) * n));
+    out->sorted_indices = static_cast<int *>(malloc(sizeof(int) * n));
+    out->tree_num_particles = static_cast<int *>(malloc(sizeof(int) * nn));
+    out->tree_num_internal_children = static_cast<int *>(malloc(sizeof(int) * nn));
+    out->tree_first_particle_sorted_idx = static_cast<int *>(malloc(sizeof(int) * nn));
+    out->tree_first_internal_children_idx = static_cast<int *>(malloc(sizeof(int) * nn));
+    out->tree_mass = static_cast<double *>(malloc(sizeof(double) * nn));
+    out->tree_center_of_mass_x = static_cast<double *>(malloc(sizeof(double) * nn));
+    out->tree_center_of_mass_y = static_cast<double *>(malloc(sizeof(double) * nn));
+    out->tree_center_of_mass_z = static_cast<double *>(malloc(sizeof(double) * nn));
+    if (!out->particle_morton_indices_deepest_level || !out->sorted_indices ||
+        !out->tree_num_particles || !out->tree_num_internal_children ||
+        !out->tree_first_particle_sorted_idx || !out->tree_first_internal_children_idx ||
+        !out->tree_mass || !out->tree_center_of_mass_x || !out->tree_center_of_mass_y ||
+        !out->tree_center_of_mass_z) {
+        GS_HIP(hipStreamSynchronize(c->stream));
+        free_linear_octree(out);
+        return WRAP_RAISE_ERROR(GRAV_MEMORY_ERROR, "Failed to allocate memory for the octree");
+    }
+    GS_HIP(hipMemcpyAsync(out->particle_morton_indices_deepest_level, t->skeys, sizeof(int64) * n,
+                          hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipMemcpyAsync(out->sorted_indices, t->sidx, sizeof(int) * n, hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nn; i++) {
+        out->tree_num_particles[i] = h[i].count;
+        out->tree_num_internal_children[i] = h[i].nchild;
+        out->tree_first_particle_sorted_idx[i] = h[i].first;
+        out->tree_first_internal_children_idx[i] = h[i].child0;
+        out->tree_mass[i] = h[i].mass;
+        out->tree_center_of_mass_x[i] = h[i].cx;
+        out->tree_center_of_mass_y[i] = h[i].cy;
+        out->tree_center_of_mass_z[i] = h[i].cz;
+    }
+    return make_success_error_status();
+}
+
+}  // namespace gs

@@ -1,0 +1,303 @@
+"""ctypes binding of libgrav_sim.so (the MI355X force-evaluation library).
+
+Mirrors how the reference's Python layer binds its C library
+(grav_sim/utils.py:12-58: load the shared library, set restypes, call with
+numpy buffers) and raises ``RuntimeError`` on a non-zero return code, as
+grav_sim/simulator.py:117-123 does.  The library computes on the GPU only:
+without a HIP device every compute call fails with GRAV_FAILURE.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import PD, PI, PI64, AccelerationParam, ErrorStatus, LinearOctree, System
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libgrav_sim.so")
+
+# every symbol the C-ABI headers (include/grav_sim_hip.h, include/grav_sim_hip_ext.h) declare
+EXPORTS = [
+    "make_success_error_status", "raise_warning", "raise_warning_fmt", "raise_error", "raise_error_fmt",
+    "traceback", "free_traceback", "print_and_free_traceback", "get_new_settings", "get_new_system",
+    "get_initialized_system", "finalize_system", "free_system", "get_new_output_param",
+    "finalize_output_param", "get_new_acceleration_param", "finalize_acceleration_param", "acceleration",
+    "acceleration_barnes_hut", "benchmark_acceleration", "get_new_linear_octree", "construct_octree",
+    "free_linear_octree", "linear_octree_check_if_included", "get_new_integrator_param",
+    "finalize_integration_param", "integrator_launch_simulation", "launch_simulation", "compute_energy",
+    "grav_get_current_time", "launch_simulation_python", "compute_energy_python",
+    "gs_device_info", "gs_synchronize", "gs_timing_enable", "gs_timing_reset", "gs_timing_query",
+    "gs_sim_create", "gs_sim_destroy", "gs_sim_upload", "gs_sim_download", "gs_sim_acceleration",
+    "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
+    "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
+    "gs_set_fp32_config",
+]
+
+
+class GsDeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("cu_count", C.c_int),
+                ("clock_khz", C.c_int), ("fp32_vector_tflops", C.c_double),
+                ("fp64_vector_tflops", C.c_double), ("hbm_gbs", C.c_double)]
+
+
+_lib = None
+
+
+def load(path=None):
+    """Load and configure libgrav_sim.so (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(p)
+    ES = ErrorStatus
+    pS, pP = C.POINTER(System), C.POINTER(AccelerationParam)
+    sigs = {
+        "acceleration": (ES, [PD, pS, pP]),
+        "acceleration_barnes_hut": (ES, [PD, pS, pP]),
+        "finalize_acceleration_param": (ES, [pP]),
+        "get_new_acceleration_param": (AccelerationParam, []),
+        "get_new_integrator_param": (_abi.IntegratorParam, []),
+        "finalize_integration_param": (ES, [C.POINTER(_abi.IntegratorParam)]),
+        "get_new_output_param": (_abi.OutputParam, []),
+        "get_new_settings": (_abi.Settings, []),
+        "get_new_system": (System, []),
+        "finalize_system": (ES, [pS]),
+        "construct_octree": (ES, [C.POINTER(LinearOctree), pS, pP, PD, C.c_double]),
+        "free_linear_octree": (None, [C.POINTER(LinearOctree)]),
+        "get_new_linear_octree": (LinearOctree, []),
+        "linear_octree_check_if_included": (C.c_bool, [C.c_int64, C.c_int64, C.c_int]),
+        "launch_simulation": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
+                                   C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
+        "compute_energy": (C.c_double, [pS]),
+        "free_traceback": (None, [C.POINTER(ES)]),
+        "gs_device_info": (ES, [C.POINTER(GsDeviceInfo)]),
+        "gs_synchronize": (ES, []),
+        "gs_timing_enable": (ES, [C.c_int]),
+        "gs_timing_reset": (ES, []),
+        "gs_timing_query": (ES, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+        "gs_sim_create": (ES, [C.POINTER(C.c_void_p), C.c_int, C.c_double]),
+        "gs_sim_destroy": (ES, [C.c_void_p]),
+        "gs_sim_upload": (ES, [C.c_void_p, PD, PD, PD]),
+        "gs_sim_download": (ES, [C.c_void_p, PD, PD]),
+        "gs_sim_acceleration": (ES, [C.c_void_p, pP, PD]),
+        "gs_sim_leapfrog_begin": (ES, [C.c_void_p, pP, C.c_double]),
+        "gs_sim_leapfrog_steps": (ES, [C.c_void_p, pP, C.c_double, C.c_int64]),
+        "gs_sim_leapfrog_end": (ES, [C.c_void_p, C.c_double]),
+        "gs_sim_energy": (ES, [C.c_void_p, PD]),
+        "gs_sim_accel_error": (ES, [C.c_void_p, pP, C.c_int, PD, PD, PD]),
+        "gs_octree_sort": (ES, [pS, PI64, PI, PD, PD]),
+        "gs_last_walk_interactions": (ES, [C.POINTER(C.c_longlong)]),
+        "gs_autotune_fp32": (ES, [C.c_int, PI, PI, PI]),
+        "gs_set_fp32_config": (ES, [C.c_int, C.c_int, C.c_int]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def tb_text(st):
+    if not st.traceback:
+        return ""
+    return C.string_at(st.traceback).decode(errors="replace")
+
+
+def check(st, what="call"):
+    """Raise RuntimeError (with the C traceback) for a failed ErrorStatus."""
+    if st.return_code != _abi.GRAV_SUCCESS:
+        text = tb_text(st)
+        load().free_traceback(C.byref(st))
+        raise RuntimeError(f"{what} failed (code {st.return_code}):\n{text}")
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def param(method, theta=1.0, eps=0.0, leaf=-1):
+    return AccelerationParam(int(method), float(theta), float(eps), int(leaf))
+
+
+def acceleration(x, m, method=1, G=1.0, eps=0.0, theta=1.0, leaf=-1):
+    """The drop-in acceleration() on host arrays: returns a (N,3) float64."""
+    L = load()
+    x, m = _c(x), _c(m)
+    v = np.zeros_like(x)
+    s = _abi.make_system(x, v, m, G)
+    p = param(method, theta, eps, leaf)
+    a = np.zeros_like(x)
+    check(L.acceleration(_abi.dptr(a), C.byref(s), C.byref(p)), "acceleration")
+    return a
+
+
+def construct_octree(x, m, leaf=1, box_center=None, box_width=-1.0):
+    """GPU construct_octree; returns a dict of numpy copies of the LinearOctree."""
+    L = load()
+    x, m = _c(x), _c(m)
+    v = np.zeros_like(x)
+    s = _abi.make_system(x, v, m, 1.0)
+    p = param(3, 0.5, 0.0, leaf)
+    t = L.get_new_linear_octree()
+    bc = _abi.dptr(_c(box_center)) if box_center is not None else None
+    check(L.construct_octree(C.byref(t), C.byref(s), C.byref(p), bc, float(box_width)), "construct_octree")
+    n, nn = len(m), t.num_internal_nodes
+
+    def arr(ptr, k, dt):
+        return np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt).copy()
+
+    out = dict(width=t.box_width, num_nodes=nn,
+               sorted_keys=arr(t.particle_morton_indices_deepest_level, n, np.int64),
+               sorted_idx=arr(t.sorted_indices, n, np.int32),
+               count=arr(t.tree_num_particles, nn, np.int32),
+               nchild=arr(t.tree_num_internal_children, nn, np.int32),
+               first=arr(t.tree_first_particle_sorted_idx, nn, np.int32),
+               child0=arr(t.tree_first_internal_children_idx, nn, np.int32),
+               mass=arr(t.tree_mass, nn, np.float64),
+               comx=arr(t.tree_center_of_mass_x, nn, np.float64),
+               comy=arr(t.tree_center_of_mass_y, nn, np.float64),
+               comz=arr(t.tree_center_of_mass_z, nn, np.float64))
+    L.free_linear_octree(C.byref(t))
+    return out
+
+
+def octree_sort(x):
+    L = load()
+    x = _c(x)
+    n = len(x)
+    s = _abi.make_system(x, np.zeros_like(x), np.ones(n), 1.0)
+    keys = np.zeros(n, dtype=np.int64)
+    idx = np.zeros(n, dtype=np.int32)
+    c = np.zeros(3)
+    w = C.c_double()
+    check(L.gs_octree_sort(C.byref(s), keys.ctypes.data_as(PI64), _abi.iptr(idx), _abi.dptr(c), C.byref(w)),
+          "gs_octree_sort")
+    return keys, idx, c, w.value
+
+
+def last_walk_interactions():
+    n = C.c_longlong()
+    check(load().gs_last_walk_interactions(C.byref(n)))
+    return n.value
+
+
+def device_info():
+    info = GsDeviceInfo()
+    check(load().gs_device_info(C.byref(info)), "gs_device_info")
+    return info
+
+
+def synchronize():
+    check(load().gs_synchronize(), "gs_synchronize")
+
+
+def timing_enable(on=True):
+    check(load().gs_timing_enable(int(on)))
+
+
+def timing_reset():
+    check(load().gs_timing_reset())
+
+
+def timing_query(kernel):
+    ms, calls = C.c_double(), C.c_longlong()
+    check(load().gs_timing_query(kernel.encode(), C.byref(ms), C.byref(calls)))
+    return ms.value, calls.value
+
+
+def autotune_fp32(n):
+    b, i, s = C.c_int(), C.c_int(), C.c_int()
+    check(load().gs_autotune_fp32(int(n), C.byref(b), C.byref(i), C.byref(s)), "gs_autotune_fp32")
+    return b.value, i.value, s.value
+
+
+def set_fp32_config(block, ipt, split):
+    check(load().gs_set_fp32_config(int(block), int(ipt), int(split)), "gs_set_fp32_config")
+
+
+class Sim:
+    """Device-resident simulation state (GsSim)."""
+
+    def __init__(self, x, v, m, G=1.0):
+        L = load()
+        self.n = len(m)
+        self.h = C.c_void_p()
+        check(L.gs_sim_create(C.byref(self.h), self.n, float(G)), "gs_sim_create")
+        self.upload(x, v, m)
+
+    def upload(self, x=None, v=None, m=None):
+        f = lambda a: _abi.dptr(_c(a)) if a is not None else None  # noqa: E731
+        keep = [_c(a) for a in (x, v, m) if a is not None]  # noqa: F841
+        check(load().gs_sim_upload(self.h, f(x), f(v), f(m)), "gs_sim_upload")
+
+    def download(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        check(load().gs_sim_download(self.h, _abi.dptr(x), _abi.dptr(v)), "gs_sim_download")
+        return x, v
+
+    def acceleration(self, p, copy=True):
+        a = np.zeros((self.n, 3)) if copy else None
+        check(load().gs_sim_acceleration(self.h, C.byref(p), _abi.dptr(a) if copy else None),
+              "gs_sim_acceleration")
+        return a
+
+    def leapfrog_begin(self, p, dt):
+        check(load().gs_sim_leapfrog_begin(self.h, C.byref(p), float(dt)), "leapfrog_begin")
+
+    def leapfrog_steps(self, p, dt, k):
+        check(load().gs_sim_leapfrog_steps(self.h, C.byref(p), float(dt), int(k)), "leapfrog_steps")
+
+    def leapfrog_end(self, dt):
+        check(load().gs_sim_leapfrog_end(self.h, float(dt)), "leapfrog_end")
+
+    def energy(self):
+        e = C.c_double()
+        check(load().gs_sim_energy(self.h, C.byref(e)), "gs_sim_energy")
+        return e.value
+
+    def accel_error(self, p, samples=0):
+        md, p99, mx = C.c_double(), C.c_double(), C.c_double()
+        check(load().gs_sim_accel_error(self.h, C.byref(p), int(samples), C.byref(md), C.byref(p99),
+                                        C.byref(mx)), "gs_sim_accel_error")
+        return md.value, p99.value, mx.value
+
+    def close(self):
+        if self.h:
+            load().gs_sim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def launch_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, theta=1.0, leaf=-1,
+                      remove_invalid=True):
+    """launch_simulation with output disabled; returns (ids, x, v, m, steps)."""
+    L = load()
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m).copy()
+    ids = np.arange(len(m), dtype=np.int32)
+    s = _abi.make_system(x, v, m, G, ids)
+    ip = L.get_new_integrator_param()
+    ip.integrator = integrator
+    ip.dt = dt
+    ip.whfast_remove_invalid_particles = remove_invalid
+    ap = param(method, theta, eps, leaf)
+    op = L.get_new_output_param()
+    st = _abi.SimulationStatus()
+    se = L.get_new_settings()
+    se.verbose = 0
+    se.enable_progress_bar = False
+    check(L.launch_simulation(C.byref(s), C.byref(ip), C.byref(ap), C.byref(op), C.byref(st), C.byref(se),
+                              float(tf)), "launch_simulation")
+    k = s.num_particles
+    return ids[:k].copy(), x[:k].copy(), v[:k].copy(), m[:k].copy(), st.num_steps

@@ -1,0 +1,343 @@
+/*
+ * grav_sim_hip.h -- C-ABI of the MI355X force-evaluation library (libgrav_sim.so).
+ *
+ * Every type and function in this header keeps the name, layout, argument
+ * meaning and error behaviour of the grav_sim C library it replaces, so that
+ * the existing C integrators and the Python ctypes layer bind to it unchanged.
+ * Each declaration cites the reference declaration it replaces
+ * (paths relative to the grav_sim source tree).
+ *
+ * Additive entry points (new method IDs, device-resident calls, the batched
+ * WHFast ensemble, the learned Barnes-Hut correction, timing) live in
+ * grav_sim_hip_ext.h.
+ *
+ * Usable from C99 and C++.  Plain pointers and sizes only.
+ */
+#ifndef GRAV_SIM_HIP_H
+#define GRAV_SIM_HIP_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define GS_RESTRICT __restrict__
+extern "C" {
+#else
+#define GS_RESTRICT restrict
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Basic types                      (src/common.h:24-27)               */
+/* ------------------------------------------------------------------ */
+typedef int32_t int32;
+typedef int64_t int64;
+typedef uint32_t uint32;
+typedef uint64_t uint64;
+
+/* ------------------------------------------------------------------ */
+/* Error handling                   (src/error.h:14-36)                */
+/* ------------------------------------------------------------------ */
+#define GRAV_UNKNOWN_ERROR -1
+#define GRAV_SUCCESS 0
+#define GRAV_FAILURE 1
+#define GRAV_VALUE_ERROR 2
+#define GRAV_POINTER_ERROR 3
+#define GRAV_MEMORY_ERROR 4
+#define GRAV_OS_ERROR 5
+#define GRAV_NOT_IMPLEMENTED_ERROR 6
+
+#define GRAV_TRACEBACK_NOT_INITIALIZED -1
+#define GRAV_TRACEBACK_SUCCESS 0
+#define GRAV_TRACEBACK_MALLOC_FAILED 1
+#define GRAV_TRACEBACK_TRUNCATED 2
+#define GRAV_TRACEBACK_SNPRINTF_FAILED 3
+
+/* 24 bytes: return_code@0, traceback@8, traceback_code_@16 (src/error.h:31-36) */
+typedef struct ErrorStatus
+{
+    int return_code;
+    char *traceback;
+    int traceback_code_;
+} ErrorStatus;
+
+/* replaces src/error.h:155 */
+ErrorStatus make_success_error_status(void);
+/* replaces src/error.h:165-170 */
+void raise_warning(const char *GS_RESTRICT warning_file, const int warning_line,
+                   const char *GS_RESTRICT warning_func, const char *GS_RESTRICT warning_msg);
+/* replaces src/error.h:187-193 */
+ErrorStatus raise_warning_fmt(const char *GS_RESTRICT warning_file, const int warning_line,
+                              const char *GS_RESTRICT warning_func,
+                              const char *GS_RESTRICT format, ...);
+/* replaces src/error.h:206-212 */
+ErrorStatus raise_error(const char *GS_RESTRICT error_file, const int error_line,
+                        const char *GS_RESTRICT error_func, const int error_code,
+                        const char *GS_RESTRICT error_msg);
+/* replaces src/error.h:226-233 */
+ErrorStatus raise_error_fmt(const char *GS_RESTRICT error_file, const int error_line,
+                            const char *GS_RESTRICT error_func, const int error_code,
+                            const char *GS_RESTRICT format, ...);
+/* replaces src/error.h:246-252 */
+ErrorStatus traceback(ErrorStatus error_status, const char *GS_RESTRICT function_call_source_code,
+                      const char *GS_RESTRICT error_file, const int error_line,
+                      const char *GS_RESTRICT error_func);
+/* replaces src/error.h:257 */
+void free_traceback(ErrorStatus *GS_RESTRICT error_status);
+/* replaces src/error.h:264 */
+void print_and_free_traceback(ErrorStatus *GS_RESTRICT error_status);
+
+#define WRAP_RAISE_WARNING(error_msg) raise_warning(__FILE__, __LINE__, __func__, error_msg)
+#define WRAP_RAISE_WARNING_FMT(format, ...) \
+    raise_warning_fmt(__FILE__, __LINE__, __func__, format, ##__VA_ARGS__)
+#define WRAP_RAISE_ERROR(error_code, error_msg) \
+    raise_error(__FILE__, __LINE__, __func__, error_code, error_msg)
+#define WRAP_RAISE_ERROR_FMT(error_code, format, ...) \
+    raise_error_fmt(__FILE__, __LINE__, __func__, error_code, format, ##__VA_ARGS__)
+#define WRAP_TRACEBACK(function_call) \
+    traceback(function_call, #function_call, __FILE__, __LINE__, __func__)
+
+/* ------------------------------------------------------------------ */
+/* Settings                         (src/settings.h:6-16)              */
+/* ------------------------------------------------------------------ */
+#define GRAV_VERBOSITY_IGNORE_ALL 0
+#define GRAV_VERBOSITY_IGNORE_INFO 1
+#define GRAV_VERBOSITY_NORMAL 2
+#define GRAV_VERBOSITY_VERBOSE 3
+
+typedef struct Settings
+{
+    int verbose;
+    bool enable_progress_bar;
+    bool *is_exit_ptr;
+} Settings;
+
+/* replaces src/settings.h:23 (shared static exit flag, src/settings.c:5-14) */
+Settings get_new_settings(void);
+
+/* ------------------------------------------------------------------ */
+/* System                           (src/system.h:12-20)               */
+/* ------------------------------------------------------------------ */
+/* 48 bytes: num_particles@0, particle_ids@8, x@16, v@24, m@32, G@40.
+ * x and v are AoS double[N][3]; m is double[N]. */
+typedef struct System
+{
+    int num_particles;
+    int *particle_ids;
+    double *x;
+    double *v;
+    double *m;
+    double G;
+} System;
+
+/* replaces src/system.h:45 */
+System get_new_system(void);
+/* replaces src/system.h:55-58 */
+ErrorStatus get_initialized_system(System *GS_RESTRICT system, const int num_particles);
+/* replaces src/system.h (finalize_system; definition src/system.c:70-111) */
+ErrorStatus finalize_system(System *GS_RESTRICT system);
+/* replaces src/system.h (free_system; definition src/system.c:113-119) */
+void free_system(System *GS_RESTRICT system);
+
+/* ------------------------------------------------------------------ */
+/* Integrator / output parameters   (src/common.h:29-56)               */
+/* ------------------------------------------------------------------ */
+/* 40 bytes */
+typedef struct IntegratorParam
+{
+    int integrator;
+    double dt;
+    double tolerance;
+    double initial_dt;
+    bool whfast_remove_invalid_particles;
+} IntegratorParam;
+
+typedef struct OutputParam
+{
+    int method;
+    char *output_dir;
+    bool output_initial;
+    double output_interval;
+    int coordinate_output_dtype;
+    int velocity_output_dtype;
+    int mass_output_dtype;
+
+    int output_count_;
+} OutputParam;
+
+typedef struct SimulationStatus
+{
+    double t;
+    double dt;
+    int64 num_steps;
+} SimulationStatus;
+
+/* Output method and dtype IDs (src/output.h:13-19) */
+#define OUTPUT_METHOD_DISABLED 1
+#define OUTPUT_METHOD_CSV 2
+#define OUTPUT_METHOD_HDF5 3
+#define OUTPUT_DTYPE_FLOAT 1
+#define OUTPUT_DTYPE_DOUBLE 2
+
+/* replaces src/output.h:34 (defaults src/output.c:85-98) */
+OutputParam get_new_output_param(void);
+/* replaces src/output.h:44-47 (src/output.c:126-240).  Snapshot writing is
+ * not part of this library: only OUTPUT_METHOD_DISABLED is accepted by the
+ * integrators; CSV/HDF5 return GRAV_NOT_IMPLEMENTED_ERROR there. */
+ErrorStatus finalize_output_param(OutputParam *GS_RESTRICT output_param,
+                                  const Settings *GS_RESTRICT settings);
+
+/* ------------------------------------------------------------------ */
+/* Acceleration                     (src/acceleration.h:16-26)          */
+/* ------------------------------------------------------------------ */
+#define ACCELERATION_METHOD_PAIRWISE 1
+#define ACCELERATION_METHOD_MASSLESS 2
+#define ACCELERATION_METHOD_BARNES_HUT 3
+
+/* 32 bytes: method@0, opening_angle@8, softening_length@16,
+ * max_num_particles_per_leaf@24 */
+typedef struct AccelerationParam
+{
+    int method;
+    double opening_angle;
+    double softening_length;
+    int max_num_particles_per_leaf;
+} AccelerationParam;
+
+/* replaces src/acceleration.h:33 (defaults src/acceleration.c:62-71) */
+AccelerationParam get_new_acceleration_param(void);
+
+/* replaces src/acceleration.h:41-43 (validation src/acceleration.c:73-129) */
+ErrorStatus finalize_acceleration_param(AccelerationParam *GS_RESTRICT acceleration_param);
+
+/* THE drop-in boundary.  replaces src/acceleration.h:53-57
+ * (dispatcher src/acceleration.c:131-154).  `a` (N x 3 doubles, host memory)
+ * is fully overwritten; `system` and `acceleration_param` are read only.
+ * Methods 1-3 are the reference methods, computed on the GPU; method IDs >= 16
+ * (grav_sim_hip_ext.h) select the additional GPU variants. */
+ErrorStatus acceleration(double *GS_RESTRICT a, const System *GS_RESTRICT system,
+                         const AccelerationParam *GS_RESTRICT acceleration_param);
+
+/* replaces src/acceleration.h:66-70 (src/acceleration_barnes_hut.c:33-76) */
+ErrorStatus acceleration_barnes_hut(double *GS_RESTRICT a, const System *GS_RESTRICT system,
+                                    const AccelerationParam *GS_RESTRICT acceleration_param);
+
+/* replaces src/acceleration.h:98-103 (src/acceleration.c:369-507).  Same
+ * printout; the reference acceleration for the MAE is always computed. */
+ErrorStatus benchmark_acceleration(const System *GS_RESTRICT system,
+                                   const AccelerationParam *acceleration_params,
+                                   const int num_acceleration_params,
+                                   const int *GS_RESTRICT num_times_acceleration_param);
+
+/* ------------------------------------------------------------------ */
+/* Linear octree                    (src/linear_octree.h:14-101)        */
+/* ------------------------------------------------------------------ */
+#define MORTON_MAX_LEVEL 21
+
+/* 96 bytes, SoA (src/linear_octree.h:20-57).  Host-side copy of the tree the
+ * GPU builds.  Node numbering is level order (children of a node are
+ * contiguous, as in the reference); the node SET equals the reference's. */
+typedef struct LinearOctree
+{
+    double box_width;
+    int num_internal_nodes;
+    int64 *particle_morton_indices_deepest_level;
+    int *sorted_indices;
+    int *tree_num_particles;
+    int *tree_num_internal_children;
+    int *tree_first_particle_sorted_idx;
+    int *tree_first_internal_children_idx;
+    double *tree_mass;
+    double *tree_center_of_mass_x;
+    double *tree_center_of_mass_y;
+    double *tree_center_of_mass_z;
+} LinearOctree;
+
+/* replaces src/linear_octree.h:64 (every pointer NULL) */
+LinearOctree get_new_linear_octree(void);
+
+/* replaces src/linear_octree.h:75-81 (src/linear_octree.c:825-962).  Builds
+ * the tree on the GPU and fills `octree` with host copies (malloc'd; release
+ * with free_linear_octree).  Unlike the reference, leaf nodes also carry
+ * their mass and centre of mass. */
+ErrorStatus construct_octree(LinearOctree *GS_RESTRICT octree, const System *GS_RESTRICT system,
+                             const AccelerationParam *GS_RESTRICT acceleration_param,
+                             const double *GS_RESTRICT box_center, const double box_width);
+
+/* replaces src/linear_octree.h:88 */
+void free_linear_octree(LinearOctree *GS_RESTRICT octree);
+
+/* replaces src/linear_octree.h:97-101 */
+bool linear_octree_check_if_included(const int64 morton_index_i, const int64 morton_index_j,
+                                     const int level);
+
+/* ------------------------------------------------------------------ */
+/* Integrators                      (src/integrator.h:17-139)           */
+/* ------------------------------------------------------------------ */
+#define INTEGRATOR_EULER 1
+#define INTEGRATOR_EULER_CROMER 2
+#define INTEGRATOR_RK4 3
+#define INTEGRATOR_LEAPFROG 4
+#define INTEGRATOR_RKF45 5
+#define INTEGRATOR_DOPRI 6
+#define INTEGRATOR_DVERK 7
+#define INTEGRATOR_RKF78 8
+#define INTEGRATOR_IAS15 9
+#define INTEGRATOR_WHFAST 10
+
+/* replaces src/integrator.h:33 (defaults src/integrator.c:119-129) */
+IntegratorParam get_new_integrator_param(void);
+/* replaces src/integrator.h:45 (src/integrator.c:131-189) */
+ErrorStatus finalize_integration_param(IntegratorParam *GS_RESTRICT integration_param);
+
+/* replaces src/integrator.h:62-70 (src/integrator.c:191-279).  LEAPFROG runs
+ * device-resident on the GPU (src/integrator.c:894-1121 semantics, identical
+ * FP64 op order); WHFAST runs the GPU WHFast (src/integrator_whfast.c:200-407
+ * semantics).  The other integrators return GRAV_NOT_IMPLEMENTED_ERROR. */
+ErrorStatus integrator_launch_simulation(System *GS_RESTRICT system,
+                                         IntegratorParam *GS_RESTRICT integrator_param,
+                                         AccelerationParam *GS_RESTRICT acceleration_param,
+                                         OutputParam *GS_RESTRICT output_param,
+                                         SimulationStatus *GS_RESTRICT simulation_status,
+                                         Settings *GS_RESTRICT settings, const double tf);
+
+/* replaces src/grav_sim.h:47-55 (src/grav_sim.c:61-138) */
+ErrorStatus launch_simulation(System *GS_RESTRICT system,
+                              IntegratorParam *GS_RESTRICT integrator_param,
+                              AccelerationParam *GS_RESTRICT acceleration_param,
+                              OutputParam *GS_RESTRICT output_param,
+                              SimulationStatus *GS_RESTRICT simulation_status,
+                              Settings *GS_RESTRICT settings, const double tf);
+
+/* replaces src/utils.h compute_energy (src/utils.c:27-59); FP64 on the GPU */
+double compute_energy(const System *GS_RESTRICT system);
+
+/* replaces src/utils.h grav_get_current_time (src/utils.c:20-25) */
+double grav_get_current_time(void);
+
+/* replaces src/python_interface.h:86-117 (src/python_interface.c:94-193).
+ * Returns 0 on success, 1 on failure (traceback printed to stderr). */
+int launch_simulation_python(int32 *GS_RESTRICT num_particles, int32 *particle_ids, double *x,
+                             double *v, double *m, int32 **new_particle_ids_ptr,
+                             double **new_x_ptr, double **new_v_ptr, double **new_m_ptr,
+                             const double G, const int32 integrator, const double dt,
+                             const double tolerance, const double initial_dt,
+                             const bool whfast_remove_invalid_particles,
+                             const int32 acceleration_method, const double opening_angle,
+                             const double softening_length,
+                             const int32 max_num_particles_per_leaf, const int32 output_method,
+                             char *output_dir, const bool output_initial,
+                             const double output_interval, const int32 coordinate_output_dtype,
+                             const int32 velocity_output_dtype, const int32 mass_output_dtype,
+                             const int32 verbose, const bool enable_progress_bar,
+                             bool *is_exit_ptr, const double tf);
+
+/* replaces src/python_interface.h:128-134 (src/python_interface.c:195-243) */
+void compute_energy_python(double *GS_RESTRICT energy, const double G,
+                           const double *GS_RESTRICT sol_state, const int32 num_snapshots,
+                           const int32 num_particles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAV_SIM_HIP_H */

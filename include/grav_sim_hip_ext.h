@@ -1,0 +1,109 @@
+/*
+ * grav_sim_hip_ext.h -- additive entry points of libgrav_sim.so.
+ *
+ * None of these has a counterpart in the reference library; they extend it
+ * without changing any reference struct.  All return ErrorStatus (see
+ * grav_sim_hip.h) and follow the reference's ownership rules: the caller owns
+ * every buffer it passes; the library owns device memory behind handles.
+ */
+#ifndef GRAV_SIM_HIP_EXT_H
+#define GRAV_SIM_HIP_EXT_H
+
+#include "grav_sim_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- additional acceleration methods (AccelerationParam.method) ----------
+ * IDs >= 16 so the reference IDs 1..3 (src/acceleration.h:16-18) and the
+ * struct layout stay untouched.                                              */
+
+/* Direct sum in FP32 on coordinates relative to each target block's centre,
+ * tile partial sums combined with Kahan compensation, result in FP64.
+ * Same physics as ACCELERATION_METHOD_PAIRWISE (src/acceleration.c:177-234). */
+#define ACCELERATION_METHOD_PAIRWISE_FP32 16
+/* As above, massive sources only (src/acceleration.c:236-367 physics). */
+#define ACCELERATION_METHOD_MASSLESS_FP32 17
+/* Barnes-Hut walk at a coarse opening angle plus a learned residual
+ * correction; falls back to the plain walk at theta = 0.5 when a direct-sum
+ * spot check fails (see gs_learned_*). */
+#define ACCELERATION_METHOD_BARNES_HUT_LEARNED 18
+/* Barnes-Hut with quadrupole moments (same tree and opening criterion as
+ * ACCELERATION_METHOD_BARNES_HUT, multipole order 2). */
+#define ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE 19
+
+/* ---- device information --------------------------------------------------- */
+typedef struct GsDeviceInfo
+{
+    char name[256];
+    char arch[64];
+    int cu_count;
+    int clock_khz;
+    double fp32_vector_tflops; /* published dense vector peak, packed FP32 */
+    double fp64_vector_tflops;
+    double hbm_gbs;
+} GsDeviceInfo;
+
+ErrorStatus gs_device_info(GsDeviceInfo *info);
+
+/* Wait for all work the library has queued. */
+ErrorStatus gs_synchronize(void);
+
+/* ---- kernel timing (HIP events on the library's stream) ------------------ */
+ErrorStatus gs_timing_enable(int on);
+ErrorStatus gs_timing_reset(void);
+/* Total device milliseconds and launch count of the named kernel since the
+ * last reset (synchronises).  Unknown names give 0 / 0. */
+ErrorStatus gs_timing_query(const char *kernel, double *total_ms, long long *calls);
+
+/* ---- device-resident simulation handle ------------------------------------
+ * Holds x, v, m (FP64, AoS) and the integrator state in device memory so
+ * that force evaluations and Leapfrog steps run without host transfers.     */
+typedef struct GsSim GsSim;
+
+ErrorStatus gs_sim_create(GsSim **out, int num_particles, double G);
+ErrorStatus gs_sim_destroy(GsSim *sim);
+ErrorStatus gs_sim_upload(GsSim *sim, const double *x, const double *v, const double *m);
+ErrorStatus gs_sim_download(GsSim *sim, double *x, double *v);
+/* a(x) with the given parameters into device storage; copied to a_host
+ * (N x 3) if not NULL. */
+ErrorStatus gs_sim_acceleration(GsSim *sim, const AccelerationParam *param, double *a_host);
+/* Leapfrog pieces with the reference's operation order
+ * (src/integrator.c:962-1094): begin = a(x0) + compensated half kick;
+ * steps = k x (compensated drift, a(x), compensated kick);
+ * end = velocity synchronisation v -= a dt / 2. */
+ErrorStatus gs_sim_leapfrog_begin(GsSim *sim, const AccelerationParam *param, double dt);
+ErrorStatus gs_sim_leapfrog_steps(GsSim *sim, const AccelerationParam *param, double dt, int64 k);
+ErrorStatus gs_sim_leapfrog_end(GsSim *sim, double dt);
+/* Total energy of the device state (src/utils.c:27-59 formula). */
+ErrorStatus gs_sim_energy(GsSim *sim, double *energy);
+/* Relative acceleration error of `param` against the FP64 direct sum on the
+ * GPU, over `num_samples` targets (evenly strided; all if <= 0 or >= N).
+ * Writes median, 99th percentile and maximum of |a - a64| / |a64|. */
+ErrorStatus gs_sim_accel_error(GsSim *sim, const AccelerationParam *param, int num_samples,
+                               double *median, double *p99, double *max);
+
+/* ---- octree pieces ----------------------------------------------------------
+ * Morton keys (sorted) and the stable sort permutation of system->x, computed
+ * on the GPU (src/linear_octree.c:113-326).  Arrays are N long. */
+ErrorStatus gs_octree_sort(const System *system, int64 *sorted_keys, int *sorted_idx,
+                           double *center, double *width);
+
+/* Number of interactions (accepted nodes + leaf particle pairs) of the last
+ * Barnes-Hut walk. */
+ErrorStatus gs_last_walk_interactions(long long *count);
+
+/* ---- FP32 direct-sum autotuner ---------------------------------------------
+ * Times the candidate (block, targets-per-thread, source-split) shapes on an
+ * N-particle input resident on the device and keeps the fastest for later
+ * calls.  Writes the chosen shape. */
+ErrorStatus gs_autotune_fp32(int num_particles, int *block, int *targets_per_thread,
+                             int *source_split);
+ErrorStatus gs_set_fp32_config(int block, int targets_per_thread, int source_split);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAV_SIM_HIP_EXT_H */

@@ -1,0 +1,860 @@
+/*
+ * oracle.c -- CPU restatement of grav_sim's force-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Build with -ffp-contract=off:
+ * every FP64 expression below is written in the association order of the
+ * reference expression it restates, so results are bitwise reproducible.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAX_LEVEL 21
+
+/* src/math_functions.c:28-31 */
+static double norm3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* ------------------------------------------------------------------ */
+/* Direct sums                                                        */
+/* ------------------------------------------------------------------ */
+
+/* One source term as the reference forms it (src/acceleration.c:206-229):
+ * R = x_l - x_k (the sign flip of the j>k orientation is exact in IEEE),
+ * r = sqrt(((R0^2 + R1^2) + R2^2) + eps^2), t = G / ((r*r)*r),
+ * term = (t * R) * m_l. */
+static void add_term(double acc[3], const double *xk, const double *xl, double ml, double G,
+                     double eps2)
+{
+    const double R0 = xl[0] - xk[0];
+    const double R1 = xl[1] - xk[1];
+    const double R2 = xl[2] - xk[2];
+    const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2 + eps2);
+    const double t = G / (r * r * r);
+    acc[0] += (t * R0) * ml;
+    acc[1] += (t * R1) * ml;
+    acc[2] += (t * R2) * ml;
+}
+
+void orc_pairwise(double *a, int n, const double *x, const double *m, double G, double eps)
+{
+    const double eps2 = eps * eps;
+    for (int k = 0; k < n; k++)
+    {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int l = 0; l < n; l++)
+        {
+            if (l == k)
+                continue;
+            add_term(acc, &x[k * 3], &x[l * 3], m[l], G, eps2);
+        }
+        a[k * 3 + 0] = acc[0];
+        a[k * 3 + 1] = acc[1];
+        a[k * 3 + 2] = acc[2];
+    }
+}
+
+void orc_massless(double *a, int n, const double *x, const double *m, double G, double eps)
+{
+    const double eps2 = eps * eps;
+    for (int k = 0; k < n; k++)
+    {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int l = 0; l < n; l++)
+        {
+            if (l == k || m[l] == 0.0)
+                continue;
+            add_term(acc, &x[k * 3], &x[l * 3], m[l], G, eps2);
+        }
+        a[k * 3 + 0] = acc[0];
+        a[k * 3 + 1] = acc[1];
+        a[k * 3 + 2] = acc[2];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Octree pipeline                                                     */
+/* ------------------------------------------------------------------ */
+
+void orc_bbox(double center[3], double *width, int n, const double *x)
+{
+    double lo[3] = {x[0], x[1], x[2]};
+    double hi[3] = {x[0], x[1], x[2]};
+    for (int i = 1; i < n; i++)
+        for (int d = 0; d < 3; d++)
+        {
+            lo[d] = fmin(lo[d], x[i * 3 + d]);
+            hi[d] = fmax(hi[d], x[i * 3 + d]);
+        }
+    for (int d = 0; d < 3; d++)
+        center[d] = (hi[d] + lo[d]) / 2.0;
+    *width = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+}
+
+void orc_morton(int64_t *keys, int n, const double *x, const double center[3], double width)
+{
+    for (int i = 0; i < n; i++)
+    {
+        int64_t q[3];
+        for (int d = 0; d < 3; d++)
+        {
+            /* src/linear_octree.c:170-179: normalise, scale by 2^21, truncate, mask */
+            const double u = (x[i * 3 + d] - center[d]) / width + 0.5;
+            int64_t t = (int64_t)(u * (double)(1 << 21));
+            q[d] = t & 0x1fffff;
+        }
+        int64_t key = 0;
+        for (int b = 0; b < MAX_LEVEL; b++)
+            for (int d = 0; d < 3; d++)
+                key |= ((q[d] >> b) & 1LL) << (3 * b + d);
+        keys[i] = key;
+    }
+}
+
+void orc_sort(int64_t *keys, int32_t *idx, int n)
+{
+    /* bottom-up merge sort, stable */
+    int64_t *k2 = malloc((size_t)n * sizeof(int64_t));
+    int32_t *i2 = malloc((size_t)n * sizeof(int32_t));
+    for (int i = 0; i < n; i++)
+        idx[i] = i;
+    int64_t *ks = keys, *kd = k2;
+    int32_t *is = idx, *id = i2;
+    for (int w = 1; w < n; w *= 2)
+    {
+        for (int lo = 0; lo < n; lo += 2 * w)
+        {
+            int mid = lo + w < n ? lo + w : n;
+            int hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int p = lo, q = mid, o = lo;
+            while (p < mid && q < hi)
+            {
+                if (ks[q] < ks[p])
+                {
+                    kd[o] = ks[q];
+                    id[o++] = is[q++];
+                }
+                else
+                {
+                    kd[o] = ks[p];
+                    id[o++] = is[p++];
+                }
+            }
+            while (p < mid)
+            {
+                kd[o] = ks[p];
+                id[o++] = is[p++];
+            }
+            while (q < hi)
+            {
+                kd[o] = ks[q];
+                id[o++] = is[q++];
+            }
+        }
+        int64_t *tk = ks;
+        ks = kd;
+        kd = tk;
+        int32_t *ti = is;
+        is = id;
+        id = ti;
+    }
+    if (ks != keys)
+    {
+        memcpy(keys, ks, (size_t)n * sizeof(int64_t));
+        memcpy(idx, is, (size_t)n * sizeof(int32_t));
+    }
+    free(k2);
+    free(i2);
+}
+
+int orc_octree(int max_nodes, int n, const double *x, const double *m,
+               const int64_t *sorted_keys, const int32_t *sorted_idx, int leaf_cap,
+               int *depth, int64_t *prefix, int *count, int *first, int *nchild, int *child0,
+               double *mass, double *mx, double *my, double *mz,
+               double *comx, double *comy, double *comz)
+{
+    if (max_nodes < 1)
+        return -1;
+    /* root */
+    depth[0] = 0;
+    prefix[0] = 0;
+    count[0] = n;
+    first[0] = 0;
+    int num = 1;
+    /* level-order expansion: every node in [level_begin, level_end) is visited
+     * in order, and its children (runs of equal prefix at depth+1) appended */
+    for (int node = 0; node < num; node++)
+    {
+        const int d = depth[node];
+        const int internal = (node == 0) || (count[node] > leaf_cap && d < MAX_LEVEL);
+        nchild[node] = 0;
+        child0[node] = -1;
+        if (!internal)
+            continue;
+        const int cd = d + 1;
+        const int shift = 3 * (MAX_LEVEL - cd);
+        const int s = first[node], e = s + count[node];
+        int i = s;
+        child0[node] = num;
+        while (i < e)
+        {
+            const int64_t p = sorted_keys[i] >> shift;
+            int j = i + 1;
+            while (j < e && (sorted_keys[j] >> shift) == p)
+                j++;
+            if (num >= max_nodes)
+                return -1;
+            depth[num] = cd;
+            prefix[num] = p;
+            count[num] = j - i;
+            first[num] = i;
+            nchild[node]++;
+            num++;
+            i = j;
+        }
+    }
+    /* bottom-up multipoles: level order means children have larger indices */
+    for (int node = num - 1; node >= 0; node--)
+    {
+        double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+        if (nchild[node] == 0)
+        {
+            for (int t = first[node]; t < first[node] + count[node]; t++)
+            {
+                const int p = sorted_idx[t];
+                M += m[p];
+                X += m[p] * x[p * 3 + 0];
+                Y += m[p] * x[p * 3 + 1];
+                Z += m[p] * x[p * 3 + 2];
+            }
+        }
+        else
+        {
+            for (int c = child0[node]; c < child0[node] + nchild[node]; c++)
+            {
+                M += mass[c];
+                X += mx[c];
+                Y += my[c];
+                Z += mz[c];
+            }
+        }
+        mass[node] = M;
+        mx[node] = X;
+        my[node] = Y;
+        mz[node] = Z;
+        comx[node] = X / M;
+        comy[node] = Y / M;
+        comz[node] = Z / M;
+    }
+    return num;
+}
+
+typedef struct
+{
+    const double *x, *m;
+    double G, eps2, theta2, width;
+    const int64_t *keys;
+    const int32_t *sidx;
+    const int *depth, *count, *first, *nchild, *child0;
+    const int64_t *prefix;
+    const double *mass, *comx, *comy, *comz;
+} Walk;
+
+static void walk_node(const Walk *w, int node, int i, int64_t key_i, const double xi[3],
+                      double acc[3], int64_t *nint)
+{
+    const int d = w->depth[node];
+    const int included = (key_i >> (3 * (MAX_LEVEL - d))) == w->prefix[node];
+    if (!included)
+    {
+        /* src/acceleration_barnes_hut.c:152-175 */
+        const double R0 = xi[0] - w->comx[node];
+        const double R1 = xi[1] - w->comy[node];
+        const double R2 = xi[2] - w->comz[node];
+        const double l = (w->width * 2.0) / (double)(2LL << d);
+        const double d2 = R0 * R0 + R1 * R1 + R2 * R2;
+        if ((l * l) < w->theta2 * d2)
+        {
+            const double r = sqrt(d2 + w->eps2);
+            const double t = w->G * w->mass[node] / (r * r * r);
+            acc[0] -= t * R0;
+            acc[1] -= t * R1;
+            acc[2] -= t * R2;
+            (*nint)++;
+            return;
+        }
+    }
+    if (w->nchild[node] == 0)
+    {
+        /* src/acceleration_barnes_hut.c:181-214 */
+        for (int t = w->first[node]; t < w->first[node] + w->count[node]; t++)
+        {
+            const int j = w->sidx[t];
+            if (j == i)
+                continue;
+            const double R0 = xi[0] - w->x[j * 3 + 0];
+            const double R1 = xi[1] - w->x[j * 3 + 1];
+            const double R2 = xi[2] - w->x[j * 3 + 2];
+            const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2 + w->eps2);
+            const double tv = w->G * w->m[j] / (r * r * r);
+            acc[0] -= tv * R0;
+            acc[1] -= tv * R1;
+            acc[2] -= tv * R2;
+            (*nint)++;
+        }
+        return;
+    }
+    for (int c = w->child0[node]; c < w->child0[node] + w->nchild[node]; c++)
+        walk_node(w, c, i, key_i, xi, acc, nint);
+}
+
+void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double *m, double G,
+                 double eps, double theta, double width, const int64_t *sorted_keys,
+                 const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
+                 const int *count, const int *first, const int *nchild, const int *child0,
+                 const double *mass, const double *comx, const double *comy, const double *comz)
+{
+    Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
+              depth, count, first,  nchild,    child0,        prefix, mass,        comx,
+              comy,  comz};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int s = 0; s < n; s++)
+    {
+        const int i = sorted_idx[s];
+        const double xi[3] = {x[i * 3 + 0], x[i * 3 + 1], x[i * 3 + 2]};
+        double acc[3] = {0.0, 0.0, 0.0};
+        int64_t cnt = 0;
+        /* the root is always internal: start from its children */
+        for (int c = child0[0]; c < child0[0] + nchild[0]; c++)
+            walk_node(&w, c, i, sorted_keys[s], xi, acc, &cnt);
+        a[i * 3 + 0] = acc[0];
+        a[i * 3 + 1] = acc[1];
+        a[i * 3 + 2] = acc[2];
+        if (n_int)
+            n_int[i] = cnt;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Leapfrog and energy                                                 */
+/* ------------------------------------------------------------------ */
+
+static void accel(double *a, int n, const double *x, const double *m, double G, int method,
+                  double eps)
+{
+    if (method == 2)
+        orc_massless(a, n, x, m, G, eps);
+    else
+        orc_pairwise(a, n, x, m, G, eps);
+}
+
+int64_t orc_leapfrog(double *x, double *v, int n, const double *m, double G, int method,
+                     double eps, double dt, double tf)
+{
+    const size_t sz = (size_t)n * 3;
+    double *a = malloc(sz * sizeof(double));
+    double *xe = calloc(sz, sizeof(double));
+    double *ve = calloc(sz, sizeof(double));
+
+    /* src/integrator.c:962-982: a(x0), half kick with compensated sum */
+    accel(a, n, x, m, G, method, eps);
+    for (size_t q = 0; q < sz; q++)
+    {
+        const double v0 = v[q];
+        ve[q] += 0.5 * a[q] * dt;
+        v[q] = v0 + ve[q];
+        ve[q] += v0 - v[q];
+    }
+    /* src/integrator.c:985-1085 */
+    const int64_t total = (int64_t)ceil(tf / dt);
+    double t = 0.0;
+    int64_t steps = 0;
+    while (steps < total)
+    {
+        if (t + dt > tf)
+            dt = tf - t;
+        for (size_t q = 0; q < sz; q++)
+        {
+            const double x0 = x[q];
+            xe[q] += v[q] * dt;
+            x[q] = x0 + xe[q];
+            xe[q] += x0 - x[q];
+        }
+        accel(a, n, x, m, G, method, eps);
+        for (size_t q = 0; q < sz; q++)
+        {
+            const double v0 = v[q];
+            ve[q] += a[q] * dt;
+            v[q] = v0 + ve[q];
+            ve[q] += v0 - v[q];
+        }
+        steps++;
+        t = steps * dt;
+    }
+    /* src/integrator.c:1087-1094 */
+    for (size_t q = 0; q < sz; q++)
+        v[q] -= 0.5 * a[q] * dt;
+    free(a);
+    free(xe);
+    free(ve);
+    return steps;
+}
+
+double orc_energy(int n, const double *x, const double *v, const double *m, double G)
+{
+    double e = 0.0;
+    for (int i = 0; i < n; i++)
+    {
+        const double vn = norm3(&v[i * 3]);
+        e += (0.5 * m[i] * vn * vn);
+        for (int j = i + 1; j < n; j++)
+        {
+            const double r[3] = {x[i * 3 + 0] - x[j * 3 + 0], x[i * 3 + 1] - x[j * 3 + 1],
+                                 x[i * 3 + 2] - x[j * 3 + 2]};
+            e -= (G * m[i] * m[j] / norm3(r));
+        }
+    }
+    return e;
+}
+
+/* ------------------------------------------------------------------ */
+/* WHFast                                                              */
+/* ------------------------------------------------------------------ */
+
+void orc_stumpff(double z, double c[4])
+{
+    /* src/integrator_whfast.c:782-814 */
+    int k = 0;
+    while (fabs(z) > 0.1)
+    {
+        z /= 4.0;
+        k++;
+    }
+    double c3 = (1.0 - z / 20.0 *
+                           (1.0 - z / 42.0 *
+                                      (1.0 - z / 72.0 *
+                                                 (1.0 - z / 110.0 *
+                                                            (1.0 - z / 156.0 * (1.0 - z / 210.0)))))) /
+                6.0;
+    double c2 = (1.0 - z / 12.0 *
+                           (1.0 - z / 30.0 *
+                                      (1.0 - z / 56.0 *
+                                                 (1.0 - z / 90.0 *
+                                                            (1.0 - z / 132.0 * (1.0 - z / 182.0)))))) /
+                2.0;
+    double c1 = 1.0 - z * c3;
+    double c0 = 1.0 - z * c2;
+    while (k > 0)
+    {
+        c3 = (c2 + c0 * c3) / 4.0;
+        c2 = (c1 * c1) / 2.0;
+        c1 = c0 * c1;
+        c0 = (2.0 * c0 * c0) - 1.0;
+        k--;
+    }
+    c[0] = c0;
+    c[1] = c1;
+    c[2] = c2;
+    c[3] = c3;
+}
+
+typedef struct
+{
+    double d;
+    int i;
+} DistIdx;
+
+static int cmp_dist(const void *pa, const void *pb)
+{
+    const DistIdx *a = pa, *b = pb;
+    if (a->d < b->d)
+        return -1;
+    if (a->d > b->d)
+        return 1;
+    return (a->i > b->i) - (a->i < b->i); /* stable for ties */
+}
+
+/* src/system.c:1210-1335 applied to (pos, vel, m, ids); distance from the
+ * particle whose id is 0, which is forced to distance 0. */
+static void sort_by_distance(int n, int32_t *ids, double *pos, double *vel, double *m)
+{
+    int primary = -1;
+    if (0 < n && ids[0] == 0)
+        primary = 0;
+    else
+        for (int i = 0; i < n; i++)
+            if (ids[i] == 0)
+            {
+                primary = i;
+                break;
+            }
+    if (primary < 0)
+        return;
+    DistIdx *h = malloc((size_t)n * sizeof(DistIdx));
+    for (int i = 0; i < n; i++)
+    {
+        const double dv[3] = {pos[i * 3 + 0] - pos[primary * 3 + 0],
+                              pos[i * 3 + 1] - pos[primary * 3 + 1],
+                              pos[i * 3 + 2] - pos[primary * 3 + 2]};
+        h[i].d = norm3(dv);
+        h[i].i = i;
+    }
+    h[primary].d = 0.0;
+    qsort(h, (size_t)n, sizeof(DistIdx), cmp_dist);
+    int32_t *ni = malloc((size_t)n * sizeof(int32_t));
+    double *np = malloc((size_t)n * 3 * sizeof(double));
+    double *nv = malloc((size_t)n * 3 * sizeof(double));
+    double *nm = malloc((size_t)n * sizeof(double));
+    for (int i = 0; i < n; i++)
+    {
+        const int s = h[i].i;
+        ni[i] = ids[s];
+        nm[i] = m[s];
+        for (int d = 0; d < 3; d++)
+        {
+            np[i * 3 + d] = pos[s * 3 + d];
+            nv[i * 3 + d] = vel[s * 3 + d];
+        }
+    }
+    memcpy(ids, ni, (size_t)n * sizeof(int32_t));
+    memcpy(pos, np, (size_t)n * 3 * sizeof(double));
+    memcpy(vel, nv, (size_t)n * 3 * sizeof(double));
+    memcpy(m, nm, (size_t)n * sizeof(double));
+    free(h);
+    free(ni);
+    free(np);
+    free(nv);
+    free(nm);
+}
+
+static void compute_eta(double *eta, int n, const double *m)
+{
+    /* src/integrator_whfast.c:1266-1279 */
+    eta[0] = m[0];
+    for (int i = 1; i < n; i++)
+        eta[i] = eta[i - 1] + m[i];
+}
+
+static void cart_to_jacobi(double *jx, double *jv, int n, const double *x, const double *v,
+                           const double *m, const double *eta)
+{
+    /* src/integrator_whfast.c:682-724 */
+    double xc[3], vc[3];
+    for (int d = 0; d < 3; d++)
+    {
+        xc[d] = m[0] * x[d];
+        vc[d] = m[0] * v[d];
+    }
+    for (int i = 1; i < n; i++)
+        for (int d = 0; d < 3; d++)
+        {
+            jx[i * 3 + d] = x[i * 3 + d] - xc[d] / eta[i - 1];
+            jv[i * 3 + d] = v[i * 3 + d] - vc[d] / eta[i - 1];
+            xc[d] = xc[d] * (1.0 + m[i] / eta[i - 1]) + m[i] * jx[i * 3 + d];
+            vc[d] = vc[d] * (1.0 + m[i] / eta[i - 1]) + m[i] * jv[i * 3 + d];
+        }
+    for (int d = 0; d < 3; d++)
+    {
+        jx[d] = xc[d] / eta[n - 1];
+        jv[d] = vc[d] / eta[n - 1];
+    }
+}
+
+static void jacobi_to_cart(double *x, double *v, int n, const double *jx, const double *jv,
+                           const double *m, const double *eta)
+{
+    /* src/integrator_whfast.c:726-772 */
+    double xc[3], vc[3];
+    for (int d = 0; d < 3; d++)
+    {
+        xc[d] = eta[n - 1] * jx[d];
+        vc[d] = eta[n - 1] * jv[d];
+    }
+    for (int i = n - 1; i > 0; i--)
+        for (int d = 0; d < 3; d++)
+        {
+            xc[d] = (xc[d] - m[i] * jx[i * 3 + d]) / eta[i];
+            vc[d] = (vc[d] - m[i] * jv[i * 3 + d]) / eta[i];
+            x[i * 3 + d] = jx[i * 3 + d] + xc[d];
+            v[i * 3 + d] = jv[i * 3 + d] + vc[d];
+            xc[d] = eta[i - 1] * xc[d];
+            vc[d] = eta[i - 1] * vc[d];
+        }
+    for (int d = 0; d < 3; d++)
+    {
+        x[d] = xc[d] / m[0];
+        v[d] = vc[d] / m[0];
+    }
+}
+
+/* Kepler drift of one Jacobi particle (src/integrator_whfast.c:459-578).
+ * Returns 1 if the particle must be removed. */
+static int drift_one(double *xo, double *vo, double gm, double dt)
+{
+    const double x[3] = {xo[0], xo[1], xo[2]};
+    const double v[3] = {vo[0], vo[1], vo[2]};
+    const double xn = norm3(x);
+    const double vn = norm3(v);
+    const double rv = (x[0] * v[0] + x[1] * v[1] + x[2] * v[2]) / xn;
+    const double alpha = 2.0 * gm / xn - (vn * vn);
+    double s = dt / xn;
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    int converged = 0, zbad = 0;
+    for (int it = 0; it < 500; it++)
+    {
+        const double z = alpha * (s * s);
+        if (!isfinite(z))
+        {
+            zbad = 1;
+            break;
+        }
+        orc_stumpff(z, c);
+        const double F = (xn * s * c[1] + xn * rv * (s * s) * c[2] + gm * (s * s * s) * c[3] - dt);
+        const double dF = (xn * c[0] + xn * rv * s * c[1] + gm * (s * s) * c[2]);
+        const double ds = -F / dF;
+        s += ds;
+        if (fabs(ds) < 1e-12)
+        {
+            converged = 1;
+            break;
+        }
+    }
+    const double r = xn * c[0] + xn * rv * s * c[1] + gm * (s * s) * c[2];
+    int remove = 0;
+    if (!converged)
+    {
+        const double err = (xn * s * c[1] + xn * rv * (s * s) * c[2] + gm * (s * s * s) * c[3] - dt) / r;
+        if (err > 1e-5 || zbad)
+            remove = 1;
+    }
+    const double f = 1.0 - gm * (s * s) * c[2] / xn;
+    const double g = dt - gm * (s * s * s) * c[3];
+    const double df = -gm * s * c[1] / (r * xn);
+    const double dg = 1.0 - gm * (s * s) * c[2] / r;
+    for (int d = 0; d < 3; d++)
+    {
+        xo[d] = f * x[d] + g * v[d];
+        vo[d] = df * x[d] + dg * v[d];
+    }
+    return remove;
+}
+
+/* One interaction term of the WHFast accelerations: G*m*(R)/(|R|^3 + eps^3). */
+static double cube_plus(const double *tv, double eps3)
+{
+    const double nrm = norm3(tv);
+    return (nrm * nrm * nrm) + eps3;
+}
+
+static void whfast_accel(double *a, int n, int method, const double *x, const double *jx,
+                         const double *m, const double *eta, double G, double eps)
+{
+    const double eps3 = eps * eps * eps;
+    int *mi = malloc((size_t)n * sizeof(int));
+    int *li = malloc((size_t)n * sizeof(int));
+    int nm = 0, nl = 0;
+    for (int i = 0; i < n; i++)
+    {
+        if (method == 1 || m[i] != 0.0)
+            mi[nm++] = i;
+        else
+            li[nl++] = i;
+    }
+    a[0] = a[1] = a[2] = 0.0; /* particle 0 is never written by the reference */
+    /* massive (or all, for the pairwise form): src/integrator_whfast.c:859-954, 1022-1127 */
+    for (int p = 1; p < nm; p++)
+    {
+        const int i = mi[p];
+        double tv[3], aux[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; d++)
+            tv[d] = x[i * 3 + d] - x[d];
+        const double c0 = cube_plus(tv, eps3);
+        const double cj = cube_plus(&jx[i * 3], eps3);
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] = G * m[0] * eta[i] / eta[i - 1] * (jx[i * 3 + d] / cj - tv[d] / c0);
+        for (int q = 1; q < p; q++)
+        {
+            const int j = mi[q];
+            for (int d = 0; d < 3; d++)
+                tv[d] = x[i * 3 + d] - x[j * 3 + d];
+            const double cc = cube_plus(tv, eps3);
+            for (int d = 0; d < 3; d++)
+                aux[d] += G * m[j] * tv[d] / cc;
+        }
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] -= aux[d] * eta[i] / eta[i - 1];
+        aux[0] = aux[1] = aux[2] = 0.0;
+        for (int q = p + 1; q < nm; q++)
+        {
+            const int j = mi[q];
+            for (int d = 0; d < 3; d++)
+                tv[d] = x[j * 3 + d] - x[i * 3 + d];
+            const double cc = cube_plus(tv, eps3);
+            for (int d = 0; d < 3; d++)
+                aux[d] += G * m[j] * tv[d] / cc;
+        }
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] += aux[d];
+        aux[0] = aux[1] = aux[2] = 0.0;
+        for (int q = 0; q < p; q++)
+        {
+            const int j = mi[q];
+            for (int r = p + 1; r < nm; r++)
+            {
+                const int k = mi[r];
+                for (int d = 0; d < 3; d++)
+                    tv[d] = x[k * 3 + d] - x[j * 3 + d];
+                const double cc = cube_plus(tv, eps3);
+                for (int d = 0; d < 3; d++)
+                    aux[d] += G * m[j] * m[k] * tv[d] / cc;
+            }
+        }
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] -= aux[d] / eta[i - 1];
+    }
+    /* massless: src/integrator_whfast.c:1133-1258 */
+    for (int p = 0; p < nl; p++)
+    {
+        const int i = li[p];
+        if (i == 0)
+            continue;
+        double tv[3], aux[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; d++)
+            tv[d] = x[i * 3 + d] - x[d];
+        const double c0 = cube_plus(tv, eps3);
+        const double cj = cube_plus(&jx[i * 3], eps3);
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] = G * m[0] * (jx[i * 3 + d] / cj - tv[d] / c0);
+        for (int q = 1; q < nm; q++)
+        {
+            const int j = mi[q];
+            if (j >= i)
+                break;
+            for (int d = 0; d < 3; d++)
+                tv[d] = x[i * 3 + d] - x[j * 3 + d];
+            const double cc = cube_plus(tv, eps3);
+            for (int d = 0; d < 3; d++)
+                aux[d] += G * m[j] * tv[d] / cc;
+        }
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] -= aux[d];
+        aux[0] = aux[1] = aux[2] = 0.0;
+        for (int q = 1; q < nm; q++)
+        {
+            const int j = mi[q];
+            if (j <= i)
+                continue;
+            for (int d = 0; d < 3; d++)
+                tv[d] = x[j * 3 + d] - x[i * 3 + d];
+            const double cc = cube_plus(tv, eps3);
+            for (int d = 0; d < 3; d++)
+                aux[d] += G * m[j] * tv[d] / cc;
+        }
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] += aux[d];
+        aux[0] = aux[1] = aux[2] = 0.0;
+        for (int q = 0; q < nm; q++)
+        {
+            const int j = mi[q];
+            if (j >= i)
+                break;
+            for (int r = q + 1; r < nm; r++)
+            {
+                const int k = mi[r];
+                if (k <= i)
+                    continue;
+                for (int d = 0; d < 3; d++)
+                    tv[d] = x[k * 3 + d] - x[j * 3 + d];
+                const double cc = cube_plus(tv, eps3);
+                for (int d = 0; d < 3; d++)
+                    aux[d] += G * m[j] * m[k] * tv[d] / cc;
+            }
+        }
+        for (int d = 0; d < 3; d++)
+            a[i * 3 + d] -= aux[d] / eta[i - 1];
+    }
+    free(mi);
+    free(li);
+}
+
+/* compaction in ascending index order (src/system.c:444-528, 530-577) */
+static int compact(int n, const int *rm, int nrm, int32_t *ids, double *x, double *v, double *m,
+                   double *jx, double *jv)
+{
+    int o = 0, r = 0;
+    for (int i = 0; i < n; i++)
+    {
+        if (r < nrm && rm[r] == i)
+        {
+            r++;
+            continue;
+        }
+        ids[o] = ids[i];
+        m[o] = m[i];
+        for (int d = 0; d < 3; d++)
+        {
+            x[o * 3 + d] = x[i * 3 + d];
+            v[o * 3 + d] = v[i * 3 + d];
+            jx[o * 3 + d] = jx[i * 3 + d];
+            jv[o * 3 + d] = jv[i * 3 + d];
+        }
+        o++;
+    }
+    return o;
+}
+
+int orc_whfast(int *np, int32_t *ids, double *x, double *v, double *m, double G, int method,
+               double eps, double dt, double tf, int remove_invalid)
+{
+    int n = *np;
+    const size_t sz = (size_t)n * 3;
+    double *jx = calloc(sz, sizeof(double));
+    double *jv = malloc(sz * sizeof(double));
+    double *a = calloc(sz, sizeof(double));
+    double *eta = malloc((size_t)n * sizeof(double));
+    int *rm = malloc((size_t)n * sizeof(int));
+
+    sort_by_distance(n, ids, x, v, m);
+    compute_eta(eta, n, m);
+    cart_to_jacobi(jx, jv, n, x, v, m, eta);
+    whfast_accel(a, n, method, x, jx, m, eta, G, eps);
+    for (int q = 0; q < 3 * n; q++)
+        jv[q] += a[q] * (0.5 * dt);
+
+    const int64_t total = (int64_t)ceil(tf / dt);
+    double t = 0.0;
+    int64_t steps = 0;
+    while (steps < total)
+    {
+        if (t + dt > tf)
+            dt = tf - t;
+        sort_by_distance(n, ids, jx, jv, m);
+        compute_eta(eta, n, m);
+        int nrm = 0;
+        for (int i = 1; i < n; i++)
+        {
+            const double gm = G * m[0] * eta[i] / eta[i - 1];
+            if (drift_one(&jx[i * 3], &jv[i * 3], gm, dt) && remove_invalid)
+                rm[nrm++] = i;
+        }
+        if (nrm > 0)
+        {
+            n = compact(n, rm, nrm, ids, x, v, m, jx, jv);
+            compute_eta(eta, n, m);
+        }
+        jacobi_to_cart(x, v, n, jx, jv, m, eta);
+        whfast_accel(a, n, method, x, jx, m, eta, G, eps);
+        for (int q = 0; q < 3 * n; q++)
+            jv[q] += a[q] * dt;
+        steps++;
+        t = steps * dt;
+    }
+    *np = n;
+    free(jx);
+    free(jv);
+    free(a);
+    free(eta);
+    free(rm);
+    return 0;
+}

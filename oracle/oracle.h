@@ -1,0 +1,109 @@
+/*
+ * oracle.h -- CPU restatement of grav_sim's force-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the GPU
+ * library: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it.  Nothing in the product links or calls it.
+ *
+ * Every function names the reference code (paths relative to the grav_sim
+ * source tree) whose behaviour it restates.  Results are FP64 and, where the
+ * reference fixes an operation order, follow that order exactly (built with
+ * -ffp-contract=off), so they are bitwise comparable with the reference
+ * build and with the GPU's FP64 kernels.
+ *
+ * Parity status of each function (how it is pinned) is listed in the
+ * docstring of oracle/oracle.py, which wraps this library for the tests.
+ */
+#ifndef GS_ORACLE_H
+#define GS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Direct pairwise accelerations (src/acceleration.c:177-234), restated per
+ * target in the summation order the reference's i<j loop produces for every
+ * particle: sources l = 0..N-1 ascending, l != k, each term
+ * fl(fl(G / fl(fl(r*r)*r) * fl(x_l - x_k)) * m_l). */
+void orc_pairwise(double *a, int n, const double *x, const double *m, double G, double eps);
+
+/* Massive + massless accelerations (src/acceleration.c:236-367) with the
+ * physically correct source mass m[idx_i] (the reference reads m[i], the
+ * ordinal among massive particles; both agree when massive come first).
+ * Targets: every particle; sources: massive particles in index order. */
+void orc_massless(double *a, int n, const double *x, const double *m, double G, double eps);
+
+/* Bounding box (src/linear_octree.c:113-146). */
+void orc_bbox(double center[3], double *width, int n, const double *x);
+
+/* 63-bit Morton keys (src/linear_octree.c:159-202): same FP64 quantisation,
+ * bits interleaved one at a time instead of by magic numbers. */
+void orc_morton(int64_t *keys, int n, const double *x, const double center[3], double width);
+
+/* Stable sort of keys with the identity payload (src/linear_octree.c:216-326,
+ * :913-916).  Merge sort: any stable sort yields the reference permutation. */
+void orc_sort(int64_t *keys, int32_t *idx, int n);
+
+/* Octree node set (src/linear_octree.c:342-823).  Nodes in level order
+ * (depth, prefix ascending): node 0 is the root.  A node at depth d >= 1 exists
+ * for every non-empty octant of an internal parent; a node is internal iff it
+ * is the root, or holds more than leaf_cap particles and d < 21.
+ * Returns the node count, or -1 if max_nodes is too small.
+ * Multipoles: leaf = sequential sums over its particles in sorted order;
+ * internal = sequential sums over its children; com = (sum m x) / M. */
+int orc_octree(int max_nodes, int n, const double *x, const double *m,
+               const int64_t *sorted_keys, const int32_t *sorted_idx, int leaf_cap,
+               int *depth, int64_t *prefix, int *count, int *first, int *nchild, int *child0,
+               double *mass, double *mx, double *my, double *mz,
+               double *comx, double *comy, double *comz);
+
+/* Barnes-Hut walk (src/acceleration_barnes_hut.c:78-248) with the reference's
+ * quirks fixed: leaf nodes carry their own mass and centre of mass, and the
+ * "target inside node" test compares the target's own key with the node's
+ * prefix.  Per target, depth-first over children in octant order:
+ *   included  = (key_i >> 3(21-d)) == prefix
+ *   accept    = !included && (w/2^d)^2 < theta^2 * |x_i - com|^2
+ *   accept    -> a -= G M R / (|R|^2 + eps^2)^{3/2}
+ *   leaf      -> direct sum over its particles, skipping i
+ *   otherwise -> descend.
+ * Writes a[idx*3..] for every particle and, if n_int != NULL, the number of
+ * accepted nodes + leaf-particle interactions per particle. */
+void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double *m, double G,
+                 double eps, double theta, double width, const int64_t *sorted_keys,
+                 const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
+                 const int *count, const int *first, const int *nchild, const int *child0,
+                 const double *mass, const double *comx, const double *comy, const double *comz);
+
+/* Leapfrog KDK with compensated summation (src/integrator.c:894-1121), using
+ * orc_pairwise (method 1) or orc_massless (method 2) for the accelerations.
+ * Runs ceil(tf/dt) steps with the reference's last-step clamp and final
+ * velocity synchronisation.  x, v updated in place. Returns steps taken. */
+int64_t orc_leapfrog(double *x, double *v, int n, const double *m, double G, int method,
+                     double eps, double dt, double tf);
+
+/* Total energy (src/utils.c:27-59): KE via vec_norm_3d squared, PE over i<j
+ * without softening, one running sum in the reference order. */
+double orc_energy(int n, const double *x, const double *v, const double *m, double G);
+
+/* ---- WHFast (src/integrator_whfast.c) ---- */
+
+/* Stumpff functions c0..c3 (src/integrator_whfast.c:774-815). */
+void orc_stumpff(double z, double c[4]);
+
+/* Full WHFast run of one system (src/integrator_whfast.c:200-407):
+ * initial sort by distance, per-step sort-and-permute of the Jacobi arrays
+ * (stable order for ties), Kepler drift with removal of invalid particles in
+ * ascending index order, Jacobi<->Cartesian, massless-form accelerations with
+ * the auxiliary sums zero-initialised.  State arrays are updated in place;
+ * *n is updated when particles are removed.  method: 1 pairwise, 2 massless.
+ * Returns 0, or 1 on an invalid Stumpff argument when removal is disabled. */
+int orc_whfast(int *n, int32_t *ids, double *x, double *v, double *m, double G, int method,
+               double eps, double dt, double tf, int remove_invalid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,113 @@
+"""Write the golden fixtures from the reference C library.
+
+Run in a container that holds the reference sources:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+The reference library (oracle/_ref/libgrav_sim_ref.so) is compiled from the
+reference's own sources by oracle/Makefile; this script only calls its
+exported functions on seeded inputs and stores inputs and outputs as data:
+
+  f1_pairwise.npz   N=1024 Plummer (seed 1): reference acceleration_pairwise,
+                    eps=0 and eps=1e-3 (src/acceleration.c:177-234)
+  f2_massless.npz   one planetary system (massive first) with test particles:
+                    reference acceleration_massless (src/acceleration.c:236-367)
+  f3_morton.npz     N=4096 two-sphere (seed 4): reference Morton keys (sorted)
+                    and sorted_indices (src/linear_octree.c:159-326), plus
+                    SHA-256 of keys/indices for the N=2^20 (seed 4) and N=2^21
+                    (seed 5) two-sphere inputs
+  f4_octree.npz     N=4096 two-sphere internal-node set and monopoles for leaf
+                    capacity 1 and 8 (src/linear_octree.c:590-823)
+  f5_leapfrog.npz   N=1024 Plummer (seed 1), eps=0, dt=1e-3, 1000 steps:
+                    reference launch_simulation(LEAPFROG) final x, v and
+                    energies (src/integrator.c:894-1121, src/utils.c:27-59)
+  f6_whfast.npz     8 systems of the ensemble (seed 3), WHFast massless,
+                    dt=0.01, 10^4 steps: final ids, x, v
+                    (src/integrator_whfast.c:200-407)
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "gravity-simulator_amd"))
+
+from grav_sim_amd import ic  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def f1():
+    x, v, m = ic.plummer(1024, 1)
+    a0 = O.ref_acceleration(x, m, 1, eps=0.0)
+    a1 = O.ref_acceleration(x, m, 1, eps=1e-3)
+    np.savez_compressed(os.path.join(HERE, "f1_pairwise.npz"), x=x, v=v, m=m, a_eps0=a0, a_eps1e3=a1)
+
+
+def f2():
+    off, X, V, M = ic.ensemble(16, 11, max_test=600)
+    s = int(np.argmax(np.diff(off)))
+    sl = slice(off[s], off[s + 1])
+    x, m = X[sl].copy(), M[sl].copy()
+    a = O.ref_acceleration(x, m, 2, eps=0.0)
+    a_s = O.ref_acceleration(x, m, 2, eps=1e-2)
+    np.savez_compressed(os.path.join(HERE, "f2_massless.npz"), x=x, m=m, a_eps0=a, a_eps1e2=a_s)
+
+
+def f3_f4():
+    x, v, m = ic.two_spheres(4096, 4)
+    r1 = O.ref_octree(x, m, 1)
+    r8 = O.ref_octree(x, m, 8)
+    out = dict(x=x, m=m, width=r1["width"], sorted_keys=r1["sorted_keys"], sorted_idx=r1["sorted_idx"])
+    for n, seed in ((1 << 20, 4), (1 << 21, 5)):
+        xb, vb, mb = ic.two_spheres(n, seed)
+        rb = O.ref_octree(xb, mb, 1)
+        out[f"sha_keys_{n}"] = sha(rb["sorted_keys"])
+        out[f"sha_idx_{n}"] = sha(rb["sorted_idx"])
+        out[f"sha_x_{n}"] = sha(xb)
+        out[f"num_nodes_{n}"] = len(rb["depth"])
+    np.savez_compressed(os.path.join(HERE, "f3_morton.npz"), **out)
+    t = {}
+    for cap, r in ((1, r1), (8, r8)):
+        internal = r["nchild"] > 0
+        internal[0] = True
+        for k in ("depth", "prefix", "count", "first", "mass", "comx", "comy", "comz"):
+            t[f"cap{cap}_{k}"] = r[k][internal]
+        t[f"cap{cap}_num_nodes"] = len(r["depth"])
+    np.savez_compressed(os.path.join(HERE, "f4_octree.npz"), x=x, m=m, **t)
+
+
+def f5():
+    x, v, m = ic.plummer(1024, 1)
+    e0 = O.ref_energy(x, v, m, 1.0)
+    ids, xf, vf, mf, steps = O.ref_simulation(x, v, m, 1.0, 4, 1e-3, 1.0, method=1, eps=0.0)
+    ef = O.ref_energy(xf, vf, mf, 1.0)
+    np.savez_compressed(os.path.join(HERE, "f5_leapfrog.npz"), x=x, v=v, m=m, x_final=xf, v_final=vf,
+                        steps=steps, e0=e0, ef=ef)
+
+
+def f6():
+    off, X, V, M = ic.ensemble(8, 3)
+    out = dict(offsets=off, x=X, v=V, m=M)
+    for s in range(8):
+        sl = slice(off[s], off[s + 1])
+        ids, xf, vf, mf, steps = O.ref_simulation(X[sl], V[sl], M[sl], 1.0, 10, 0.01, 100.0, method=2)
+        out[f"ids_{s}"] = ids
+        out[f"x_{s}"] = xf
+        out[f"v_{s}"] = vf
+        print("whfast system", s, len(ids), "steps", steps, flush=True)
+    np.savez_compressed(os.path.join(HERE, "f6_whfast.npz"), **out)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6"]
+    for w in which:
+        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6}[w]()
+        print("wrote", w, flush=True)

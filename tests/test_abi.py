@@ -1,0 +1,154 @@
+"""C-ABI checks that need no GPU: the library loads, exports every declared
+symbol, keeps the reference struct layouts, and its host-side validation
+returns the reference's error codes."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from grav_sim_amd import _abi
+from grav_sim_amd import lib as L
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _have_gpu():
+    try:
+        return os.path.exists("/dev/kfd")
+    except Exception:
+        return False
+
+
+def test_struct_layouts_match_reference():
+    # sizes/offsets of src/system.h:12-20, src/acceleration.h:20-26,
+    # src/error.h:31-36, src/common.h:29-36, src/linear_octree.h:20-57 on LP64
+    assert C.sizeof(_abi.System) == 48
+    assert _abi.System.x.offset == 16 and _abi.System.m.offset == 32 and _abi.System.G.offset == 40
+    assert C.sizeof(_abi.AccelerationParam) == 32
+    assert _abi.AccelerationParam.max_num_particles_per_leaf.offset == 24
+    assert C.sizeof(_abi.ErrorStatus) == 24
+    assert C.sizeof(_abi.IntegratorParam) == 40
+    assert C.sizeof(_abi.LinearOctree) == 96
+
+
+def test_header_layout_compiles_with_static_asserts(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include "grav_sim_hip.h"\n#include "grav_sim_hip_ext.h"\n#include <stddef.h>\n'
+        "_Static_assert(sizeof(System) == 48, \"System\");\n"
+        "_Static_assert(offsetof(System, G) == 40, \"System.G\");\n"
+        "_Static_assert(sizeof(AccelerationParam) == 32, \"AccelerationParam\");\n"
+        "_Static_assert(sizeof(ErrorStatus) == 24, \"ErrorStatus\");\n"
+        "_Static_assert(sizeof(IntegratorParam) == 40, \"IntegratorParam\");\n"
+        "_Static_assert(sizeof(LinearOctree) == 96, \"LinearOctree\");\n"
+        "int main(void) { return 0; }\n")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"), str(src),
+                        "-o", str(tmp_path / "layout")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    declared = set()
+    for h in ("grav_sim_hip.h", "grav_sim_hip_ext.h"):
+        text = open(os.path.join(REPO, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
+        for mm in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = mm.group(1)
+            if name not in ("sizeof", "struct", "if", "while", "return", "GS_RESTRICT", "typedef"):
+                declared.add(name)
+    declared = {d for d in declared if not d.startswith("WRAP_")}
+    missing = [d for d in sorted(declared) if not hasattr(lib, d)]
+    assert not missing, missing
+    assert set(L.EXPORTS) <= declared
+
+
+def test_defaults_match_reference():
+    lib = L.load()
+    p = lib.get_new_acceleration_param()
+    assert (p.method, p.opening_angle, p.softening_length, p.max_num_particles_per_leaf) == (1, 1.0, 0.0, -1)
+    ip = lib.get_new_integrator_param()
+    assert (ip.integrator, ip.dt, ip.tolerance, ip.initial_dt, ip.whfast_remove_invalid_particles) == \
+        (-1, -1.0, -1.0, -1.0, True)
+    op = lib.get_new_output_param()
+    assert op.method == _abi.OUTPUT_METHOD_DISABLED and op.output_interval == -1.0
+    s = lib.get_new_system()
+    assert s.num_particles == 0 and abs(s.G - 0.000295912208284119496676630) < 1e-20
+
+
+@pytest.mark.parametrize("method,theta,eps,leaf,code,new_leaf", [
+    (1, 1.0, 0.0, -1, 0, -1),
+    (3, 0.5, 0.0, -1, 0, 1),       # BH: leaf -1 -> 1 (src/acceleration.c:113-116)
+    (3, 0.5, 0.0, 4, 0, 4),
+    (3, -0.1, 0.0, 1, 2, 1),       # negative opening angle
+    (1, 1.0, -1e-3, -1, 2, -1),    # negative softening
+    (3, 0.5, 0.0, 0, 2, 0),        # leaf < 1
+    (7, 0.5, 0.0, 1, 2, 1),        # unknown method
+    (16, 1.0, 1e-3, -1, 0, -1),    # FP32 split (additive method id)
+])
+def test_finalize_acceleration_param(method, theta, eps, leaf, code, new_leaf):
+    lib = L.load()
+    p = L.param(method, theta, eps, leaf)
+    st = lib.finalize_acceleration_param(C.byref(p))
+    assert st.return_code == code
+    if code:
+        text = L.tb_text(st)
+        assert "ValueError" in text
+        lib.free_traceback(C.byref(st))
+    assert p.max_num_particles_per_leaf == new_leaf
+
+
+def test_finalize_integration_param():
+    lib = L.load()
+    ip = lib.get_new_integrator_param()
+    ip.integrator = 4
+    st = lib.finalize_integration_param(C.byref(ip))
+    assert st.return_code == _abi.GRAV_VALUE_ERROR  # dt must be positive
+    lib.free_traceback(C.byref(st))
+    ip.dt = 0.1
+    assert lib.finalize_integration_param(C.byref(ip)).return_code == 0
+    ip.integrator = 42
+    st = lib.finalize_integration_param(C.byref(ip))
+    assert st.return_code == _abi.GRAV_VALUE_ERROR
+    lib.free_traceback(C.byref(st))
+
+
+def test_check_if_included():
+    lib = L.load()
+    a = (5 << 60) | 123
+    b = (5 << 60) | 999
+    assert lib.linear_octree_check_if_included(a, b, 1)
+    assert not lib.linear_octree_check_if_included(a, b, 21)
+
+
+def test_unknown_method_rejected_before_device_use():
+    import numpy as np
+    lib = L.load()
+    x = np.zeros((4, 3))
+    m = np.ones(4)
+    s = _abi.make_system(x, x.copy(), m, 1.0)
+    p = L.param(99)
+    a = np.zeros((4, 3))
+    st = lib.acceleration(_abi.dptr(a), C.byref(s), C.byref(p))
+    assert st.return_code == _abi.GRAV_VALUE_ERROR
+    assert "Unknown acceleration method" in L.tb_text(st)
+    lib.free_traceback(C.byref(st))
+
+
+@pytest.mark.skipif(_have_gpu(), reason="checks the no-device path")
+def test_no_device_fails_loudly():
+    """Without a HIP device the library reports GRAV_FAILURE: there is no CPU fallback."""
+    import numpy as np
+    r = subprocess.run(
+        ["python", "-c",
+         "import sys; sys.path.insert(0, %r)\n"
+         "import numpy as np\nfrom grav_sim_amd import lib\n"
+         "try:\n    lib.acceleration(np.zeros((3, 3)) + np.arange(9).reshape(3, 3), np.ones(3))\n"
+         "except RuntimeError as e:\n    print('RAISED', 'No HIP device' in str(e))\n"
+         % os.path.join(REPO, "gravity-simulator_amd")],
+        capture_output=True, text=True, timeout=120)
+    assert "RAISED True" in r.stdout, r.stdout + r.stderr
+    del np
