@@ -61,11 +61,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the FP32 shape autotuner first (default: the library's default shape)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -78,8 +79,9 @@ def main():
     info = L.device_info()
     x, v, m = ic.plummer(N_DIRECT, 2)
     tune = None
-    if not args.no_autotune:
+    if args.autotune:
         tune = L.autotune_fp32(N_DIRECT)
+    copy_gbs = L.copy_calibrate(1 << 28, 5)
     p = L.param(16, eps=EPS)
     sim = L.Sim(x, v, m, 1.0)
     sim.leapfrog_begin(p, DT)
@@ -121,6 +123,7 @@ def main():
                      "flops_per_launch": FLOPS_PER_INTERACTION * inter},
         "accuracy": {"vs": "FP64 GPU direct sum, 4096 sampled targets", "median_rel": med, "p99_rel": p99,
                      "max_rel": mx},
+        "hbm_copy_gbs": copy_gbs,
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "cu": info.cu_count,
                    "clock_khz": info.clock_khz},
     }

@@ -13,6 +13,7 @@
 
 namespace gs {
 
+hipError_t launch_copy(Context *c, const void *src, void *dst, size_t bytes);
 hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *m, int n,
                                      double G, double eps, bool skip_massless,
                                      const int *targets, int num_targets, double *out);
@@ -496,17 +497,52 @@ ErrorStatus gs_last_walk_interactions(long long *count)
     return make_success_error_status();
 }
 
+// ---------------------------------------------------------------- copy calibration
+ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs)
+{
+    if (!gbs) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "gbs is NULL");
+    if (bytes < 16 || reps < 1) return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "bytes >= 16, reps >= 1");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    bytes &= ~static_cast<size_t>(15);
+    void *src = nullptr, *dst = nullptr;
+    GS_HIP(hipMalloc(&src, bytes));
+    hipError_t e = hipMalloc(&dst, bytes);
+    if (!e) e = hipMemsetAsync(src, 0, bytes, c->stream);
+    if (!e) e = launch_copy(c, src, dst, bytes);  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!e) e = hipEventCreate(&e0);
+    if (!e) e = hipEventCreate(&e1);
+    if (!e) e = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && !e; r++) e = launch_copy(c, src, dst, bytes);
+    if (!e) e = hipEventRecord(e1, c->stream);
+    if (!e) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (!e) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e) return hip_error(e, "copy calibration", __FILE__, __LINE__, __func__);
+    *gbs = 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
+    return make_success_error_status();
+}
+
 // ---------------------------------------------------------------- FP32 autotuner
+static const int kShapes[][2] = {{0, 1},   {0, 2},   {0, 4},   {256, 1}, {256, 2},
+                                  {256, 4}, {128, 2}, {128, 4}, {512, 1}, {512, 2}};
+
 ErrorStatus gs_set_fp32_config(int block, int ipt, int split)
 {
-    const int shape = block * 16 + ipt;
-    const int ok[] = {256 * 16 + 1, 256 * 16 + 2, 256 * 16 + 4, 128 * 16 + 2,
-                      128 * 16 + 4, 512 * 16 + 1, 512 * 16 + 2};
-    if (std::find(std::begin(ok), std::end(ok), shape) == std::end(ok) || split < 0)
+    bool ok = false;
+    for (auto &sh : kShapes) ok = ok || (sh[0] == block && sh[1] == ipt);
+    if (!ok || split < 0)
         return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unsupported FP32 shape (%d, %d, %d)", block,
                                     ipt, split);
     Fp32Config &cfg = fp32_config();
-    cfg.block = block;
+    cfg.kind = block == 0 ? 1 : 0;
+    cfg.block = block == 0 ? 256 : block;
     cfg.ipt = ipt;
     cfg.jsplit = split;
     return make_success_error_status();
@@ -532,18 +568,18 @@ ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
         gs_sim_destroy(s);
         return es;
     }
-    const int shapes[][2] = {{256, 1}, {256, 2}, {256, 4}, {128, 2}, {128, 4}, {512, 1}, {512, 2}};
-    const int splits[] = {0, 1, 2, 4, 8, 16, 32};
+    const int splits[] = {0, 4, 8, 16, 32, 64};
     const Fp32Config saved = fp32_config();
     Fp32Config best = saved;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    for (auto &sh : shapes)
+    for (auto &sh : kShapes)
         for (int sp : splits) {
             Fp32Config cfg;
-            cfg.block = sh[0];
+            cfg.kind = sh[0] == 0 ? 1 : 0;
+            cfg.block = sh[0] == 0 ? 256 : sh[0];
             cfg.ipt = sh[1];
             cfg.jsplit = sp;
             const size_t pb = direct_f32_partial_bytes(n, cfg, c->cu_count);
@@ -570,7 +606,7 @@ ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
     (void)hipEventDestroy(e1);
     gs_sim_destroy(s);
     fp32_config() = best;
-    if (block) *block = best.block;
+    if (block) *block = best.kind == 1 ? 0 : best.block;
     if (ipt) *ipt = best.ipt;
     if (split) *split = best.jsplit;
     return make_success_error_status();

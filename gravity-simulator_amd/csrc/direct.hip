@@ -212,6 +212,108 @@ __global__ __launch_bounds__(BS) void k_direct_f32(const double *__restrict__ x,
     }
 }
 
+// Scalar-source variant: sources are pre-converted once per evaluation to
+// FP32 offsets from the centre of their own source tile (k_f32_sources);
+// every lane converts its FP64 target position to the same tile frame once
+// per tile, then streams the tile's sources through wave-uniform (scalar)
+// loads, so the sources sit in SGPRs and no LDS or barrier is involved.
+constexpr int ST_TILE = 128;
+
+__global__ void k_f32_sources(const double *__restrict__ x, const double *__restrict__ m, int n,
+                              int n_pad, float4 *__restrict__ S, double4 *__restrict__ C)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_pad) return;
+    const int c0 = (j / ST_TILE) * ST_TILE;  // < n for every tile of n_pad
+    const double cx = x[3 * c0 + 0], cy = x[3 * c0 + 1], cz = x[3 * c0 + 2];
+    if (j == c0) C[j / ST_TILE] = make_double4(cx, cy, cz, 0.0);
+    float4 s;
+    if (j < n) {
+        s.x = static_cast<float>(x[3 * j + 0] - cx);
+        s.y = static_cast<float>(x[3 * j + 1] - cy);
+        s.z = static_cast<float>(x[3 * j + 2] - cz);
+        s.w = static_cast<float>(m[j]);
+    } else {
+        s.x = s.y = s.z = s.w = 0.0f;
+    }
+    S[j] = s;
+}
+
+template <int IPT, bool SOFT>
+__global__ __launch_bounds__(256) void k_direct_f32s(const double *__restrict__ x, int n,
+                                                     const float4 *__restrict__ S,
+                                                     const double4 *__restrict__ C, float eps2,
+                                                     int tiles_per_chunk, int ntiles,
+                                                     double *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int base = wave * 64 * IPT;
+    if (base >= n) return;
+    double xd[IPT], yd[IPT], zd[IPT];
+    float sx[IPT], sy[IPT], sz[IPT], cx[IPT], cy[IPT], cz[IPT];
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int i = base + r * 64 + lane;
+        const int ii = i < n ? i : base;
+        xd[r] = x[3 * ii + 0];
+        yd[r] = x[3 * ii + 1];
+        zd[r] = x[3 * ii + 2];
+        sx[r] = sy[r] = sz[r] = 0.0f;
+        cx[r] = cy[r] = cz[r] = 0.0f;
+    }
+    const int J0 = blockIdx.y * tiles_per_chunk;
+    const int J1 = min(ntiles, J0 + tiles_per_chunk);
+    for (int J = J0; J < J1; J++) {
+        const double4 c = C[J];
+        float xi[IPT], yi[IPT], zi[IPT], px[IPT], py[IPT], pz[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            xi[r] = static_cast<float>(xd[r] - c.x);
+            yi[r] = static_cast<float>(yd[r] - c.y);
+            zi[r] = static_cast<float>(zd[r] - c.z);
+            px[r] = py[r] = pz[r] = 0.0f;
+        }
+        const float4 *__restrict__ Sj = S + static_cast<size_t>(J) * ST_TILE;
+#pragma unroll 16
+        for (int jj = 0; jj < ST_TILE; jj++) {
+            const float4 s = Sj[jj];
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const float dx = s.x - xi[r];
+                const float dy = s.y - yi[r];
+                const float dz = s.z - zi[r];
+                float r2 = __builtin_fmaf(dx, dx, eps2);
+                r2 = __builtin_fmaf(dy, dy, r2);
+                r2 = __builtin_fmaf(dz, dz, r2);
+                float ri = __builtin_amdgcn_rsqf(r2);
+                if (!SOFT) ri = r2 > 0.0f ? ri : 0.0f;
+                const float mr = s.w * ri;
+                const float w = mr * (ri * ri);
+                px[r] = __builtin_fmaf(dx, w, px[r]);
+                py[r] = __builtin_fmaf(dy, w, py[r]);
+                pz[r] = __builtin_fmaf(dz, w, pz[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            kahan_add(sx[r], cx[r], px[r]);
+            kahan_add(sy[r], cy[r], py[r]);
+            kahan_add(sz[r], cz[r], pz[r]);
+        }
+    }
+    double *out = partial + static_cast<size_t>(blockIdx.y) * 3 * n;
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int i = base + r * 64 + lane;
+        if (i < n) {
+            out[3 * i + 0] = static_cast<double>(sx[r]) - static_cast<double>(cx[r]);
+            out[3 * i + 1] = static_cast<double>(sy[r]) - static_cast<double>(cy[r]);
+            out[3 * i + 2] = static_cast<double>(sz[r]) - static_cast<double>(cz[r]);
+        }
+    }
+}
+
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nsplit, int n3,
                                   double G, double *__restrict__ a)
 {
@@ -228,21 +330,29 @@ Fp32Config &fp32_config()
     return cfg;
 }
 
+// Source chunks per evaluation when the configuration leaves it to us:
+// about 2048 sources per chunk (64 chunks at N = 2^17: the best measured
+// balance on MI355X between enough workgroups to fill 256 CUs and the cost
+// of the FP64 partial sums written per chunk and reduced afterwards).
 static int resolve_split(int n, const Fp32Config &cfg, int cu_count)
 {
+    (void)cu_count;
     if (cfg.jsplit > 0) return cfg.jsplit;
-    const int tb = cfg.block * cfg.ipt;
-    const int blocks = (n + tb - 1) / tb;
-    const int want = 8 * (cu_count > 0 ? cu_count : 256);  // workgroups to fill the chip
-    int split = (want + blocks - 1) / blocks;
-    const int max_split = (n + 4 * cfg.block - 1) / (4 * cfg.block);  // >= 4 tiles per chunk
-    if (split > max_split) split = max_split;
+    const int per_chunk = 2048;
+    const int split = (n + per_chunk - 1) / per_chunk;
     return split < 1 ? 1 : split;
 }
 
+static int f32s_tiles(int n) { return (n + ST_TILE - 1) / ST_TILE; }
+
 size_t direct_f32_partial_bytes(int n, const Fp32Config &cfg, int cu_count)
 {
-    return static_cast<size_t>(resolve_split(n, cfg, cu_count)) * 3 * n * sizeof(double);
+    size_t b = static_cast<size_t>(resolve_split(n, cfg, cu_count)) * 3 * n * sizeof(double);
+    if (cfg.kind == 1) {
+        const size_t nt = f32s_tiles(n);
+        b += 256 + nt * ST_TILE * sizeof(float4) + nt * sizeof(double4);
+    }
+    return b;
 }
 
 template <int BS, int IPT>
@@ -257,39 +367,96 @@ static void launch_f32_shape(Context *c, dim3 grid, const double *x, const doubl
                            n, eps2, chunk, partial);
 }
 
+template <int IPT>
+static void launch_f32s_shape(Context *c, dim3 grid, const double *x, int n, const float4 *S,
+                              const double4 *C, float eps2, bool soft, int tpc, int ntiles,
+                              double *partial)
+{
+    if (soft)
+        hipLaunchKernelGGL((k_direct_f32s<IPT, true>), grid, dim3(256), 0, c->stream, x, n, S, C,
+                           eps2, tpc, ntiles, partial);
+    else
+        hipLaunchKernelGGL((k_direct_f32s<IPT, false>), grid, dim3(256), 0, c->stream, x, n, S,
+                           C, eps2, tpc, ntiles, partial);
+}
+
 hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n, double G,
                              double eps, double *a, void *partial_buf, size_t partial_bytes,
                              const Fp32Config &cfg)
 {
     const int split = resolve_split(n, cfg, c->cu_count);
-    if (partial_bytes < static_cast<size_t>(split) * 3 * n * sizeof(double))
-        return hipErrorInvalidValue;
-    const int tb = cfg.block * cfg.ipt;
-    const int blocks = (n + tb - 1) / tb;
-    int chunk = (n + split - 1) / split;
-    chunk = (chunk + cfg.block - 1) / cfg.block * cfg.block;
-    const int nsplit = (n + chunk - 1) / chunk;
-    dim3 grid(blocks, nsplit);
+    if (partial_bytes < direct_f32_partial_bytes(n, cfg, c->cu_count)) return hipErrorInvalidValue;
     const float eps2 = static_cast<float>(eps * eps);
     const bool soft = eps2 > 0.0f;
     double *partial = static_cast<double *>(partial_buf);
-    const int rec = timing_begin(c, "direct_f32");
-    switch (cfg.block * 16 + cfg.ipt) {
-        case 256 * 16 + 1: launch_f32_shape<256, 1>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        case 256 * 16 + 2: launch_f32_shape<256, 2>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        case 256 * 16 + 4: launch_f32_shape<256, 4>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        case 128 * 16 + 2: launch_f32_shape<128, 2>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        case 128 * 16 + 4: launch_f32_shape<128, 4>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        case 512 * 16 + 1: launch_f32_shape<512, 1>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        case 512 * 16 + 2: launch_f32_shape<512, 2>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
-        default: timing_end(c, rec); return hipErrorInvalidValue;
+    int nsplit;
+    if (cfg.kind == 1) {
+        const int ntiles = f32s_tiles(n);
+        const int n_pad = ntiles * ST_TILE;
+        char *tail = static_cast<char *>(partial_buf) + static_cast<size_t>(split) * 3 * n * sizeof(double);
+        tail = reinterpret_cast<char *>((reinterpret_cast<uintptr_t>(tail) + 255) & ~uintptr_t(255));
+        float4 *S = reinterpret_cast<float4 *>(tail);
+        double4 *C = reinterpret_cast<double4 *>(tail + static_cast<size_t>(n_pad) * sizeof(float4));
+        hipLaunchKernelGGL(k_f32_sources, dim3((n_pad + 255) / 256), dim3(256), 0, c->stream, x, m,
+                           n, n_pad, S, C);
+        const int tpc = (ntiles + split - 1) / split;
+        nsplit = (ntiles + tpc - 1) / tpc;
+        const int waves = (n + 64 * cfg.ipt - 1) / (64 * cfg.ipt);
+        dim3 grid((waves + 3) / 4, nsplit);
+        const int rec = timing_begin(c, "direct_f32");
+        switch (cfg.ipt) {
+            case 1: launch_f32s_shape<1>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+            case 2: launch_f32s_shape<2>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+            case 4: launch_f32s_shape<4>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+            default: timing_end(c, rec); return hipErrorInvalidValue;
+        }
+        timing_end(c, rec);
+    } else {
+        const int tb = cfg.block * cfg.ipt;
+        const int blocks = (n + tb - 1) / tb;
+        int chunk = (n + split - 1) / split;
+        chunk = (chunk + cfg.block - 1) / cfg.block * cfg.block;
+        nsplit = (n + chunk - 1) / chunk;
+        dim3 grid(blocks, nsplit);
+        const int rec = timing_begin(c, "direct_f32");
+        switch (cfg.block * 16 + cfg.ipt) {
+            case 256 * 16 + 1: launch_f32_shape<256, 1>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            case 256 * 16 + 2: launch_f32_shape<256, 2>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            case 256 * 16 + 4: launch_f32_shape<256, 4>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            case 128 * 16 + 2: launch_f32_shape<128, 2>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            case 128 * 16 + 4: launch_f32_shape<128, 4>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            case 512 * 16 + 1: launch_f32_shape<512, 1>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            case 512 * 16 + 2: launch_f32_shape<512, 2>(c, grid, x, m, n, eps2, soft, chunk, partial); break;
+            default: timing_end(c, rec); return hipErrorInvalidValue;
+        }
+        timing_end(c, rec);
     }
-    timing_end(c, rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int n3 = 3 * n;
     hipLaunchKernelGGL(k_reduce_partials, dim3((n3 + 255) / 256), dim3(256), 0, c->stream,
                        partial, nsplit, n3, G, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ copy
+// Plain streaming copy (16 B per lane per iteration), used to calibrate the
+// HBM byte counters and to measure achievable copy bandwidth.
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n16)
+{
+    for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < n16;
+         q += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[q] = src[q];
+}
+
+hipError_t launch_copy(Context *c, const void *src, void *dst, size_t bytes)
+{
+    const size_t n16 = bytes / 16;
+    const int grid = 8 * (c->cu_count > 0 ? c->cu_count : 256);
+    const int rec = timing_begin(c, "copy");
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream,
+                       static_cast<const double2 *>(src), static_cast<double2 *>(dst), n16);
+    timing_end(c, rec);
     return hipGetLastError();
 }
 

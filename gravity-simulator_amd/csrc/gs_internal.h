@@ -106,8 +106,9 @@ hipError_t launch_direct_f64(Context *c, const double *x, const double *m, int n
 
 // FP32 tile-relative direct sum with tile-level Kahan compensation.
 struct Fp32Config {
-    int block = 256;     // threads per workgroup
-    int ipt = 2;         // targets per thread
+    int kind = 1;        // 0: LDS source tiles, 1: scalar-loaded source tiles
+    int block = 256;     // threads per workgroup (kind 0)
+    int ipt = 1;         // targets per thread
     int jsplit = 0;      // source chunks (0 = automatic)
 };
 hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n, double G,

@@ -31,7 +31,7 @@ EXPORTS = [
     "gs_sim_create", "gs_sim_destroy", "gs_sim_upload", "gs_sim_download", "gs_sim_acceleration",
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
-    "gs_set_fp32_config",
+    "gs_set_fp32_config", "gs_copy_calibrate",
 ]
 
 
@@ -93,6 +93,7 @@ def load(path=None):
         "gs_last_walk_interactions": (ES, [C.POINTER(C.c_longlong)]),
         "gs_autotune_fp32": (ES, [C.c_int, PI, PI, PI]),
         "gs_set_fp32_config": (ES, [C.c_int, C.c_int, C.c_int]),
+        "gs_copy_calibrate": (ES, [C.c_size_t, C.c_int, PD]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -215,6 +216,12 @@ def autotune_fp32(n):
     b, i, s = C.c_int(), C.c_int(), C.c_int()
     check(load().gs_autotune_fp32(int(n), C.byref(b), C.byref(i), C.byref(s)), "gs_autotune_fp32")
     return b.value, i.value, s.value
+
+
+def copy_calibrate(nbytes=1 << 28, reps=5):
+    g = C.c_double()
+    check(load().gs_copy_calibrate(int(nbytes), int(reps), C.byref(g)), "gs_copy_calibrate")
+    return g.value
 
 
 def set_fp32_config(block, ipt, split):

@@ -9,6 +9,8 @@
 #ifndef GRAV_SIM_HIP_EXT_H
 #define GRAV_SIM_HIP_EXT_H
 
+#include <stddef.h>
+
 #include "grav_sim_hip.h"
 
 #ifdef __cplusplus
@@ -94,10 +96,18 @@ ErrorStatus gs_octree_sort(const System *system, int64 *sorted_keys, int *sorted
  * Barnes-Hut walk. */
 ErrorStatus gs_last_walk_interactions(long long *count);
 
+/* Streams `bytes` through a plain device copy kernel `reps` times and writes
+ * the achieved bandwidth (read + write bytes / time).  Its kernel ("copy")
+ * is the reference point for the HBM byte counters. */
+ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs);
+
 /* ---- FP32 direct-sum autotuner ---------------------------------------------
  * Times the candidate (block, targets-per-thread, source-split) shapes on an
  * N-particle input resident on the device and keeps the fastest for later
- * calls.  Writes the chosen shape. */
+ * calls.  Writes the chosen shape.  block = 0 denotes the scalar-source
+ * kernel (independent wavefronts reading pre-converted source tiles through
+ * wave-uniform loads); block = 128/256/512 the LDS-tile kernel.  A source
+ * split of 0 picks the split automatically. */
 ErrorStatus gs_autotune_fp32(int num_particles, int *block, int *targets_per_thread,
                              int *source_split);
 ErrorStatus gs_set_fp32_config(int block, int targets_per_thread, int source_split);

@@ -91,3 +91,19 @@ def test_energy_matches_oracle(golden):
     s = _abi.make_system(f["x"], f["v"], f["m"], 1.0)
     e = L.load().compute_energy(C.byref(s))
     assert abs(e - float(f["e0"])) <= 1e-12 * abs(float(f["e0"]))
+
+
+@pytest.mark.parametrize("block,ipt", [(0, 1), (0, 2), (0, 4), (256, 1), (256, 2), (256, 4), (128, 2),
+                                       (128, 4), (512, 1), (512, 2)])
+@pytest.mark.parametrize("split", [0, 3])
+def test_fp32_every_shape_vs_oracle(block, ipt, split):
+    """Each kernel shape the autotuner may pick gives the same accuracy (ragged N)."""
+    x, v, m = ic.plummer(3001, 13)
+    try:
+        L.set_fp32_config(block, ipt, split)
+        a32 = L.acceleration(x, m, 16, eps=1e-3)
+    finally:
+        L.set_fp32_config(0, 1, 0)
+    a64 = O.pairwise(x, m, eps=1e-3)
+    e = rel_err(a32, a64)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
