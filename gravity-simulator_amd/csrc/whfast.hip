@@ -1,13 +1,774 @@
-// whfast.hip -- GPU WHFast (src/integrator_whfast.c:200-407).
+// whfast.hip -- WHFast on the GPU for ensembles of independent systems
+// (src/integrator_whfast.c:200-1279, src/system.c:444-528,1210-1335).
+//
+// One workgroup integrates one system for a chunk of steps; the system's
+// arrays stay in device memory between chunks.  Every FP64 expression keeps
+// the reference's association order (contraction is off for this file), and
+// every sum runs in the reference's index order, so a system's trajectory is
+// bitwise equal to the reference's (with the particle-0 acceleration slot the
+// reference never writes taken as zero).
+//
+// Per step:  distances |jx_i - jx_primary| -> bitonic sort of (distance,
+// index) in LDS -> permute jx, jv, m, ids -> eta (cumulative mass over the
+// massive particles; massless ones add exactly 0) -> universal-variable
+// Kepler drift per particle -> removal of unconverged particles (ascending
+// order) -> Jacobi->Cartesian positions (a sequential recurrence, one lane
+// per component) -> massless-form accelerations and kick.  Cartesian
+// velocities are only needed at the end and are reconstructed there.
+#include <cmath>
+#include <vector>
+
 #include "gs_internal.h"
 
+#pragma clang fp contract(off)
+
 namespace gs {
+
+constexpr int WF_BLOCK = 256;
+constexpr int WF_MAX_N = 2048;      // particles per system (LDS sort capacity)
+constexpr int WF_MAX_MASSIVE = 64;  // massive particles per system
+
+struct WfSys {
+    long long off;  // first particle in the flat arrays
+    int cap;        // particles at the start
+};
+
+struct WfArgs {
+    const WfSys *sys;
+    double *jx, *jv, *x, *v, *m, *jx2, *jv2, *m2;
+    int *ids, *ids2;
+    int *count;       // live particles per system
+    int *status;      // per system: bit 0 = Stumpff argument not finite (no removal)
+    double G, eps, dt, dt_last;
+    long long step0, nsteps, total_steps;
+    int method;       // 1 pairwise form, 2 massless form
+    int remove_invalid;
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ double norm3(double a, double b, double c)
+{
+    return __builtin_sqrt(a * a + b * b + c * c);
+}
+
+// src/integrator_whfast.c:774-815
+__device__ void stumpff(double z, double &c0, double &c1, double &c2, double &c3)
+{
+    int k = 0;
+    while (fabs(z) > 0.1) {
+        z /= 4.0;
+        k++;
+    }
+    double t3 = (1.0 - z / 20.0 *
+                           (1.0 - z / 42.0 *
+                                      (1.0 - z / 72.0 *
+                                                 (1.0 - z / 110.0 *
+                                                            (1.0 - z / 156.0 * (1.0 - z / 210.0)))))) /
+                6.0;
+    double t2 = (1.0 - z / 12.0 *
+                           (1.0 - z / 30.0 *
+                                      (1.0 - z / 56.0 *
+                                                 (1.0 - z / 90.0 *
+                                                            (1.0 - z / 132.0 * (1.0 - z / 182.0)))))) /
+                2.0;
+    double t1 = 1.0 - z * t3;
+    double t0 = 1.0 - z * t2;
+    while (k > 0) {
+        t3 = (t2 + t0 * t3) / 4.0;
+        t2 = (t1 * t1) / 2.0;
+        t1 = t0 * t1;
+        t0 = (2.0 * t0 * t0) - 1.0;
+        k--;
+    }
+    c0 = t0;
+    c1 = t1;
+    c2 = t2;
+    c3 = t3;
+}
+
+// Kepler drift of one Jacobi particle (src/integrator_whfast.c:459-578).
+// Returns 1 when the particle is to be removed (non-converged with error
+// above 1e-5 or a non-finite Stumpff argument), 2 when the Stumpff argument
+// was not finite (reported when removal is off).
+__device__ int kepler_drift(double *jx, double *jv, double gm, double dt)
+{
+    const double x0 = jx[0], x1 = jx[1], x2 = jx[2];
+    const double v0 = jv[0], v1 = jv[1], v2 = jv[2];
+    const double xn = norm3(x0, x1, x2);
+    const double vn = norm3(v0, v1, v2);
+    const double rv = (x0 * v0 + x1 * v1 + x2 * v2) / xn;
+    const double alpha = 2.0 * gm / xn - (vn * vn);
+    double s = dt / xn;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    bool converged = false, zbad = false;
+    for (int it = 0; it < 500; it++) {
+        const double z = alpha * (s * s);
+        if (!isfinite(z)) {
+            zbad = true;
+            break;
+        }
+        stumpff(z, c0, c1, c2, c3);
+        const double F = (xn * s * c1 + xn * rv * (s * s) * c2 + gm * (s * s * s) * c3 - dt);
+        const double dF = (xn * c0 + xn * rv * s * c1 + gm * (s * s) * c2);
+        const double ds = -F / dF;
+        s += ds;
+        if (fabs(ds) < 1e-12) {
+            converged = true;
+            break;
+        }
+    }
+    const double r = xn * c0 + xn * rv * s * c1 + gm * (s * s) * c2;
+    int flag = 0;
+    if (!converged) {
+        const double err = (xn * s * c1 + xn * rv * (s * s) * c2 + gm * (s * s * s) * c3 - dt) / r;
+        if (err > 1e-5 || zbad) flag = 1;
+        if (zbad) flag |= 2;
+    }
+    const double f = 1.0 - gm * (s * s) * c2 / xn;
+    const double g = dt - gm * (s * s * s) * c3;
+    const double df = -gm * s * c1 / (r * xn);
+    const double dg = 1.0 - gm * (s * s) * c2 / r;
+    jx[0] = f * x0 + g * v0;
+    jx[1] = f * x1 + g * v1;
+    jx[2] = f * x2 + g * v2;
+    jv[0] = df * x0 + dg * v0;
+    jv[1] = df * x1 + dg * v1;
+    jv[2] = df * x2 + dg * v2;
+    return flag;
+}
+
+__device__ __forceinline__ double cube_plus(double a, double b, double c, double eps3)
+{
+    const double n = norm3(a, b, c);
+    return (n * n * n) + eps3;
+}
+
+// Block-wide exclusive scan of small integers (WF_BLOCK threads).
+__device__ int block_scan_excl(int v, int *total, int *buf)
+{
+    const int t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (int off = 1; off < WF_BLOCK; off <<= 1) {
+        const int add = t >= off ? buf[t - off] : 0;
+        __syncthreads();
+        buf[t] += add;
+        __syncthreads();
+    }
+    const int incl = buf[t];
+    *total = buf[WF_BLOCK - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+struct WfShared {
+    double key[WF_MAX_N];
+    short idx[WF_MAX_N];
+    int scan_buf[WF_BLOCK];
+    int mlist[WF_MAX_MASSIVE];   // ordinals of massive particles (sorted order)
+    double mprefix[WF_MAX_MASSIVE];  // eta at each massive particle
+    double mx[WF_MAX_MASSIVE][3];    // cartesian positions of massive particles
+    double mm[WF_MAX_MASSIVE];
+    // straddle sums: massless particles in gap g (g massive particles inside)
+    // and massive particle p (pairs with one member inside, one outside)
+    double strad_l[WF_MAX_MASSIVE + 1][3];
+    double strad_m[WF_MAX_MASSIVE][3];
+    int nm;
+    int n;
+    int any_remove;
+    int primary;
+};
+
+// Sort (distance, index) ascending, lexicographically, for n <= WF_MAX_N.
+__device__ void bitonic_sort(WfShared &S, int n)
+{
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = n + threadIdx.x; i < P; i += WF_BLOCK) {
+        S.key[i] = INFINITY;
+        S.idx[i] = static_cast<short>(i);
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += WF_BLOCK) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const double ki = S.key[i], kl = S.key[l];
+                    const short ii = S.idx[i], il = S.idx[l];
+                    const bool gt = (ki > kl) || (ki == kl && ii > il);
+                    const bool up = (i & k) == 0;
+                    if (gt == up) {
+                        S.key[i] = kl;
+                        S.key[l] = ki;
+                        S.idx[i] = il;
+                        S.idx[l] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// distances from the particle whose id is 0 (src/system.c:1232-1277), then
+// sort and permute (pos, vel, m, ids) into the alternate buffers and back.
+__device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, double *m,
+                                 int *ids, double *pos2, double *vel2, double *m2, int *ids2)
+{
+    if (threadIdx.x == 0) {
+        int pr = -1;
+        if (n > 0 && ids[0] == 0)
+            pr = 0;
+        else
+            for (int i = 0; i < n; i++)
+                if (ids[i] == 0) {
+                    pr = i;
+                    break;
+                }
+        S.primary = pr;
+    }
+    __syncthreads();
+    const int pr = S.primary;
+    if (pr < 0) return;  // no particle with id 0: the reference reports an error; keep order
+    const double p0 = pos[3 * pr], p1 = pos[3 * pr + 1], p2 = pos[3 * pr + 2];
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+        S.key[i] = i == pr ? 0.0 : norm3(pos[3 * i] - p0, pos[3 * i + 1] - p1, pos[3 * i + 2] - p2);
+        S.idx[i] = static_cast<short>(i);
+    }
+    __syncthreads();
+    bitonic_sort(S, n);
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+        const int s = S.idx[i];
+        pos2[3 * i] = pos[3 * s];
+        pos2[3 * i + 1] = pos[3 * s + 1];
+        pos2[3 * i + 2] = pos[3 * s + 2];
+        vel2[3 * i] = vel[3 * s];
+        vel2[3 * i + 1] = vel[3 * s + 1];
+        vel2[3 * i + 2] = vel[3 * s + 2];
+        m2[i] = m[s];
+        ids2[i] = ids[s];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+        pos[3 * i] = pos2[3 * i];
+        pos[3 * i + 1] = pos2[3 * i + 1];
+        pos[3 * i + 2] = pos2[3 * i + 2];
+        vel[3 * i] = vel2[3 * i];
+        vel[3 * i + 1] = vel2[3 * i + 1];
+        vel[3 * i + 2] = vel2[3 * i + 2];
+        m[i] = m2[i];
+        ids[i] = ids2[i];
+    }
+    __syncthreads();
+}
+
+// Massive list (m != 0, or every particle for the pairwise form) and the
+// cumulative masses eta at each massive particle, summed in index order
+// (src/integrator_whfast.c:1266-1279; massless particles add exactly 0).
+__device__ void massive_list(WfShared &S, int n, const double *m, int method)
+{
+    // ordinals of massive particles, ascending: block scan over chunks
+    if (threadIdx.x == 0) S.nm = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += WF_BLOCK) {
+        const int i = base + threadIdx.x;
+        // particle 0 (the primary) always anchors the list
+        const int f = (i < n && (i == 0 || method == 1 || m[i] != 0.0)) ? 1 : 0;
+        int tot;
+        const int pos = block_scan_excl(f, &tot, S.scan_buf);
+        if (f && S.nm + pos < WF_MAX_MASSIVE) S.mlist[S.nm + pos] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) S.nm += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
+        double e = 0.0;
+        for (int q = 0; q < nm; q++) {
+            const int i = S.mlist[q];
+            e = (i == 0) ? m[0] : e + m[i];
+            S.mprefix[q] = e;
+            S.mm[q] = m[i];
+        }
+    }
+    __syncthreads();
+}
+
+// eta[i]: the cumulative mass up to sorted position i
+__device__ __forceinline__ double eta_at(const WfShared &S, int i)
+{
+    // largest massive ordinal <= i
+    int lo = 0, hi = S.nm - 1, best = 0;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (S.mlist[mid] <= i) {
+            best = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return S.mprefix[best];
+}
+
+// eta[i] for every particle, into S.key (free between the sorts).
+__device__ void fill_eta(WfShared &S, int n)
+{
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK) S.key[i] = eta_at(S, i);
+    __syncthreads();
+}
+
+// Jacobi -> Cartesian for one component (src/integrator_whfast.c:726-772),
+// eta from S.key.  For a massless particle the recurrence is
+//   c <- eta * ((c - 0 * jq) / eta),   q = jq + (c - 0 * jq) / eta
+// with an unchanged eta, so once c maps onto itself it stays there for the
+// rest of the run of massless particles: the loop then only forms q from the
+// cached quotient, which is bitwise what the full recurrence produces.
+__device__ void j2c_component(int n, int d, const double *jq, double *q, const double *m,
+                              const WfShared &S)
+{
+    const double *eta = S.key;
+    double c = eta[n - 1] * jq[d];
+    bool fixed = false;
+    double cq = 0.0;
+    for (int i = n - 1; i > 0; i--) {
+        const double mi = m[i];
+        const double jqi = jq[3 * i + d];
+        if (fixed && mi == 0.0 && isfinite(jqi)) {
+            q[3 * i + d] = jqi + cq;
+            continue;
+        }
+        const double ei = eta[i];
+        const double ei1 = eta[i - 1];
+        const double cn = (c - mi * jqi) / ei;
+        q[3 * i + d] = jqi + cn;
+        const double c2 = ei1 * cn;
+        fixed = (mi == 0.0) && (ei == ei1) && (c2 == c) && (c != 0.0);
+        cq = cn;
+        c = c2;
+    }
+    q[d] = c / m[0];
+}
+
+// Cartesian -> Jacobi (src/integrator_whfast.c:682-724), one lane per
+// component; particle 0 gets the centre of mass.
+__device__ void c2j_component(int n, int d, const double *q, double *jq, const double *m,
+                              const WfShared &S)
+{
+    const double *eta = S.key;
+    double c = m[0] * q[d];
+    for (int i = 1; i < n; i++) {
+        const double ei1 = eta[i - 1];
+        jq[3 * i + d] = q[3 * i + d] - c / ei1;
+        c = c * (1.0 + m[i] / ei1) + m[i] * jq[3 * i + d];
+    }
+    jq[d] = c / eta[n - 1];
+}
+
+// Interaction accelerations and kick for particle i >= 1 (massless form,
+// src/integrator_whfast.c:1022-1258; the pairwise form is the same with
+// every particle in the massive list).
+__device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const double *x,
+                           const double *jx, const double *m, double G, double eps3,
+                           double *jv, double dt)
+{
+    double a[3];
+    const double tv0 = x[3 * i] - x[0], tv1 = x[3 * i + 1] - x[1], tv2 = x[3 * i + 2] - x[2];
+    const double c0 = cube_plus(tv0, tv1, tv2, eps3);
+    const double cj = cube_plus(jx[3 * i], jx[3 * i + 1], jx[3 * i + 2], eps3);
+    const double ei = S.key[i], ei1 = S.key[i - 1];
+    const double tv[3] = {tv0, tv1, tv2};
+    if (massive) {
+        for (int d = 0; d < 3; d++)
+            a[d] = G * m[0] * ei / ei1 * (jx[3 * i + d] / cj - tv[d] / c0);
+    } else {
+        for (int d = 0; d < 3; d++) a[d] = G * m[0] * (jx[3 * i + d] / cj - tv[d] / c0);
+    }
+    const int nm = S.nm;
+    double aux0 = 0.0, aux1 = 0.0, aux2 = 0.0;
+    // interior massive particles (ordinal 1.., position < i)
+    for (int q = 1; q < nm; q++) {
+        const int j = S.mlist[q];
+        if (massive ? q >= p : j >= i) break;
+        const double t0 = x[3 * i] - S.mx[q][0], t1 = x[3 * i + 1] - S.mx[q][1],
+                     t2 = x[3 * i + 2] - S.mx[q][2];
+        const double cc = cube_plus(t0, t1, t2, eps3);
+        aux0 += G * S.mm[q] * t0 / cc;
+        aux1 += G * S.mm[q] * t1 / cc;
+        aux2 += G * S.mm[q] * t2 / cc;
+    }
+    if (massive) {
+        a[0] -= aux0 * ei / ei1;
+        a[1] -= aux1 * ei / ei1;
+        a[2] -= aux2 * ei / ei1;
+    } else {
+        a[0] -= aux0;
+        a[1] -= aux1;
+        a[2] -= aux2;
+    }
+    aux0 = aux1 = aux2 = 0.0;
+    // exterior massive particles
+    for (int q = 1; q < nm; q++) {
+        const int j = S.mlist[q];
+        if (massive ? q <= p : j <= i) continue;
+        const double t0 = S.mx[q][0] - x[3 * i], t1 = S.mx[q][1] - x[3 * i + 1],
+                     t2 = S.mx[q][2] - x[3 * i + 2];
+        const double cc = cube_plus(t0, t1, t2, eps3);
+        aux0 += G * S.mm[q] * t0 / cc;
+        aux1 += G * S.mm[q] * t1 / cc;
+        aux2 += G * S.mm[q] * t2 / cc;
+    }
+    a[0] += aux0;
+    a[1] += aux1;
+    a[2] += aux2;
+    aux0 = aux1 = aux2 = 0.0;
+    // pairs straddling i: interior j (ordinal from 0), exterior k; the sum
+    // depends only on where i sits among the massive particles
+    if (massive) {
+        aux0 = S.strad_m[p][0];
+        aux1 = S.strad_m[p][1];
+        aux2 = S.strad_m[p][2];
+    } else {
+        int g = 0;  // massive particles inside i
+        while (g < nm && S.mlist[g] < i) g++;
+        aux0 = S.strad_l[g][0];
+        aux1 = S.strad_l[g][1];
+        aux2 = S.strad_l[g][2];
+    }
+    a[0] -= aux0 / ei1;
+    a[1] -= aux1 / ei1;
+    a[2] -= aux2 / ei1;
+    jv[3 * i] += a[0] * dt;
+    jv[3 * i + 1] += a[1] * dt;
+    jv[3 * i + 2] += a[2] * dt;
+}
+
+__device__ void load_massive_positions(WfShared &S, const double *x)
+{
+    for (int q = threadIdx.x; q < S.nm; q += WF_BLOCK) {
+        const int j = S.mlist[q];
+        S.mx[q][0] = x[3 * j];
+        S.mx[q][1] = x[3 * j + 1];
+        S.mx[q][2] = x[3 * j + 2];
+    }
+    __syncthreads();
+}
+
+// Straddle sums in the reference's order (interior ordinal ascending, then
+// exterior ordinal ascending): for a massless particle with g massive
+// particles inside it, pairs (q < g <= r); for massive particle p, pairs
+// (q < p < r) (src/integrator_whfast.c:930-949, 1099-1119, 1222-1250).
+__device__ void straddle_sums(WfShared &S, double G, double eps3)
+{
+    const int nm = S.nm;
+    // t = 0..nm: gap g = t; t = nm+1..2nm-1: massive ordinal p = t - nm
+    for (int t = threadIdx.x; t < 2 * nm; t += WF_BLOCK) {
+        const bool massive = t > nm;
+        const int g = massive ? t - nm : t;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int q = 0; q < g; q++)
+            for (int r = q + 1; r < nm; r++) {
+                if (massive ? r <= g : r < g) continue;
+                const double t0 = S.mx[r][0] - S.mx[q][0], t1 = S.mx[r][1] - S.mx[q][1],
+                             t2 = S.mx[r][2] - S.mx[q][2];
+                const double cc = cube_plus(t0, t1, t2, eps3);
+                a0 += G * S.mm[q] * S.mm[r] * t0 / cc;
+                a1 += G * S.mm[q] * S.mm[r] * t1 / cc;
+                a2 += G * S.mm[q] * S.mm[r] * t2 / cc;
+            }
+        double *dst = massive ? S.strad_m[g] : S.strad_l[g];
+        dst[0] = a0;
+        dst[1] = a1;
+        dst[2] = a2;
+    }
+    __syncthreads();
+}
+
+__device__ void accel_kick_all(WfShared &S, int n, const double *x, const double *jx,
+                               const double *m, double G, double eps3, double *jv, double dt,
+                               int method)
+{
+    load_massive_positions(S, x);
+    straddle_sums(S, G, eps3);
+    // massive particles (ordinal >= 1), one per thread
+    for (int p = 1 + static_cast<int>(threadIdx.x); p < S.nm; p += WF_BLOCK)
+        accel_kick(S, S.mlist[p], true, p, x, jx, m, G, eps3, jv, dt);
+    if (method == 2) {
+        for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK)
+            if (m[i] == 0.0) accel_kick(S, i, false, -1, x, jx, m, G, eps3, jv, dt);
+    }
+    __syncthreads();
+}
+
+// Remove flagged particles preserving order (src/system.c:444-528 and the
+// Jacobi / eta arrays in src/integrator_whfast.c:624-670).
+__device__ int compact(WfShared &S, int n, const int *flag, double *jx, double *jv, double *m,
+                       int *ids, double *jx2, double *jv2, double *m2, int *ids2)
+{
+    int o = 0;
+    for (int base = 0; base < n; base += WF_BLOCK) {
+        const int i = base + threadIdx.x;
+        const int keep = (i < n && !flag[i]) ? 1 : 0;
+        int tot;
+        const int pos = block_scan_excl(keep, &tot, S.scan_buf);
+        if (keep) {
+            const int d = o + pos;
+            jx2[3 * d] = jx[3 * i];
+            jx2[3 * d + 1] = jx[3 * i + 1];
+            jx2[3 * d + 2] = jx[3 * i + 2];
+            jv2[3 * d] = jv[3 * i];
+            jv2[3 * d + 1] = jv[3 * i + 1];
+            jv2[3 * d + 2] = jv[3 * i + 2];
+            m2[d] = m[i];
+            ids2[d] = ids[i];
+        }
+        o += tot;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < o; i += WF_BLOCK) {
+        jx[3 * i] = jx2[3 * i];
+        jx[3 * i + 1] = jx2[3 * i + 1];
+        jx[3 * i + 2] = jx2[3 * i + 2];
+        jv[3 * i] = jv2[3 * i];
+        jv[3 * i + 1] = jv2[3 * i + 1];
+        jv[3 * i + 2] = jv2[3 * i + 2];
+        m[i] = m2[i];
+        ids[i] = ids2[i];
+    }
+    __syncthreads();
+    return o;
+}
+
+// Initialisation (src/integrator_whfast.c:241-273): sort by Cartesian
+// distance, eta, Cartesian -> Jacobi, accelerations, half kick.
+__global__ __launch_bounds__(WF_BLOCK) void k_whfast_init(WfArgs A)
+{
+    __shared__ WfShared S;
+    const WfSys sy = A.sys[blockIdx.x];
+    const long long o = sy.off;
+    int n = A.count[blockIdx.x];
+    double *x = A.x + 3 * o, *v = A.v + 3 * o, *jx = A.jx + 3 * o, *jv = A.jv + 3 * o;
+    double *m = A.m + o, *m2 = A.m2 + o;
+    int *ids = A.ids + o, *ids2 = A.ids2 + o;
+    sort_by_distance(S, n, x, v, m, ids, A.jx2 + 3 * o, A.jv2 + 3 * o, m2, ids2);
+    massive_list(S, n, m, A.method);
+    fill_eta(S, n);
+    if (threadIdx.x < 3) c2j_component(n, threadIdx.x, x, jx, m, S);
+    else if (threadIdx.x >= 64 && threadIdx.x < 67) c2j_component(n, threadIdx.x - 64, v, jv, m, S);
+    __syncthreads();
+    const double eps3 = A.eps * A.eps * A.eps;
+    accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, 0.5 * A.dt, A.method);
+}
+
+// `nsteps` steps of the main loop (src/integrator_whfast.c:290-378).
+__global__ __launch_bounds__(WF_BLOCK) void k_whfast_steps(WfArgs A)
+{
+    __shared__ WfShared S;
+    __shared__ int flags[WF_MAX_N];
+    const WfSys sy = A.sys[blockIdx.x];
+    const long long o = sy.off;
+    int n = A.count[blockIdx.x];
+    double *x = A.x + 3 * o, *jx = A.jx + 3 * o, *jv = A.jv + 3 * o;
+    double *jx2 = A.jx2 + 3 * o, *jv2 = A.jv2 + 3 * o;
+    double *m = A.m + o, *m2 = A.m2 + o;
+    int *ids = A.ids + o, *ids2 = A.ids2 + o;
+    const double eps3 = A.eps * A.eps * A.eps;
+    for (long long s = A.step0; s < A.step0 + A.nsteps; s++) {
+        const double dt = (s == A.total_steps - 1) ? A.dt_last : A.dt;
+        sort_by_distance(S, n, jx, jv, m, ids, jx2, jv2, m2, ids2);
+        massive_list(S, n, m, A.method);
+        fill_eta(S, n);
+        if (threadIdx.x == 0) S.any_remove = 0;
+        for (int i = threadIdx.x; i < n; i += WF_BLOCK) flags[i] = 0;
+        __syncthreads();
+        const double gm0 = A.G * m[0];
+        for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
+            const double gm = gm0 * S.key[i] / S.key[i - 1];
+            const int f = kepler_drift(&jx[3 * i], &jv[3 * i], gm, dt);
+            if (f & 2) atomicOr(&A.status[blockIdx.x], 1);
+            if ((f & 1) && A.remove_invalid) {
+                flags[i] = 1;
+                S.any_remove = 1;
+            }
+        }
+        __syncthreads();
+        if (S.any_remove) {
+            n = compact(S, n, flags, jx, jv, m, ids, jx2, jv2, m2, ids2);
+            massive_list(S, n, m, A.method);
+            fill_eta(S, n);
+        }
+        if (threadIdx.x < 3) j2c_component(n, threadIdx.x, jx, x, m, S);
+        // Cartesian velocities only matter for the state returned to the
+        // caller: rebuild them (from the drifted, not yet kicked, Jacobi
+        // velocities, as the reference's last jacobi_to_cartesian does) on
+        // the last step of each launch, in a second wavefront
+        if (s == A.step0 + A.nsteps - 1 && threadIdx.x >= 64 && threadIdx.x < 67)
+            j2c_component(n, threadIdx.x - 64, jv, A.v + 3 * o, m, S);
+        __syncthreads();
+        accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, dt, A.method);
+    }
+    if (threadIdx.x == 0) A.count[blockIdx.x] = n;
+}
+
+// ---------------------------------------------------------------- host
+ErrorStatus whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, double *x, double *v,
+                            double *m, double G, int method, double eps, double dt, double tf,
+                            bool remove_invalid, int *out_counts, int64 *steps_done,
+                            const bool *is_exit)
+{
+    if (num_systems <= 0 || !offsets || !ids || !x || !v || !m || !out_counts)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument or no systems");
+    if (method != ACCELERATION_METHOD_PAIRWISE && method != ACCELERATION_METHOD_MASSLESS)
+        return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR,
+                                "Invalid acceleration method for WHFast integrator. Only pairwise "
+                                "and massless methods are supported.");
+    if (dt <= 0.0) return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "dt must be positive. Got: %g", dt);
+    std::vector<WfSys> sys(num_systems);
+    for (int s = 0; s < num_systems; s++) {
+        const long long n = offsets[s + 1] - offsets[s];
+        if (n <= 0 || n > WF_MAX_N)
+            return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
+                                        "System %d has %lld particles; the GPU WHFast supports "
+                                        "1..%d per system",
+                                        s, n, WF_MAX_N);
+        int nm = 0;
+        for (long long i = offsets[s]; i < offsets[s + 1]; i++)
+            nm += (method == ACCELERATION_METHOD_PAIRWISE || m[i] != 0.0) ? 1 : 0;
+        if (nm > WF_MAX_MASSIVE)
+            return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
+                                        "System %d has %d massive particles; the GPU WHFast "
+                                        "supports up to %d",
+                                        s, nm, WF_MAX_MASSIVE);
+        sys[s].off = offsets[s];
+        sys[s].cap = static_cast<int>(n);
+    }
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const long long P = offsets[num_systems] - offsets[0];
+    const long long base = offsets[0];
+    for (auto &s : sys) s.off -= base;
+    // one allocation: 7 x 3P doubles (x, v, jx, jv, jx2, jv2) + 2P doubles (m, m2)
+    // + 2P ints (ids, ids2) + per-system arrays
+    const size_t b3 = sizeof(double) * 3 * P;
+    // (each of the 13 sub-buffers below is rounded up to 256 bytes)
+    const size_t bytes = 6 * b3 + 2 * sizeof(double) * P + 2 * sizeof(int) * P +
+                         sizeof(WfSys) * num_systems + 2 * sizeof(int) * num_systems + 13 * 256;
+    char *buf = nullptr;
+    GS_HIP(hipMalloc(&buf, bytes));
+    WfArgs A;
+    char *p = buf;
+    auto take = [&](size_t b) {
+        char *r = p;
+        p += (b + 255) & ~static_cast<size_t>(255);
+        return r;
+    };
+    // p never passes buf + bytes: 13 takes of their size rounded up to 256
+    A.x = reinterpret_cast<double *>(take(b3));
+    A.v = reinterpret_cast<double *>(take(b3));
+    A.jx = reinterpret_cast<double *>(take(b3));
+    A.jv = reinterpret_cast<double *>(take(b3));
+    A.jx2 = reinterpret_cast<double *>(take(b3));
+    A.jv2 = reinterpret_cast<double *>(take(b3));
+    A.m = reinterpret_cast<double *>(take(sizeof(double) * P));
+    A.m2 = reinterpret_cast<double *>(take(sizeof(double) * P));
+    A.ids = reinterpret_cast<int *>(take(sizeof(int) * P));
+    A.ids2 = reinterpret_cast<int *>(take(sizeof(int) * P));
+    WfSys *dsys = reinterpret_cast<WfSys *>(take(sizeof(WfSys) * num_systems));
+    A.count = reinterpret_cast<int *>(take(sizeof(int) * num_systems));
+    A.status = reinterpret_cast<int *>(take(sizeof(int) * num_systems));
+    A.sys = dsys;
+    std::vector<int> cnt(num_systems);
+    for (int s = 0; s < num_systems; s++) cnt[s] = sys[s].cap;
+    ErrorStatus st = make_success_error_status();
+    hipError_t e = hipMemcpyAsync(A.x, x + 3 * base, b3, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(A.v, v + 3 * base, b3, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(A.m, m + base, sizeof(double) * P, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(A.ids, ids + base, sizeof(int) * P, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(dsys, sys.data(), sizeof(WfSys) * num_systems, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemcpyAsync(A.count, cnt.data(), sizeof(int) * num_systems, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemsetAsync(A.status, 0, sizeof(int) * num_systems, c->stream);
+    if (!e) e = hipMemsetAsync(A.jx, 0, b3, c->stream);
+    A.G = G;
+    A.eps = eps;
+    A.dt = dt;
+    A.method = method;
+    A.remove_invalid = remove_invalid ? 1 : 0;
+    // dt clamp of the last step (src/integrator_whfast.c:292-296)
+    const long long total = static_cast<long long>(ceil(tf / dt));
+    A.total_steps = total;
+    A.dt_last = dt;
+    if (total > 0) {
+        const double t_last = static_cast<double>(total - 1) * dt;
+        if (t_last + dt > tf) A.dt_last = tf - t_last;
+    }
+    A.step0 = 0;
+    A.nsteps = 0;
+    if (!e) {
+        const int rec = timing_begin(c, "whfast_init");
+        hipLaunchKernelGGL(k_whfast_init, dim3(num_systems), dim3(WF_BLOCK), 0, c->stream, A);
+        timing_end(c, rec);
+        e = hipGetLastError();
+    }
+    const long long chunk = 256;
+    long long done = 0;
+    while (!e && done < total) {
+        A.step0 = done;
+        A.nsteps = std::min(chunk, total - done);
+        const int rec = timing_begin(c, "whfast_steps");
+        hipLaunchKernelGGL(k_whfast_steps, dim3(num_systems), dim3(WF_BLOCK), 0, c->stream, A);
+        timing_end(c, rec);
+        e = hipGetLastError();
+        done += A.nsteps;
+        if (!e) e = hipStreamSynchronize(c->stream);
+        if (is_exit && *is_exit) break;
+    }
+    std::vector<int> stat(num_systems);
+    if (!e) e = hipMemcpyAsync(cnt.data(), A.count, sizeof(int) * num_systems, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(stat.data(), A.status, sizeof(int) * num_systems, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(x + 3 * base, A.x, b3, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(v + 3 * base, A.v, b3, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(m + base, A.m, sizeof(double) * P, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipMemcpyAsync(ids + base, A.ids, sizeof(int) * P, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e) return hip_error(e, "GPU WHFast", __FILE__, __LINE__, __func__);
+    for (int s = 0; s < num_systems; s++) out_counts[s] = cnt[s];
+    if (steps_done) *steps_done = done;
+    return st;
+}
 
 ErrorStatus whfast_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
                        SimulationStatus *status, Settings *settings, double tf)
 {
-    (void)system; (void)ip; (void)ap; (void)status; (void)settings; (void)tf;
-    return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR, "GPU WHFast is not available yet");
+    const int64 off[2] = {0, system->num_particles};
+    int count = 0;
+    int64 steps = 0;
+    GS_TRY(whfast_ensemble(1, off, system->particle_ids, system->x, system->v, system->m,
+                           system->G, ap->method, ap->softening_length, ip->dt, tf,
+                           ip->whfast_remove_invalid_particles, &count, &steps,
+                           settings->is_exit_ptr));
+    system->num_particles = count;
+    status->num_steps = steps;
+    const long long total = static_cast<long long>(ceil(tf / ip->dt));
+    double dt = ip->dt;
+    if (total > 0 && steps == total) {
+        const double t_last = static_cast<double>(total - 1) * dt;
+        if (t_last + dt > tf) dt = tf - t_last;
+    }
+    status->dt = dt;
+    status->t = static_cast<double>(steps) * dt;
+    return make_success_error_status();
 }
 
 }  // namespace gs
+
+extern "C" ErrorStatus gs_whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids,
+                                          double *x, double *v, double *m, double G, int method,
+                                          double eps, double dt, double tf, bool remove_invalid,
+                                          int *out_counts)
+{
+    return gs::whfast_ensemble(num_systems, offsets, ids, x, v, m, G, method, eps, dt, tf,
+                               remove_invalid, out_counts, nullptr, nullptr);
+}

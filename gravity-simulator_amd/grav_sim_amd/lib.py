@@ -31,7 +31,7 @@ EXPORTS = [
     "gs_sim_create", "gs_sim_destroy", "gs_sim_upload", "gs_sim_download", "gs_sim_acceleration",
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
-    "gs_set_fp32_config", "gs_copy_calibrate",
+    "gs_set_fp32_config", "gs_copy_calibrate", "gs_whfast_ensemble",
 ]
 
 
@@ -94,6 +94,8 @@ def load(path=None):
         "gs_autotune_fp32": (ES, [C.c_int, PI, PI, PI]),
         "gs_set_fp32_config": (ES, [C.c_int, C.c_int, C.c_int]),
         "gs_copy_calibrate": (ES, [C.c_size_t, C.c_int, PD]),
+        "gs_whfast_ensemble": (ES, [C.c_int, PI64, PI, PD, PD, PD, C.c_double, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_bool, PI]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -308,3 +310,23 @@ def launch_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, theta=1
                               float(tf)), "launch_simulation")
     k = s.num_particles
     return ids[:k].copy(), x[:k].copy(), v[:k].copy(), m[:k].copy(), st.num_steps
+
+
+def whfast_ensemble(offsets, x, v, m, G, dt, tf, method=2, eps=0.0, remove_invalid=True, ids=None):
+    """Batched GPU WHFast.  Returns a list of (ids, x, v, m) per system."""
+    L = load()
+    offsets = _c(offsets, np.int64)
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m).copy()
+    ns = len(offsets) - 1
+    if ids is None:
+        ids = np.concatenate([np.arange(offsets[s + 1] - offsets[s], dtype=np.int32) for s in range(ns)])
+    ids = _c(ids, np.int32).copy()
+    cnt = np.zeros(ns, dtype=np.int32)
+    check(L.gs_whfast_ensemble(ns, offsets.ctypes.data_as(PI64), _abi.iptr(ids), _abi.dptr(x), _abi.dptr(v),
+                               _abi.dptr(m), float(G), int(method), float(eps), float(dt), float(tf),
+                               bool(remove_invalid), _abi.iptr(cnt)), "gs_whfast_ensemble")
+    out = []
+    for s in range(ns):
+        o, k = int(offsets[s]), int(cnt[s])
+        out.append((ids[o:o + k].copy(), x[o:o + k].copy(), v[o:o + k].copy(), m[o:o + k].copy()))
+    return out

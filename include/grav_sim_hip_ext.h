@@ -86,6 +86,20 @@ ErrorStatus gs_sim_energy(GsSim *sim, double *energy);
 ErrorStatus gs_sim_accel_error(GsSim *sim, const AccelerationParam *param, int num_samples,
                                double *median, double *p99, double *max);
 
+/* ---- batched WHFast ------------------------------------------------------------
+ * Integrates num_systems independent systems with WHFast
+ * (src/integrator_whfast.c:200-407 semantics, one GPU workgroup per system).
+ * System s holds particles offsets[s] .. offsets[s+1]-1 of the flat arrays
+ * (x, v: N x 3; m, ids: N), with ids local to the system (the particle with
+ * id 0 is the primary the reference sorts distances from).  On return each
+ * system's survivors are stored, in the reference's final order, from
+ * offsets[s] onwards and out_counts[s] gives their number.  method is
+ * ACCELERATION_METHOD_PAIRWISE or _MASSLESS; up to 2048 particles and 64
+ * massive particles per system. */
+ErrorStatus gs_whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, double *x,
+                               double *v, double *m, double G, int method, double eps, double dt,
+                               double tf, bool remove_invalid, int *out_counts);
+
 /* ---- octree pieces ----------------------------------------------------------
  * Morton keys (sorted) and the stable sort permutation of system->x, computed
  * on the GPU (src/linear_octree.c:113-326).  Arrays are N long. */
