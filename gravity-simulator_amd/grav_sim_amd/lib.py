@@ -32,6 +32,8 @@ EXPORTS = [
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
     "gs_set_fp32_config", "gs_copy_calibrate", "gs_whfast_ensemble",
+    "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
+    "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
 ]
 
 

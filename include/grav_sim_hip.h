@@ -336,6 +336,41 @@ void compute_energy_python(double *GS_RESTRICT energy, const double G,
                            const double *GS_RESTRICT sol_state, const int32 num_snapshots,
                            const int32 num_particles);
 
+/* replaces src/python_interface.h:9-11 */
+void free_memory_int32(int32 *GS_RESTRICT ptr);
+void free_memory_double(double *GS_RESTRICT ptr);
+
+/* replaces src/python_interface.h:27-35.  Built-in initial conditions are not
+ * provided: prints a traceback and returns GRAV_NOT_IMPLEMENTED_ERROR. */
+int32 load_built_in_system_python(const char *system_name, int *num_particles_ptr,
+                                  int32 **particle_ids_ptr, double **x_ptr, double **v_ptr,
+                                  double **m_ptr, double *G_ptr);
+
+/* replaces src/utils.h keplerian_to_cartesian (src/utils.c:82-141) */
+void keplerian_to_cartesian(double *GS_RESTRICT x, double *GS_RESTRICT v,
+                            const double semi_major_axis, const double eccentricity,
+                            const double inclination, const double argument_of_periapsis,
+                            const double longitude_of_ascending_node, const double true_anomaly,
+                            const double total_mass, const double G);
+
+/* replaces src/python_interface.h:52-67 */
+void keplerian_to_cartesian_python(double *GS_RESTRICT x, double *GS_RESTRICT y,
+                                   double *GS_RESTRICT z, double *GS_RESTRICT v_x,
+                                   double *GS_RESTRICT v_y, double *GS_RESTRICT v_z,
+                                   const double semi_major_axis, const double eccentricity,
+                                   const double inclination, const double argument_of_periapsis,
+                                   const double longitude_of_ascending_node,
+                                   const double true_anomaly, const double total_mass,
+                                   const double G);
+
+/* replaces src/python_interface.h:144-164 (src/python_interface.c:245-304) */
+void compute_linear_momentum_python(double *GS_RESTRICT linear_momentum,
+                                    const double *GS_RESTRICT sol_state, const int32 num_snapshots,
+                                    const int32 num_particles);
+void compute_angular_momentum_python(double *GS_RESTRICT angular_momentum,
+                                     const double *GS_RESTRICT sol_state,
+                                     const int32 num_snapshots, const int32 num_particles);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,3 +152,26 @@ def test_no_device_fails_loudly():
         capture_output=True, text=True, timeout=120)
     assert "RAISED True" in r.stdout, r.stdout + r.stderr
     del np
+
+
+def test_python_api_symbols_and_host_helpers():
+    """Every restype grav_sim/utils.py:38-58 sets resolves, and the host helpers
+    agree with the reference formulae."""
+    import numpy as np
+    lib = L.load()
+    for name in ("launch_simulation_python", "compute_energy_python", "compute_linear_momentum_python",
+                 "compute_angular_momentum_python", "load_built_in_system_python", "free_memory_int32",
+                 "free_memory_double", "keplerian_to_cartesian_python"):
+        assert hasattr(lib, name), name
+    out = [C.c_double() for _ in range(6)]
+    lib.keplerian_to_cartesian_python.argtypes = [C.POINTER(C.c_double)] * 6 + [C.c_double] * 8
+    lib.keplerian_to_cartesian_python(*[C.byref(o) for o in out], 1.5, 0.1, 0.2, 0.3, 0.4, 0.5, 1.0, 1.0)
+    from grav_sim_amd import ic
+    x, v = ic.keplerian_to_cartesian(np.array(1.5), np.array(0.1), np.array(0.2), np.array(0.3), np.array(0.4),
+                                     np.array(0.5), 1.0, 1.0)
+    assert np.allclose([o.value for o in out], np.concatenate([x, v]), rtol=1e-14, atol=1e-15)
+    sol = np.random.default_rng(0).normal(size=(2, 5, 7))
+    lm = np.zeros(2)
+    lib.compute_linear_momentum_python.argtypes = [_abi.PD, _abi.PD, C.c_int, C.c_int]
+    lib.compute_linear_momentum_python(_abi.dptr(lm), _abi.dptr(np.ascontiguousarray(sol)), 2, 5)
+    assert np.allclose(lm, (sol[..., 0] * sol[..., 4:7].sum(-1)).sum(-1), rtol=1e-14)
