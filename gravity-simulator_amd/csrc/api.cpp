@@ -40,13 +40,14 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
             GS_HIP(launch_direct_f32(c, x, m, n, G, eps, a, part, pb, cfg));
             return make_success_error_status();
         }
-        case ACCELERATION_METHOD_BARNES_HUT: {
+        case ACCELERATION_METHOD_BARNES_HUT:
+        case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE: {
             DeviceTree *t;
             GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, &t));
-            GS_TRY(tree_walk(c, t, x, G, eps, p->opening_angle, a, &g_last_walk_interactions));
+            GS_TRY(tree_walk(c, t, x, G, eps, p->opening_angle, a, &g_last_walk_interactions,
+                             p->method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE ? 2 : 1));
             return make_success_error_status();
         }
-        case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE:
         case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
             return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
                                         "Acceleration method %d is not available yet",

@@ -135,9 +135,12 @@ struct DeviceTree;  // defined in octree.hip
 // box_center/box_width: NULL/<=0 -> bounding box.
 ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
                        const double *box_center, double box_width, DeviceTree **out);
-// Walk; writes a (device, AoS, original order).
+// Walk; writes a (device, AoS, original order).  order 1: monopoles (the
+// reference's), 2: with quadrupole terms.  feat != NULL: monopole walk that
+// also writes 8 FP32 features per particle for the learned correction.
 ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, double eps,
-                      double theta, double *a, long long *interactions);
+                      double theta, double *a, long long *interactions, int order = 1,
+                      float *feat = nullptr);
 // Host copy of the tree in the reference struct layout.
 ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out);
 

@@ -35,6 +35,7 @@ struct DeviceTree {
     double4 *P = nullptr;           // device: (x, y, z, m) in sorted order
     NodeRec *nodes = nullptr;       // device: level-ordered nodes
     double4 *mom = nullptr;         // device: (M, sum m x, sum m y, sum m z)
+    double *quad = nullptr;         // device: 8 per node (xx yy zz xy xz yz, 0, 0)
     unsigned *tmp = nullptr;        // device: per-parent child counts
     unsigned *scan_tmp = nullptr;   // device: scan scratch
 };

@@ -294,7 +294,8 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
 
 // moments: mom[node] = (sum m, sum m x, sum m y, sum m z)
 __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
-                            const double4 *__restrict__ P, double4 *__restrict__ mom)
+                            const double4 *__restrict__ P, double4 *__restrict__ mom,
+                            double *__restrict__ quad)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= le - lb) return;
@@ -331,6 +332,45 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         nd.cy = p.y;
         nd.cz = p.z;
     }
+    // second central moments Q_ab = sum m (x - com)_a (x - com)_b
+    // (xx yy zz xy xz yz): leaves from their particles, internal nodes from
+    // their children shifted to this centre of mass (parallel axes)
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
+    const double c0 = nd.cx, c1 = nd.cy, c2 = nd.cz;
+    if (nd.nchild == 0) {
+        for (int s = nd.first; s < nd.first + nd.count; s++) {
+            const double4 p = P[s];
+            const double d0 = p.x - c0, d1 = p.y - c1, d2 = p.z - c2;
+            q0 += p.w * d0 * d0;
+            q1 += p.w * d1 * d1;
+            q2 += p.w * d2 * d2;
+            q3 += p.w * d0 * d1;
+            q4 += p.w * d0 * d2;
+            q5 += p.w * d1 * d2;
+        }
+    } else {
+        for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
+            const NodeRec &ch = nodes[c];
+            const double d0 = ch.cx - c0, d1 = ch.cy - c1, d2 = ch.cz - c2;
+            const double Mc = ch.mass;
+            const double *qc = quad + 8 * static_cast<size_t>(c);
+            q0 += qc[0] + Mc * d0 * d0;
+            q1 += qc[1] + Mc * d1 * d1;
+            q2 += qc[2] + Mc * d2 * d2;
+            q3 += qc[3] + Mc * d0 * d1;
+            q4 += qc[4] + Mc * d0 * d2;
+            q5 += qc[5] + Mc * d1 * d2;
+        }
+    }
+    double *qn = quad + 8 * static_cast<size_t>(node);
+    qn[0] = q0;
+    qn[1] = q1;
+    qn[2] = q2;
+    qn[3] = q3;
+    qn[4] = q4;
+    qn[5] = q5;
+    qn[6] = 0.0;
+    qn[7] = 0.0;
 }
 
 // ------------------------------------------------------------------ walk
@@ -358,12 +398,43 @@ __device__ __forceinline__ void pp_force(double xi0, double xi1, double xi2, dou
     a2 = __builtin_fma(-w, R2, a2);
 }
 
+// Quadrupole term of an accepted node (R = x_i - com, r2 = |R|^2 + eps^2),
+// without the factor G: 3 Q R / r^5 - (15/2) (R.Q.R) R / r^7 + (3/2) tr(Q) R / r^5.
+__device__ __forceinline__ void quad_term(const double *__restrict__ q, double R0, double R1,
+                                          double R2, float ri, float &o0, float &o1, float &o2)
+{
+    const float r0 = static_cast<float>(R0), r1 = static_cast<float>(R1),
+                r2 = static_cast<float>(R2);
+    const float qxx = static_cast<float>(q[0]), qyy = static_cast<float>(q[1]),
+                qzz = static_cast<float>(q[2]), qxy = static_cast<float>(q[3]),
+                qxz = static_cast<float>(q[4]), qyz = static_cast<float>(q[5]);
+    const float QR0 = __builtin_fmaf(qxz, r2, __builtin_fmaf(qxy, r1, qxx * r0));
+    const float QR1 = __builtin_fmaf(qyz, r2, __builtin_fmaf(qyy, r1, qxy * r0));
+    const float QR2 = __builtin_fmaf(qzz, r2, __builtin_fmaf(qyz, r1, qxz * r0));
+    const float RQR = __builtin_fmaf(r2, QR2, __builtin_fmaf(r1, QR1, r0 * QR0));
+    const float tr = qxx + qyy + qzz;
+    const float ir2 = ri * ri;
+    const float ir5 = ir2 * ir2 * ri;
+    const float sc = ir5 * __builtin_fmaf(-7.5f * RQR, ir2, 1.5f * tr);
+    const float k3 = 3.0f * ir5;
+    o0 = __builtin_fmaf(k3, QR0, sc * r0);
+    o1 = __builtin_fmaf(k3, QR1, sc * r1);
+    o2 = __builtin_fmaf(k3, QR2, sc * r2);
+}
+
+// QUAD: add the quadrupole term of accepted nodes.  FEAT: do not add it but
+// accumulate per-target features of the walk for the learned correction:
+// (quadrupole sum x3, sum M l^2 / d^4, sum M l^3 / d^5, accepted nodes,
+// particle interactions, 0), in units without G, FP32, original order.
+template <bool QUAD, bool FEAT>
 __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ nodes,
                                                    const double4 *__restrict__ P,
                                                    const long long *__restrict__ skeys,
                                                    const int *__restrict__ sidx, int n,
                                                    WalkArgs args, double *__restrict__ a,
-                                                   unsigned long long *__restrict__ n_int)
+                                                   unsigned long long *__restrict__ n_int,
+                                                   const double *__restrict__ quad,
+                                                   float *__restrict__ feat)
 {
     __shared__ int stk_node[WK_WAVES][WK_STACK];
     __shared__ unsigned long long stk_mask[WK_WAVES][WK_STACK];
@@ -385,6 +456,8 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
     }
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     unsigned cnt = 0;
+    float fq0 = 0.0f, fq1 = 0.0f, fq2 = 0.0f, fs2 = 0.0f, fs3 = 0.0f;
+    unsigned nacc = 0, npp = 0;
     const unsigned long long me = 1ull << lane;
     int *snode = stk_node[wave];
     unsigned long long *smask = stk_mask[wave];
@@ -418,6 +491,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                 const double4 p = P[nd.first];
                 pp_force(xi0, xi1, xi2, p.x, p.y, p.z, p.w, args.eps2, a0, a1, a2);
                 cnt++;
+                npp++;
             }
             continue;
         }
@@ -430,11 +504,31 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
             const double l2 = __builtin_amdgcn_ldexp(args.w2, -2 * nd.depth);
             if (!included && l2 < args.theta2 * d2) {
                 const float ri = __builtin_amdgcn_rsqf(static_cast<float>(d2 + args.eps2));
-                const double w = static_cast<double>(static_cast<float>(nd.mass) * (ri * ri * ri));
+                const float mr3 = static_cast<float>(nd.mass) * (ri * ri * ri);
+                const double w = static_cast<double>(mr3);
                 a0 = __builtin_fma(-w, R0, a0);
                 a1 = __builtin_fma(-w, R1, a1);
                 a2 = __builtin_fma(-w, R2, a2);
                 cnt++;
+                if (QUAD || FEAT) {
+                    float o0, o1, o2;
+                    quad_term(quad + 8 * static_cast<size_t>(node), R0, R1, R2, ri, o0, o1, o2);
+                    if (QUAD) {
+                        a0 += static_cast<double>(o0);
+                        a1 += static_cast<double>(o1);
+                        a2 += static_cast<double>(o2);
+                    } else {
+                        fq0 += o0;
+                        fq1 += o1;
+                        fq2 += o2;
+                        // M l^2 / d^4 and M l^3 / d^5 (l: cell side, d: distance to the COM)
+                        const float lr = static_cast<float>(__builtin_sqrt(l2)) * ri;
+                        const float mr2 = mr3 * static_cast<float>(__builtin_sqrt(d2 + args.eps2));
+                        fs2 = __builtin_fmaf(mr2, lr * lr, fs2);
+                        fs3 = __builtin_fmaf(mr2, lr * lr * lr, fs3);
+                        nacc++;
+                    }
+                }
             } else {
                 open = true;
             }
@@ -448,6 +542,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                 if (mine && t != s) {
                     pp_force(xi0, xi1, xi2, p.x, p.y, p.z, p.w, args.eps2, a0, a1, a2);
                     cnt++;
+                    npp++;
                 }
             }
         } else if (sp + nd.nchild <= WK_STACK) {
@@ -470,6 +565,17 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         a[3 * i + 0] = args.G * a0;
         a[3 * i + 1] = args.G * a1;
         a[3 * i + 2] = args.G * a2;
+        if (FEAT) {
+            float *f = feat + 8 * static_cast<size_t>(i);
+            f[0] = fq0;
+            f[1] = fq1;
+            f[2] = fq2;
+            f[3] = fs2;
+            f[4] = fs3;
+            f[5] = static_cast<float>(nacc);
+            f[6] = static_cast<float>(npp);
+            f[7] = 0.0f;
+        }
     }
     // interaction count: one atomic per wave
     unsigned long long tot = cnt;
@@ -497,6 +603,10 @@ static ErrorStatus ensure_nodes(Context *c, DeviceTree *t, int need)
     GS_HIP(hipMalloc(reinterpret_cast<void **>(&nm), sizeof(double4) * cap));
     if (t->mom) GS_HIP(hipFree(t->mom));
     t->mom = nm;
+    double *nq = nullptr;
+    GS_HIP(hipMalloc(reinterpret_cast<void **>(&nq), sizeof(double) * 8 * cap));
+    if (t->quad) GS_HIP(hipFree(t->quad));
+    t->quad = nq;
     unsigned *tc = nullptr;
     GS_HIP(hipMalloc(reinterpret_cast<void **>(&tc), sizeof(unsigned) * (cap + 1)));
     if (t->tmp) GS_HIP(hipFree(t->tmp));
@@ -651,7 +761,7 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
         const int lb = t->level_begin[d], le = t->level_end[d];
         if (le <= lb) continue;
         hipLaunchKernelGGL(k_multipole, dim3((le - lb + 255) / 256), dim3(256), 0, c->stream,
-                           t->nodes, lb, le, t->P, t->mom);
+                           t->nodes, lb, le, t->P, t->mom, t->quad);
     }
     GS_HIP(hipGetLastError());
     timing_end(c, rec);
@@ -660,7 +770,7 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
 }
 
 ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, double eps,
-                      double theta, double *a, long long *interactions)
+                      double theta, double *a, long long *interactions, int order, float *feat)
 {
     (void)x;
     WalkArgs args;
@@ -675,8 +785,15 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
     const int waves = (t->n + 63) / 64;
     const int grid = (waves + WK_WAVES - 1) / WK_WAVES;
     const int rec = timing_begin(c, "bh_walk");
-    hipLaunchKernelGGL(k_walk, dim3(grid), dim3(WK_BLOCK), 0, c->stream, t->nodes, t->P, t->skeys,
-                       t->sidx, t->n, args, a, cnt);
+    if (feat)
+        hipLaunchKernelGGL((k_walk<false, true>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quad, feat);
+    else if (order >= 2)
+        hipLaunchKernelGGL((k_walk<true, false>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quad, feat);
+    else
+        hipLaunchKernelGGL((k_walk<false, false>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quad, feat);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     if (interactions) {

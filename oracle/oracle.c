@@ -259,7 +259,29 @@ typedef struct
     const int *depth, *count, *first, *nchild, *child0;
     const int64_t *prefix;
     const double *mass, *comx, *comy, *comz;
+    const double *quad; /* 6 per node (xx yy zz xy xz yz) or NULL: monopole only */
 } Walk;
+
+/* Quadrupole term of an accepted node at R = x_i - com (r^2 = |R|^2 + eps^2):
+ *   a += G [ 3 Q R / r^5 - (15/2) (R.Q.R) R / r^7 + (3/2) tr(Q) R / r^5 ]
+ * the gradient of the second-order term of the potential
+ *   phi = -G [ M / r + (3 R.Q.R - tr(Q) r^2) / (2 r^5) ],
+ * Q_ab = sum m (x - com)_a (x - com)_b. */
+static void quad_accel(double acc[3], const double *q, double R0, double R1, double R2, double r2,
+                       double G)
+{
+    const double QR0 = q[0] * R0 + q[3] * R1 + q[4] * R2;
+    const double QR1 = q[3] * R0 + q[1] * R1 + q[5] * R2;
+    const double QR2 = q[4] * R0 + q[5] * R1 + q[2] * R2;
+    const double RQR = R0 * QR0 + R1 * QR1 + R2 * QR2;
+    const double tr = q[0] + q[1] + q[2];
+    const double ir2 = 1.0 / r2;
+    const double ir5 = ir2 * ir2 / sqrt(r2);
+    const double s = 1.5 * tr * ir5 - 7.5 * RQR * ir5 * ir2;
+    acc[0] += G * (3.0 * QR0 * ir5 + s * R0);
+    acc[1] += G * (3.0 * QR1 * ir5 + s * R1);
+    acc[2] += G * (3.0 * QR2 * ir5 + s * R2);
+}
 
 static void walk_node(const Walk *w, int node, int i, int64_t key_i, const double xi[3],
                       double acc[3], int64_t *nint)
@@ -281,6 +303,8 @@ static void walk_node(const Walk *w, int node, int i, int64_t key_i, const doubl
             acc[0] -= t * R0;
             acc[1] -= t * R1;
             acc[2] -= t * R2;
+            if (w->quad)
+                quad_accel(acc, &w->quad[6 * node], R0, R1, R2, d2 + w->eps2, w->G);
             (*nint)++;
             return;
         }
@@ -313,11 +337,12 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
                  double eps, double theta, double width, const int64_t *sorted_keys,
                  const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
                  const int *count, const int *first, const int *nchild, const int *child0,
-                 const double *mass, const double *comx, const double *comy, const double *comz)
+                 const double *mass, const double *comx, const double *comy, const double *comz,
+                 const double *quad)
 {
     Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
               depth, count, first,  nchild,    child0,        prefix, mass,        comx,
-              comy,  comz};
+              comy,  comz,  quad};
 #pragma omp parallel for schedule(dynamic, 64)
     for (int s = 0; s < n; s++)
     {
@@ -333,6 +358,50 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
         a[i * 3 + 2] = acc[2];
         if (n_int)
             n_int[i] = cnt;
+    }
+}
+
+void orc_octree_quad(int num, const double *x, const double *m, const int32_t *sorted_idx,
+                     const int *count, const int *first, const int *nchild, const int *child0,
+                     const double *mass, const double *comx, const double *comy,
+                     const double *comz, double *quad)
+{
+    for (int node = num - 1; node >= 0; node--)
+    {
+        double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double c[3] = {comx[node], comy[node], comz[node]};
+        if (nchild[node] == 0)
+        {
+            for (int t = first[node]; t < first[node] + count[node]; t++)
+            {
+                const int p = sorted_idx[t];
+                const double d0 = x[p * 3 + 0] - c[0], d1 = x[p * 3 + 1] - c[1],
+                             d2 = x[p * 3 + 2] - c[2];
+                q[0] += m[p] * d0 * d0;
+                q[1] += m[p] * d1 * d1;
+                q[2] += m[p] * d2 * d2;
+                q[3] += m[p] * d0 * d1;
+                q[4] += m[p] * d0 * d2;
+                q[5] += m[p] * d1 * d2;
+            }
+        }
+        else
+        {
+            for (int ch = child0[node]; ch < child0[node] + nchild[node]; ch++)
+            {
+                const double d0 = comx[ch] - c[0], d1 = comy[ch] - c[1], d2 = comz[ch] - c[2];
+                const double M = mass[ch];
+                const double *qc = &quad[6 * ch];
+                q[0] += qc[0] + M * d0 * d0;
+                q[1] += qc[1] + M * d1 * d1;
+                q[2] += qc[2] + M * d2 * d2;
+                q[3] += qc[3] + M * d0 * d1;
+                q[4] += qc[4] + M * d0 * d2;
+                q[5] += qc[5] + M * d1 * d2;
+            }
+        }
+        for (int k = 0; k < 6; k++)
+            quad[6 * node + k] = q[k];
     }
 }
 

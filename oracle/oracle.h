@@ -74,7 +74,18 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
                  double eps, double theta, double width, const int64_t *sorted_keys,
                  const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
                  const int *count, const int *first, const int *nchild, const int *child0,
-                 const double *mass, const double *comx, const double *comy, const double *comz);
+                 const double *mass, const double *comx, const double *comy, const double *comz,
+                 const double *quad);
+
+/* Second central moments per node, Q_ab = sum m (x - com)_a (x - com)_b, in
+ * the order xx yy zz xy xz yz: leaves over their particles, internal nodes
+ * from their children by the parallel-axis shift.  An extension (the
+ * reference keeps monopoles only); with quad != NULL, orc_bh_walk adds the
+ * quadrupole term of every accepted node. */
+void orc_octree_quad(int num, const double *x, const double *m, const int32_t *sorted_idx,
+                     const int *count, const int *first, const int *nchild, const int *child0,
+                     const double *mass, const double *comx, const double *comy,
+                     const double *comz, double *quad);
 
 /* Leapfrog KDK with compensated summation (src/integrator.c:894-1121), using
  * orc_pairwise (method 1) or orc_massless (method 2) for the accelerations.

@@ -125,3 +125,27 @@ def test_bh_config4_error_vs_fp64_direct():
     # monopole Barnes-Hut at theta=0.5 (the reference's multipole order)
     assert med < 5e-3 and p99 < 3e-2
     sim.close()
+
+
+def test_quadrupole_moments_identical_to_oracle():
+    """(the moments are not exported through LinearOctree; they are checked
+    through the quadrupole walk, which uses them for every accepted node)"""
+    x, v, m = ic.two_spheres(8192, 21)
+    t = O.build_tree(x, m, 1)
+    q = O.octree_quad(x, m, t)
+    for theta in (0.5, 0.8):
+        a_o, cnt = O.bh_walk(x, m, t, theta, want_counts=True, quad=q)
+        a_g = L.acceleration(x, m, 19, theta=theta, leaf=1)
+        assert L.last_walk_interactions() == int(cnt.sum())
+        e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+        assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (theta, np.median(e), e.max())
+
+
+def test_quadrupole_beats_monopole_config4():
+    x, v, m = ic.two_spheres(1 << 20, 4)
+    sim = L.Sim(x, v, m, 1.0)
+    mono = sim.accel_error(L.param(3, theta=0.5, leaf=1), samples=4096)
+    quad = sim.accel_error(L.param(19, theta=0.5, leaf=1), samples=4096)
+    print(f"N=2^20 theta=0.5 p99: monopole {mono[1]:.3e}, quadrupole {quad[1]:.3e}")
+    assert quad[1] < 0.5 * mono[1] and quad[0] < 0.5 * mono[0]
+    sim.close()

@@ -296,3 +296,47 @@ def test_whfast_golden(golden):
         # velocities by one constant vector; positions are unaffected
         d = f[f"v_{s}"] - v
         assert np.allclose(d, d.mean(axis=0), rtol=0, atol=1e-12 * (1.0 + np.abs(d).max()))
+
+
+def test_quadrupole_moments_long_double_and_two_point_closed_form():
+    x, v, m = ic.plummer(800, 14)
+    m = np.random.default_rng(2).uniform(0.5, 1.5, 800)
+    t = O.build_tree(x, m, 4)
+    q = O.octree_quad(x, m, t)
+    xl, ml = x.astype(np.longdouble), m.astype(np.longdouble)
+    for nd in range(0, len(t["depth"]), 7):
+        sel = t["sorted_idx"][t["first"][nd]:t["first"][nd] + t["count"][nd]]
+        c = np.array([t["comx"][nd], t["comy"][nd], t["comz"][nd]], dtype=np.longdouble)
+        d = xl[sel] - c
+        want = [(ml[sel] * d[:, a] * d[:, b]).sum() for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))]
+        scale = ml[sel].sum() * t["width"] ** 2
+        assert np.all(np.abs(q[nd] - np.array(want, dtype=np.float64)) <= 1e-12 * float(scale))
+
+
+def test_quadrupole_force_two_masses_on_axis():
+    # two masses m at +-d on z, target on the axis: exact force G m (1/(r-d)^2 + 1/(r+d)^2)
+    # = 2 G m / r^2 (1 + 3 d^2 / r^2 + O(d^4/r^4)); the monopole misses the 3 d^2/r^2 term
+    # (a fourth, massless particle keeps the target off the box maximum, which
+    # the Morton quantisation wraps to 0)
+    d, r = 0.01, 1.0
+    x = np.array([[0.0, 0.0, -d], [0.0, 0.0, d], [0.0, 0.0, r], [0.0, 0.0, 2.0]])
+    m = np.array([1.0, 1.0, 1e-30, 1e-30])
+    t = O.build_tree(x, m, 2)
+    q = O.octree_quad(x, m, t)
+    exact = 1.0 / (r - d) ** 2 + 1.0 / (r + d) ** 2
+    a_mono = O.bh_walk(x, m, t, 10.0)[2, 2]  # particle 2 is the target
+    a_quad = O.bh_walk(x, m, t, 10.0, quad=q)[2, 2]
+    assert abs(-a_mono - exact) > 1e-5
+    assert abs(-a_quad - exact) < 2e-7  # next term: 10 d^4 / r^6 = 1e-7
+
+
+def test_quadrupole_walk_beats_monopole():
+    x, v, m = ic.two_spheres(4096, 5)
+    t = O.build_tree(x, m, 1)
+    q = O.octree_quad(x, m, t)
+    a_d = O.pairwise(x, m)
+    an = np.linalg.norm(a_d, axis=1)
+    e1 = np.linalg.norm(O.bh_walk(x, m, t, 0.5) - a_d, axis=1) / an
+    e2 = np.linalg.norm(O.bh_walk(x, m, t, 0.5, quad=q) - a_d, axis=1) / an
+    assert np.percentile(e2, 99) < 0.5 * np.percentile(e1, 99)
+    assert np.median(e2) < 0.5 * np.median(e1)
