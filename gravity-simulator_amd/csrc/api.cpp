@@ -14,9 +14,8 @@
 namespace gs {
 
 hipError_t launch_copy(Context *c, const void *src, void *dst, size_t bytes);
-hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *m, int n,
-                                     double G, double eps, bool skip_massless,
-                                     const int *targets, int num_targets, double *out);
+ErrorStatus learned_acceleration(Context *c, const double *x, const double *m, int n, double G,
+                                 const AccelerationParam *p, double *a, long long *interactions);
 
 static long long g_last_walk_interactions = 0;
 
@@ -49,9 +48,7 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
             return make_success_error_status();
         }
         case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
-            return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
-                                        "Acceleration method %d is not available yet",
-                                        p->method);
+            return learned_acceleration(c, x, m, n, G, p, a, &g_last_walk_interactions);
         default:
             return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unknown acceleration method. Got: %d",
                                         p->method);

@@ -439,6 +439,79 @@ hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n
     return hipGetLastError();
 }
 
+// FP32 direct sum for a list of targets against all n sources (the learned
+// correction's labels and spot check).  Same kernels as launch_direct_f32
+// (kind 1): the targets' positions are gathered into a compact array and the
+// source range is split so that even a few hundred targets fill the chip.
+__global__ void k_gather_targets(const double *__restrict__ x, const int *__restrict__ idx,
+                                 int nt, double *__restrict__ xt)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 3 * nt) return;
+    xt[q] = x[3 * static_cast<size_t>(idx[q / 3]) + q % 3];
+}
+
+hipError_t launch_gather_rows3(Context *c, const double *src, const int *idx, int nt, double *out)
+{
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_targets, dim3((3 * nt + 255) / 256), dim3(256), 0, c->stream, src,
+                       idx, nt, out);
+    return hipGetLastError();
+}
+
+static int targets_split(int n, int nt)
+{
+    const int ntiles = f32s_tiles(n);
+    const int wgs = ((nt + 63) / 64 + 3) / 4;
+    int split = (2048 + wgs - 1) / wgs;
+    if (split > ntiles) split = ntiles;
+    return split < 1 ? 1 : split;
+}
+
+size_t direct_f32_targets_bytes(int n, int nt)
+{
+    const size_t nt3 = 3 * static_cast<size_t>(nt);
+    const size_t nt_tiles = f32s_tiles(n);
+    return 512 + nt3 * sizeof(double) * (1 + targets_split(n, nt)) + nt_tiles * ST_TILE * sizeof(float4) +
+           nt_tiles * sizeof(double4);
+}
+
+hipError_t launch_direct_f32_targets(Context *c, const double *x, const double *m, int n, double G,
+                                     double eps, const int *targets, int nt, double *out,
+                                     void *buf, size_t bytes)
+{
+    if (nt <= 0) return hipSuccess;
+    if (bytes < direct_f32_targets_bytes(n, nt)) return hipErrorInvalidValue;
+    const int ntiles = f32s_tiles(n);
+    const int n_pad = ntiles * ST_TILE;
+    const int split = targets_split(n, nt);
+    const int tpc = (ntiles + split - 1) / split;
+    const int nsplit = (ntiles + tpc - 1) / tpc;
+    const size_t nt3 = 3 * static_cast<size_t>(nt);
+    auto align = [](char *p) {
+        return reinterpret_cast<char *>((reinterpret_cast<uintptr_t>(p) + 255) & ~uintptr_t(255));
+    };
+    char *p = align(static_cast<char *>(buf));
+    double *xt = reinterpret_cast<double *>(p);
+    double *partial = xt + nt3;
+    char *tail = align(reinterpret_cast<char *>(partial + static_cast<size_t>(nsplit) * nt3));
+    float4 *S = reinterpret_cast<float4 *>(tail);
+    double4 *C = reinterpret_cast<double4 *>(tail + static_cast<size_t>(n_pad) * sizeof(float4));
+    hipLaunchKernelGGL(k_gather_targets, dim3((3 * nt + 255) / 256), dim3(256), 0, c->stream, x,
+                       targets, nt, xt);
+    hipLaunchKernelGGL(k_f32_sources, dim3((n_pad + 255) / 256), dim3(256), 0, c->stream, x, m, n,
+                       n_pad, S, C);
+    const float eps2 = static_cast<float>(eps * eps);
+    const int waves = (nt + 63) / 64;
+    dim3 grid((waves + 3) / 4, nsplit);
+    launch_f32s_shape<1>(c, grid, xt, nt, S, C, eps2, eps2 > 0.0f, tpc, ntiles, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((3 * nt + 255) / 256), dim3(256), 0, c->stream,
+                       partial, nsplit, 3 * nt, G, out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ copy
 // Plain streaming copy (16 B per lane per iteration), used to calibrate the
 // HBM byte counters and to measure achievable copy bandwidth.

@@ -53,7 +53,7 @@ enum Slot : int {
     SLOT_BBOX, SLOT_KEYS, SLOT_KEYS_ALT, SLOT_IDX, SLOT_IDX_ALT, SLOT_HIST, SLOT_HIST_SCAN,
     SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_SORTED_P, SLOT_MISC,
     // learned correction
-    SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD,
+    SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD, SLOT_MLP_X, SLOT_SPOT,
     SLOT_COUNT
 };
 
@@ -115,6 +115,17 @@ hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n
                              double eps, double *a, void *partial_buf, size_t partial_bytes,
                              const Fp32Config &cfg);
 size_t direct_f32_partial_bytes(int n, const Fp32Config &cfg, int cu_count);
+// Same arithmetic for the listed targets only: out[t] = a(targets[t]).
+hipError_t launch_direct_f32_targets(Context *c, const double *x, const double *m, int n, double G,
+                                     double eps, const int *targets, int nt, double *out,
+                                     void *buf, size_t bytes);
+size_t direct_f32_targets_bytes(int n, int nt);
+// out[t] = src[idx[t]] for rows of 3 doubles.
+hipError_t launch_gather_rows3(Context *c, const double *src, const int *idx, int nt, double *out);
+// FP64 reference-order sums for the listed targets.
+hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *m, int n,
+                                     double G, double eps, bool skip_massless, const int *targets,
+                                     int num_targets, double *out);
 Fp32Config &fp32_config();
 
 // FP64 energy (src/utils.c:27-59 formula; reduction order differs).

@@ -423,9 +423,11 @@ __device__ __forceinline__ void quad_term(const double *__restrict__ q, double R
 }
 
 // QUAD: add the quadrupole term of accepted nodes.  FEAT: do not add it but
-// accumulate per-target features of the walk for the learned correction:
-// (quadrupole sum x3, sum M l^2 / d^4, sum M l^3 / d^5, accepted nodes,
-// particle interactions, 0), in units without G, FP32, original order.
+// accumulate per-target features of the walk for the learned correction,
+// 12 FP32 per particle in original order, without the factor G:
+// (quadrupole sum x3, sum (M l^3 / d^6) R x3, sum M l^2 / d^4,
+// sum M l^3 / d^5, accepted nodes, particle interactions, 0, 0) with l the
+// node's side, R = x_i - com and d = |R| (softened).
 template <bool QUAD, bool FEAT>
 __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ nodes,
                                                    const double4 *__restrict__ P,
@@ -457,6 +459,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     unsigned cnt = 0;
     float fq0 = 0.0f, fq1 = 0.0f, fq2 = 0.0f, fs2 = 0.0f, fs3 = 0.0f;
+    float fv0 = 0.0f, fv1 = 0.0f, fv2 = 0.0f;
     unsigned nacc = 0, npp = 0;
     const unsigned long long me = 1ull << lane;
     int *snode = stk_node[wave];
@@ -524,8 +527,13 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                         // M l^2 / d^4 and M l^3 / d^5 (l: cell side, d: distance to the COM)
                         const float lr = static_cast<float>(__builtin_sqrt(l2)) * ri;
                         const float mr2 = mr3 * static_cast<float>(__builtin_sqrt(d2 + args.eps2));
+                        const float lr3 = lr * lr * lr;
                         fs2 = __builtin_fmaf(mr2, lr * lr, fs2);
-                        fs3 = __builtin_fmaf(mr2, lr * lr * lr, fs3);
+                        fs3 = __builtin_fmaf(mr2, lr3, fs3);
+                        const float k3 = mr3 * lr3;
+                        fv0 = __builtin_fmaf(k3, static_cast<float>(R0), fv0);
+                        fv1 = __builtin_fmaf(k3, static_cast<float>(R1), fv1);
+                        fv2 = __builtin_fmaf(k3, static_cast<float>(R2), fv2);
                         nacc++;
                     }
                 }
@@ -566,15 +574,10 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         a[3 * i + 1] = args.G * a1;
         a[3 * i + 2] = args.G * a2;
         if (FEAT) {
-            float *f = feat + 8 * static_cast<size_t>(i);
-            f[0] = fq0;
-            f[1] = fq1;
-            f[2] = fq2;
-            f[3] = fs2;
-            f[4] = fs3;
-            f[5] = static_cast<float>(nacc);
-            f[6] = static_cast<float>(npp);
-            f[7] = 0.0f;
+            float4 *f = reinterpret_cast<float4 *>(feat + 12 * static_cast<size_t>(i));
+            f[0] = make_float4(fq0, fq1, fq2, fv0);
+            f[1] = make_float4(fv1, fv2, fs2, fs3);
+            f[2] = make_float4(static_cast<float>(nacc), static_cast<float>(npp), 0.0f, 0.0f);
         }
     }
     // interaction count: one atomic per wave

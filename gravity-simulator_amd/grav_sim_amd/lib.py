@@ -14,6 +14,12 @@ import numpy as np
 from . import _abi
 from ._abi import PD, PI, PI64, AccelerationParam, ErrorStatus, LinearOctree, System
 
+PF = C.POINTER(C.c_float)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(PF)
+
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libgrav_sim.so")
 
@@ -32,6 +38,9 @@ EXPORTS = [
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
     "gs_set_fp32_config", "gs_copy_calibrate", "gs_whfast_ensemble",
+    "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
+    "gs_learned_reset", "gs_learned_forward", "gs_learned_gradients", "gs_learned_features",
+    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report",
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
     "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
 ]
@@ -98,6 +107,18 @@ def load(path=None):
         "gs_copy_calibrate": (ES, [C.c_size_t, C.c_int, PD]),
         "gs_whfast_ensemble": (ES, [C.c_int, PI64, PI, PD, PD, PD, C.c_double, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_bool, PI]),
+        "gs_learned_num_params": (C.c_int, []),
+        "gs_learned_train": (ES, [pS, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                  C.c_uint, PD]),
+        "gs_learned_get_params": (ES, [PF, PF, PF, PF]),
+        "gs_learned_set_params": (ES, [PF, PF, PF, C.c_float]),
+        "gs_learned_reset": (ES, []),
+        "gs_learned_forward": (ES, [PF, C.c_int, PF]),
+        "gs_learned_gradients": (ES, [PF, PF, C.c_int, PF, PD]),
+        "gs_learned_features": (ES, [pS, C.c_double, C.c_double, C.c_int, PD, PF]),
+        "gs_learned_set_policy": (ES, [C.c_double, C.c_int, C.c_double]),
+        "gs_learned_status": (ES, [PD, PI]),
+        "gs_learned_training_report": (ES, [PD, PD, PD, PD, PD]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -332,3 +353,91 @@ def whfast_ensemble(offsets, x, v, m, G, dt, tf, method=2, eps=0.0, remove_inval
         o, k = int(offsets[s]), int(cnt[s])
         out.append((ids[o:o + k].copy(), x[o:o + k].copy(), v[o:o + k].copy(), m[o:o + k].copy()))
     return out
+
+
+# ---------------------------------------------------------------- learned correction
+def learned_num_params():
+    return int(load().gs_learned_num_params())
+
+
+def learned_train(x, m, G=1.0, eps=0.0, theta=0.8, leaf=1, samples=1 << 20, epochs=30, batch=4096,
+                  lr=1e-3, seed=1):
+    """Train the residual network on (x, m); returns the last batch's loss."""
+    L = load()
+    x, m = _c(x), _c(m)
+    v0 = np.zeros_like(x)
+    s = _abi.make_system(x, v0, m, G)
+    loss = C.c_double(0.0)
+    check(L.gs_learned_train(C.byref(s), float(eps), float(theta), int(leaf), int(samples), int(epochs),
+                             int(batch), float(lr), int(seed), C.byref(loss)), "gs_learned_train")
+    return loss.value
+
+
+def learned_get_params():
+    L = load()
+    w = np.zeros(learned_num_params(), dtype=np.float32)
+    mu = np.zeros(16, dtype=np.float32)
+    sd = np.zeros(16, dtype=np.float32)
+    ts = np.zeros(1, dtype=np.float32)
+    check(L.gs_learned_get_params(_fptr(w), _fptr(mu), _fptr(sd), _fptr(ts)), "gs_learned_get_params")
+    return w, mu, sd, float(ts[0])
+
+
+def learned_set_params(w, mean=None, std=None, tscale=1.0):
+    L = load()
+    w = _c(w, np.float32)
+    mean = np.zeros(16, np.float32) if mean is None else _c(mean, np.float32)
+    std = np.ones(16, np.float32) if std is None else _c(std, np.float32)
+    check(L.gs_learned_set_params(_fptr(w), _fptr(mean), _fptr(std), float(tscale)), "gs_learned_set_params")
+
+
+def learned_reset():
+    check(load().gs_learned_reset(), "gs_learned_reset")
+
+
+def learned_forward(X):
+    L = load()
+    X = _c(X, np.float32)
+    Y = np.zeros((X.shape[0], 3), dtype=np.float32)
+    check(L.gs_learned_forward(_fptr(X), X.shape[0], _fptr(Y)), "gs_learned_forward")
+    return Y
+
+
+def learned_gradients(X, T):
+    L = load()
+    X, T = _c(X, np.float32), _c(T, np.float32)
+    g = np.zeros(learned_num_params(), dtype=np.float32)
+    loss = C.c_double(0.0)
+    check(L.gs_learned_gradients(_fptr(X), _fptr(T), X.shape[0], _fptr(g), C.byref(loss)),
+          "gs_learned_gradients")
+    return g, loss.value
+
+
+def learned_features(x, m, G=1.0, eps=0.0, theta=0.8, leaf=1):
+    """(coarse accelerations (N,3) float64, raw features (N,16) float32)."""
+    L = load()
+    x, m = _c(x), _c(m)
+    a = np.zeros_like(x)
+    s = _abi.make_system(x, a, m, G)  # v is not read
+    F = np.zeros((len(m), 16), dtype=np.float32)
+    check(L.gs_learned_features(C.byref(s), float(eps), float(theta), int(leaf), _abi.dptr(a), _fptr(F)),
+          "gs_learned_features")
+    return a, F
+
+
+def learned_set_policy(tol=1e-3, samples=1024, fallback_theta=0.5):
+    check(load().gs_learned_set_policy(float(tol), int(samples), float(fallback_theta)), "gs_learned_set_policy")
+
+
+def learned_status():
+    p = C.c_double(0.0)
+    f = C.c_int(0)
+    check(load().gs_learned_status(C.byref(p), C.byref(f)), "gs_learned_status")
+    return p.value, bool(f.value)
+
+
+def learned_training_report():
+    """dict(alpha, p99_net, p99_without, median_net, median_without) of the last training."""
+    v = [C.c_double(0.0) for _ in range(5)]
+    check(load().gs_learned_training_report(*[C.byref(t) for t in v]), "gs_learned_training_report")
+    return dict(zip(("alpha", "p99_net", "p99_without", "median_net", "median_without"), [t.value for t in v]))

@@ -115,6 +115,72 @@ ErrorStatus gs_last_walk_interactions(long long *count);
  * is the reference point for the HBM byte counters. */
 ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs);
 
+/* ---- learned Barnes-Hut correction (ACCELERATION_METHOD_BARNES_HUT_LEARNED) --
+ * A monopole walk at the caller's opening angle (0.8 is the intended coarse
+ * value) accumulates per-particle features of the accepted nodes (quadrupole
+ * force sum G q, size-weighted sums such as s3 = sum M l^3 / d^5, node
+ * counts).  The residual a_direct - a_coarse is predicted as
+ * G q + G s3 * tscale * net(F): the accepted nodes' quadrupole terms plus, in
+ * units of the next multipole order's scale, what an MLP
+ *   16 -> 128 -> 128 -> 128 -> 3, tanh-GELU, BF16 matrix-core products with
+ *   FP32 accumulation, FP32 master weights
+ * learns from the normalised features F; the prediction is added to the
+ * coarse result.  Every evaluation then spot-checks
+ * spot_samples evenly strided particles against the FP32 split-precision
+ * direct sum; if their 99th-percentile relative error exceeds spot_tol the
+ * result is replaced by the plain monopole walk at fallback_theta.
+ * Parameters: weights W1 (16x128), b1, W2 (128x128), b2, W3 (128x128), b3,
+ * W4 (128x3), b4, each matrix row-major as (inputs, outputs), in one flat
+ * FP32 array of gs_learned_num_params() values.                              */
+#define GS_LEARNED_NUM_FEATURES 16
+#define GS_LEARNED_NUM_OUTPUTS 3
+int gs_learned_num_params(void);
+
+/* Trains the network on one particle set: coarse walk at theta_coarse,
+ * FP32 direct-sum labels for `samples` particles drawn without replacement
+ * with `seed`, feature normalisation and target scale from that sample
+ * (targets: the residual left after G q divided by G s3, each sample's norm
+ * clipped at the sample's 99.9th percentile, scaled to unit RMS), then
+ * `epochs` epochs of Adam (batches of `batch` rows, learning rate lr with
+ * cosine decay) from a deterministic initialisation on nine tenths of the
+ * sample.  On the held-out tenth the output scale alpha in {0, 1/8, ..., 1}
+ * that gives the smallest 99th-percentile relative error is chosen (alpha 0
+ * leaves the quadrupole sum alone) and folded into the target scale.
+ * final_loss (may be NULL): mean squared error of the last epoch's last
+ * batch. */
+ErrorStatus gs_learned_train(const System *system, double softening_length, double theta_coarse,
+                             int max_num_particles_per_leaf, int samples, int epochs, int batch,
+                             double lr, unsigned int seed, double *final_loss);
+/* Held-out statistics of the last gs_learned_train: the chosen output scale
+ * and the 99th-percentile and median relative errors with the network at that
+ * scale and without it.  Any pointer may be NULL. */
+ErrorStatus gs_learned_training_report(double *alpha, double *p99_net, double *p99_without,
+                                       double *median_net, double *median_without);
+/* Copies out / installs the parameters, the feature normalisation (16 means
+ * and 16 standard deviations) and the target scale.  set marks the network
+ * trained; reset marks it untrained (method 18 then fails with a value
+ * error). */
+ErrorStatus gs_learned_get_params(float *params, float *feature_mean, float *feature_std,
+                                  float *target_scale);
+ErrorStatus gs_learned_set_params(const float *params, const float *feature_mean,
+                                  const float *feature_std, float target_scale);
+ErrorStatus gs_learned_reset(void);
+/* Network output Y (rows x 3) for normalised features X (rows x 16). */
+ErrorStatus gs_learned_forward(const float *X, int rows, float *Y);
+/* Gradient of mean over rows of |net(X) - T|^2 with respect to the
+ * parameters (flat layout above) and that loss. */
+ErrorStatus gs_learned_gradients(const float *X, const float *T, int rows, float *grad,
+                                 double *loss);
+/* Coarse walk accelerations (N x 3) and raw (unnormalised) features (N x 16)
+ * of a particle set. */
+ErrorStatus gs_learned_features(const System *system, double softening_length, double theta,
+                                int max_num_particles_per_leaf, double *a_coarse,
+                                float *features);
+/* Spot-check policy (defaults 1e-3, 1024, 0.5; spot_samples 0 disables the
+ * check) and the outcome of the last evaluation. */
+ErrorStatus gs_learned_set_policy(double spot_tol, int spot_samples, double fallback_theta);
+ErrorStatus gs_learned_status(double *last_p99, int *fell_back);
+
 /* ---- FP32 direct-sum autotuner ---------------------------------------------
  * Times the candidate (block, targets-per-thread, source-split) shapes on an
  * N-particle input resident on the device and keeps the fastest for later

@@ -260,7 +260,19 @@ typedef struct
     const int64_t *prefix;
     const double *mass, *comx, *comy, *comz;
     const double *quad; /* 6 per node (xx yy zz xy xz yz) or NULL: monopole only */
+    int features;       /* nonzero: quad feeds the features, not the acceleration */
 } Walk;
+
+/* Walk features of one target (the learned correction's inputs, without G):
+ * q  = sum of the quadrupole accelerations of accepted nodes,
+ * v  = sum (M l^3 / r^6) R, s2 = sum M l^2 / r^4, s3 = sum M l^3 / r^5 over
+ *      accepted nodes (l: node side, R = x_i - com, r^2 = |R|^2 + eps^2),
+ * nacc / npp = accepted nodes / particle interactions, where a leaf holding a
+ * single particle always counts as a particle interaction. */
+typedef struct
+{
+    double q[3], v[3], s2, s3, nacc, npp;
+} WalkFeat;
 
 /* Quadrupole term of an accepted node at R = x_i - com (r^2 = |R|^2 + eps^2):
  *   a += G [ 3 Q R / r^5 - (15/2) (R.Q.R) R / r^7 + (3/2) tr(Q) R / r^5 ]
@@ -284,9 +296,10 @@ static void quad_accel(double acc[3], const double *q, double R0, double R1, dou
 }
 
 static void walk_node(const Walk *w, int node, int i, int64_t key_i, const double xi[3],
-                      double acc[3], int64_t *nint)
+                      double acc[3], int64_t *nint, WalkFeat *f)
 {
     const int d = w->depth[node];
+    const int single = w->nchild[node] == 0 && w->count[node] == 1;
     const int included = (key_i >> (3 * (MAX_LEVEL - d))) == w->prefix[node];
     if (!included)
     {
@@ -303,8 +316,23 @@ static void walk_node(const Walk *w, int node, int i, int64_t key_i, const doubl
             acc[0] -= t * R0;
             acc[1] -= t * R1;
             acc[2] -= t * R2;
-            if (w->quad)
+            if (w->quad && !w->features)
                 quad_accel(acc, &w->quad[6 * node], R0, R1, R2, d2 + w->eps2, w->G);
+            if (f && single)
+                f->npp += 1.0;
+            else if (f)
+            {
+                const double r2 = d2 + w->eps2;
+                const double M = w->mass[node];
+                quad_accel(f->q, &w->quad[6 * node], R0, R1, R2, r2, 1.0);
+                f->s2 += M * l * l / (r2 * r2);
+                f->s3 += M * l * l * l / (r2 * r2 * r);
+                const double k3 = M * l * l * l / (r2 * r2 * r2);
+                f->v[0] += k3 * R0;
+                f->v[1] += k3 * R1;
+                f->v[2] += k3 * R2;
+                f->nacc += 1.0;
+            }
             (*nint)++;
             return;
         }
@@ -326,11 +354,13 @@ static void walk_node(const Walk *w, int node, int i, int64_t key_i, const doubl
             acc[1] -= tv * R1;
             acc[2] -= tv * R2;
             (*nint)++;
+            if (f)
+                f->npp += 1.0;
         }
         return;
     }
     for (int c = w->child0[node]; c < w->child0[node] + w->nchild[node]; c++)
-        walk_node(w, c, i, key_i, xi, acc, nint);
+        walk_node(w, c, i, key_i, xi, acc, nint, f);
 }
 
 void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double *m, double G,
@@ -338,11 +368,11 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
                  const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
                  const int *count, const int *first, const int *nchild, const int *child0,
                  const double *mass, const double *comx, const double *comy, const double *comz,
-                 const double *quad)
+                 const double *quad, double *feat)
 {
     Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
               depth, count, first,  nchild,    child0,        prefix, mass,        comx,
-              comy,  comz,  quad};
+              comy,  comz,  quad,   feat != NULL};
 #pragma omp parallel for schedule(dynamic, 64)
     for (int s = 0; s < n; s++)
     {
@@ -350,9 +380,18 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
         const double xi[3] = {x[i * 3 + 0], x[i * 3 + 1], x[i * 3 + 2]};
         double acc[3] = {0.0, 0.0, 0.0};
         int64_t cnt = 0;
+        WalkFeat f = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, 0.0};
         /* the root is always internal: start from its children */
         for (int c = child0[0]; c < child0[0] + nchild[0]; c++)
-            walk_node(&w, c, i, sorted_keys[s], xi, acc, &cnt);
+            walk_node(&w, c, i, sorted_keys[s], xi, acc, &cnt, feat ? &f : NULL);
+        if (feat)
+        {
+            double *o = &feat[(size_t)i * 12];
+            o[0] = f.q[0], o[1] = f.q[1], o[2] = f.q[2];
+            o[3] = f.v[0], o[4] = f.v[1], o[5] = f.v[2];
+            o[6] = f.s2, o[7] = f.s3, o[8] = f.nacc, o[9] = f.npp;
+            o[10] = 0.0, o[11] = 0.0;
+        }
         a[i * 3 + 0] = acc[0];
         a[i * 3 + 1] = acc[1];
         a[i * 3 + 2] = acc[2];

@@ -69,13 +69,19 @@ int orc_octree(int max_nodes, int n, const double *x, const double *m,
  *   leaf      -> direct sum over its particles, skipping i
  *   otherwise -> descend.
  * Writes a[idx*3..] for every particle and, if n_int != NULL, the number of
- * accepted nodes + leaf-particle interactions per particle. */
+ * accepted nodes + leaf-particle interactions per particle.
+ * With feat != NULL (and quad given) the walk stays monopole-only and writes
+ * 12 features per particle (original order) for the learned correction:
+ * quadrupole-acceleration sum (3), sum (M l^3 / r^6) R (3), sum M l^2 / r^4,
+ * sum M l^3 / r^5, accepted nodes, particle interactions, 0, 0 -- without G;
+ * l is the node side, R = x_i - com, r^2 = |R|^2 + eps^2; a leaf holding one
+ * particle counts as a particle interaction. */
 void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double *m, double G,
                  double eps, double theta, double width, const int64_t *sorted_keys,
                  const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
                  const int *count, const int *first, const int *nchild, const int *child0,
                  const double *mass, const double *comx, const double *comy, const double *comz,
-                 const double *quad);
+                 const double *quad, double *feat);
 
 /* Second central moments per node, Q_ab = sum m (x - com)_a (x - com)_b, in
  * the order xx yy zz xy xz yz: leaves over their particles, internal nodes
