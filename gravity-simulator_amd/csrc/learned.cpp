@@ -46,7 +46,9 @@ static ErrorStatus spot_check(Context *c, const double *x, const double *m, int 
     double *dr = reinterpret_cast<double *>(static_cast<char *>(sp) + off);
     double *dg = dr + 3 * static_cast<size_t>(ns);
     GS_HIP(hipMemcpyAsync(dt, tg.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
+    const int rec = timing_begin(c, "spot_check");
     GS_HIP(launch_direct_f32_targets(c, x, m, n, G, eps, dt, ns, dr, buf, need_bytes));
+    timing_end(c, rec);
     GS_HIP(launch_gather_rows3(c, a, dt, ns, dg));
     // reference and gathered result: one copy
     std::vector<double> rg(6 * static_cast<size_t>(ns));
@@ -81,7 +83,9 @@ static ErrorStatus coarse_walk(Context *c, const double *x, const double *m, int
     GS_TRY(slot(c, SLOT_MLP_X, sizeof(float) * MLP_F * static_cast<size_t>(n), &F));
     GS_TRY(tree_walk(c, t, x, G, eps, theta, a, &g_learned_interactions, 1,
                      static_cast<float *>(wf)));
+    const int rec = timing_begin(c, "learned_features");
     GS_TRY(mlp_features(c, a, static_cast<float *>(wf), n, G, normalise, static_cast<float *>(F)));
+    timing_end(c, rec);
     *F_out = static_cast<float *>(F);
     if (t_out) *t_out = t;
     if (wf_out) *wf_out = static_cast<float *>(wf);
@@ -103,7 +107,9 @@ ErrorStatus learned_acceleration(Context *c, const double *x, const double *m, i
     GS_TRY(coarse_walk(c, x, m, n, G, eps, p->opening_angle, p->max_num_particles_per_leaf, a,
                        true, &F, &t, &wf));
     long long cnt = g_learned_interactions;
+    const int rec = timing_begin(c, "learned_mlp");
     GS_TRY(mlp_apply(c, F, wf, G, n, a));
+    timing_end(c, rec);
     M.last_fell_back = 0;
     M.last_p99 = -1.0;
     if (M.spot_samples > 0) {

@@ -14,7 +14,8 @@
 //   k_mse         dY = 2 (Y - T) / B and per-block loss partials
 //   k_adam        Adam on the FP32 master weights
 //   k_features    per-particle input features from the coarse walk
-//   k_apply       a = a_coarse + G q + G s3 * tscale * (y in the local frame)
+//   k_mlp_infer   fused inference + correction for method BARNES_HUT_LEARNED:
+//                 a = a_coarse + G q + G s3 * tscale * (y in the local frame)
 //                 (q: the walk's quadrupole sum, the physics part of the
 //                 predicted residual; s3 = sum M l^3 / d^5, the scale of the
 //                 next multipole order)
@@ -365,21 +366,159 @@ __global__ void k_features(const double *__restrict__ a, const float *__restrict
     for (int k = 0; k < MLP_F; k++) o[k] = mean ? (f[k] - mean[k]) / stdv[k] : f[k];
 }
 
-// a += G q + G s3 * tscale * (y0 e1 + y1 e2 + y2 e3)
-__global__ void k_apply(double *__restrict__ a, const float *__restrict__ Y,
-                        const float *__restrict__ wf, double G, int n, float tscale)
+// ------------------------------------------------------------------ fused inference
+// One kernel for the whole correction of method BARNES_HUT_LEARNED: each
+// wavefront takes 16 particles at a time through the four layers with the
+// bf16 weights resident in LDS (transposed to (outputs, inputs) so that a
+// lane's eight k-values of a B fragment are one 16-byte read), keeps the
+// 16 x 128 activations of its particles in its own LDS tile between layers,
+// and finally applies
+//   a += G q + G s3 * tscale * (y0 e1 + y1 e2 + y2 e3)
+// in the particle's local frame.  Rows of the LDS tiles are padded by 8
+// bf16 (16 B) so that the 16 rows a fragment read touches fall in
+// different banks.
+constexpr int MI_WAVES = 8;
+constexpr int MI_KP1 = 32 + 8;        // layer-1 weight rows: K = 16 padded to 32
+constexpr int MI_KPH = MLP_H + 8;     // 128-wide rows
+
+struct MiLds {
+    __bf16 w1[MLP_H * MI_KP1];
+    __bf16 w2[MLP_H * MI_KPH];
+    __bf16 w3[MLP_H * MI_KPH];
+    __bf16 w4[16 * MI_KPH];
+    float b1[MLP_H], b2[MLP_H], b3[MLP_H], b4[16];
+    __bf16 h[MI_WAVES][16 * MI_KPH];
+    float y[MI_WAVES][16 * 4];
+};
+
+// GELU (tanh form) as z * sigmoid(2u): 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u))
+__device__ __forceinline__ float gelu_fast(float z)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double ai[3] = {a[3 * i], a[3 * i + 1], a[3 * i + 2]};
-    const float *w = wf + MLP_WALK_FEAT * static_cast<size_t>(i);
-    double q[3], v[3], e1[3], e2[3], e3[3];
-    walk_vectors(w, q, v);
-    mlp_local_frame(ai, q, v, e1, e2, e3);
-    const double s = G * static_cast<double>(w[7]) * static_cast<double>(tscale);
-    const double y0 = Y[MLP_O * i], y1 = Y[MLP_O * i + 1], y2 = Y[MLP_O * i + 2];
-    for (int d = 0; d < 3; d++)
-        a[3 * i + d] = ai[d] + G * q[d] + s * (y0 * e1[d] + y1 * e2[d] + y2 * e3[d]);
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    return z / (1.0f + __expf(-2.0f * u));
+}
+
+__device__ __forceinline__ bf16x8 lds8(const __bf16 *p)
+{
+    return *reinterpret_cast<const bf16x8 *>(p);
+}
+
+// Hidden layer: 16 rows of h (K = 128) times W (LDS, [n][k]) plus bias,
+// GELU, written back into h.
+__device__ __forceinline__ void mi_hidden(__bf16 *h, const __bf16 *w, const float *b, int r, int q)
+{
+    bf16x8 a[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) a[ks] = lds8(h + r * MI_KPH + ks * 32 + 8 * q);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+    for (int nt = 0; nt < MLP_H / 16; nt++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const __bf16 *wn = w + (nt * 16 + r) * MI_KPH + 8 * q;
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], lds8(wn + ks * 32), acc, 0, 0, 0);
+        const int col = nt * 16 + r;
+        const float bc = b[col];
+#pragma unroll
+        for (int j = 0; j < 4; j++) h[(4 * q + j) * MI_KPH + col] = to_bf16(gelu_fast(acc[j] + bc));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_infer(const float *__restrict__ F,
+                                                              const float *__restrict__ wf,
+                                                              const float *__restrict__ w, int n,
+                                                              double G, float tscale,
+                                                              double *__restrict__ a)
+{
+    extern __shared__ __align__(16) unsigned char mi_smem[];
+    MiLds &S = *reinterpret_cast<MiLds *>(mi_smem);
+    const int tid = threadIdx.x;
+    // weights: FP32 master (row-major (k, n)) -> bf16 transposed in LDS
+    for (int i = tid; i < MLP_H * MI_KP1; i += 64 * MI_WAVES) {
+        const int nn = i / MI_KP1, k = i % MI_KP1;
+        S.w1[i] = to_bf16(k < MLP_F ? w[MLP_OFF_W1 + k * MLP_H + nn] : 0.0f);
+    }
+    for (int i = tid; i < MLP_H * MLP_H; i += 64 * MI_WAVES) {
+        const int k = i / MLP_H, nn = i % MLP_H;
+        S.w2[nn * MI_KPH + k] = to_bf16(w[MLP_OFF_W2 + i]);
+        S.w3[nn * MI_KPH + k] = to_bf16(w[MLP_OFF_W3 + i]);
+    }
+    for (int i = tid; i < 16 * MI_KPH; i += 64 * MI_WAVES) {
+        const int nn = i / MI_KPH, k = i % MI_KPH;
+        S.w4[i] = to_bf16(nn < MLP_O && k < MLP_H ? w[MLP_OFF_W4 + k * MLP_O + nn] : 0.0f);
+    }
+    for (int i = tid; i < MLP_H; i += 64 * MI_WAVES) {
+        S.b1[i] = w[MLP_OFF_B1 + i];
+        S.b2[i] = w[MLP_OFF_B2 + i];
+        S.b3[i] = w[MLP_OFF_B3 + i];
+    }
+    if (tid < 16) S.b4[tid] = tid < MLP_O ? w[MLP_OFF_B4 + tid] : 0.0f;
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    __bf16 *h = S.h[wave];
+    float *y = S.y[wave];
+    const int ntiles = (n + 15) / 16;
+    for (int tile = blockIdx.x * MI_WAVES + wave; tile < ntiles; tile += gridDim.x * MI_WAVES) {
+        const int row0 = tile * 16;
+        // layer 1: K = 16 features (lanes q = 2, 3 carry the zero padding)
+        bf16x8 x;
+        {
+            const int row = row0 + r;
+            float4 u0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), u1 = u0;
+            if (q < 2 && row < n) {
+                const float4 *src = reinterpret_cast<const float4 *>(F + static_cast<size_t>(row) * MLP_F + 8 * q);
+                u0 = src[0];
+                u1 = src[1];
+            }
+            x[0] = to_bf16(u0.x); x[1] = to_bf16(u0.y); x[2] = to_bf16(u0.z); x[3] = to_bf16(u0.w);
+            x[4] = to_bf16(u1.x); x[5] = to_bf16(u1.y); x[6] = to_bf16(u1.z); x[7] = to_bf16(u1.w);
+        }
+#pragma unroll
+        for (int nt = 0; nt < MLP_H / 16; nt++) {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                x, lds8(S.w1 + (nt * 16 + r) * MI_KP1 + 8 * q), acc, 0, 0, 0);
+            const int col = nt * 16 + r;
+            const float bc = S.b1[col];
+#pragma unroll
+            for (int j = 0; j < 4; j++) h[(4 * q + j) * MI_KPH + col] = to_bf16(gelu_fast(acc[j] + bc));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        mi_hidden(h, S.w2, S.b2, r, q);
+        mi_hidden(h, S.w3, S.b3, r, q);
+        // layer 4: 3 outputs (columns 3..15 of the tile are zero weights)
+        {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    lds8(h + r * MI_KPH + ks * 32 + 8 * q), lds8(S.w4 + r * MI_KPH + ks * 32 + 8 * q),
+                    acc, 0, 0, 0);
+            if (r < MLP_O) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) y[(4 * q + j) * 4 + r] = acc[j] + S.b4[r];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // correction of the tile's 16 particles, one lane each
+        if (lane < 16 && row0 + lane < n) {
+            const size_t i = static_cast<size_t>(row0 + lane);
+            const double ai[3] = {a[3 * i], a[3 * i + 1], a[3 * i + 2]};
+            const float *wi = wf + MLP_WALK_FEAT * i;
+            double qv[3], vv[3], e1[3], e2[3], e3[3];
+            walk_vectors(wi, qv, vv);
+            mlp_local_frame(ai, qv, vv, e1, e2, e3);
+            const double s = G * static_cast<double>(wi[7]) * static_cast<double>(tscale);
+            const double y0 = y[lane * 4], y1 = y[lane * 4 + 1], y2 = y[lane * 4 + 2];
+            for (int d = 0; d < 3; d++)
+                a[3 * i + d] = ai[d] + G * qv[d] + s * (y0 * e1[d] + y1 * e2[d] + y2 * e3[d]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    }
 }
 
 // ------------------------------------------------------------------ host API pieces
@@ -520,19 +659,20 @@ ErrorStatus mlp_features(Context *c, const double *a, const float *walkfeat, int
 ErrorStatus mlp_apply(Context *c, const float *F, const float *wf, double G, int n, double *a)
 {
     GS_TRY(mlp_ensure(c));
-    const int chunk = 1 << 18;
-    Acts ac;
-    GS_TRY(acts(c, std::min(n, chunk), &ac));
-    for (int s = 0; s < n; s += chunk) {
-        const int rows = std::min(chunk, n - s);
-        GS_HIP(hipMemcpyAsync(ac.X, F + static_cast<size_t>(s) * MLP_F,
-                              sizeof(float) * rows * MLP_F, hipMemcpyDeviceToDevice, c->stream));
-        GS_HIP(forward(c, g_model.w, ac, rows));
-        hipLaunchKernelGGL(k_apply, dim3((rows + 255) / 256), dim3(256), 0, c->stream,
-                           a + 3 * static_cast<size_t>(s), ac.Y,
-                           wf + MLP_WALK_FEAT * static_cast<size_t>(s), G, rows, g_model.tscale);
-        GS_HIP(hipGetLastError());
+    static bool attr_set = false;
+    const size_t lds = sizeof(MiLds);
+    if (!attr_set) {
+        GS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_mlp_infer),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        attr_set = true;
     }
+    const int ntiles = (n + 15) / 16;
+    int grid = (ntiles + MI_WAVES - 1) / MI_WAVES;
+    if (grid > c->cu_count) grid = c->cu_count;  // one resident workgroup per CU, looping
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64 * MI_WAVES), lds, c->stream, F, wf,
+                       g_model.w, n, G, g_model.tscale, a);
+    GS_HIP(hipGetLastError());
     return make_success_error_status();
 }
 
