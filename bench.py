@@ -1,18 +1,31 @@
-"""Benchmark: FP32 split-precision direct sum + Leapfrog on one MI355X.
+"""Benchmark of the force-evaluation path on one MI355X.
 
-Workload (one "step" = one Leapfrog KDK step whose force evaluation is the
-hot path): Plummer sphere, N = 131072 (2^17), G = 1, equal masses, seed 2,
-softening 1e-3, FP32 split-precision direct sum with Kahan-compensated tile
-sums, FP64 positions/velocities, dt = 1e-3.  Inputs are resident in HBM
-before the timed region.
+Headline (`value`): FP32 split-precision direct sum + Leapfrog.  One "step" =
+one Leapfrog KDK step whose force evaluation is the hot path: Plummer sphere,
+N = 131072 (2^17), G = 1, equal masses, seed 2, softening 1e-3, FP32
+split-precision direct sum with Kahan-compensated tile sums, FP64
+positions/velocities, dt = 1e-3.  Inputs are resident in HBM before the timed
+region.
 
 Printed as one JSON line:
   value      directed interactions/s = steps * N (N-1) / wall time
   roofline   dominant kernel (direct_f32): 20 flops per interaction over its
-             HIP-event-timed average launch, against the MI355X FP32 vector peak
+             HIP-event-timed average launch, against the MI355X FP32 vector
+             peak; traffic = HBM bytes per launch from rocprofv3 --pmc
   accuracy   relative error vs the FP64 GPU direct sum (4096 sampled targets)
-  bh         Barnes-Hut (N = 2^20 two Plummer spheres, theta = 0.5, leaf 1):
-             particles/s of build + walk and its error vs FP64 direct
+  bh         Barnes-Hut, two Plummer spheres (scale radius 1, centres 10
+             apart), relative errors vs the FP64 GPU direct sum on 4096
+             strided particles:
+             n2e20  N = 2^20 (seed 4), theta = 0.5, monopole (the reference's
+                    method): particles/s of sort + build + walk
+             n2e21  N = 2^21 (seed 5): "plain" = monopole theta = 0.5;
+                    candidates meeting p99 <= 1e-3 (monopole at the largest
+                    theta that does, quadrupole, learned correction trained on
+                    10^6 samples of seed 7); "chosen" = the fastest of those;
+                    the learned correction at theta = 0.8 with its spot check
+  whfast     4096 planetary systems (1 star + 8 planets + U{0..2000} test
+             particles), WHFast dt = 0.01, a bounded number of steps:
+             particle-steps/s of the batched kernel
   cpu_baseline  the reference C library (oracle/_ref) timed on this host,
              single thread (its pairwise path is serial), on a bounded sample
 """
@@ -30,12 +43,20 @@ for p in (REPO, os.path.join(REPO, "gravity-simulator_amd")):
 import numpy as np  # noqa: E402
 
 N_DIRECT = 1 << 17
-N_BH = 1 << 20
 EPS = 1e-3
 DT = 1e-3
 FLOPS_PER_INTERACTION = 20.0
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+P99_TARGET = 1e-3
+# HBM traffic of one direct_f32 launch at N = 2^17 (64 source chunks):
+# rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), copy-kernel calibrated,
+# profiles/direct_f32_plummer_n131072.md
+DIRECT_F32_TRAFFIC_BYTES = None
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(x, m, sample_n):
@@ -57,20 +78,162 @@ def cpu_baseline(x, m, sample_n):
                       f"N=2^17 Plummer input (eps=1e-3), {dt:.1f} s"}
 
 
+def timed_evals(L, sim, p, reps, kernels):
+    """Average wall ms per evaluation (device-resident) and per-kernel ms."""
+    sim.acceleration(p, copy=False)
+    L.synchronize()
+    L.timing_enable(True)
+    L.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.acceleration(p, copy=False)
+    L.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    parts = {}
+    for k in kernels:
+        ms, calls = L.timing_query(k)
+        if calls:
+            parts[k + "_ms"] = ms / reps
+    L.timing_enable(False)
+    return wall, parts
+
+
+BH_KERNELS = ["tree_sort", "tree_build", "bh_walk", "learned_features", "learned_mlp", "spot_check"]
+
+
+def bh_case(L, sim, n, method, theta, reps=5, label=None):
+    p = L.param(method, theta=theta, eps=EPS, leaf=1)
+    wall, parts = timed_evals(L, sim, p, reps, BH_KERNELS)
+    nint = L.last_walk_interactions()
+    med, p99, mx = sim.accel_error(p, samples=4096)
+    r = {"method": label or {3: "monopole", 19: "quadrupole", 18: "learned"}[method], "theta": theta,
+         "particles_per_s": n / wall, "ms_per_eval": wall * 1e3, "interactions": nint,
+         "median_rel": med, "p99_rel": p99, "max_rel": mx}
+    r.update({k: round(v, 4) for k, v in parts.items()})
+    if "bh_walk_ms" in parts:
+        r["walk_tflops"] = FLOPS_PER_INTERACTION * nint / (parts["bh_walk_ms"] / 1e3) / 1e12
+    return r
+
+
+def run_bh(L, ic):
+    out = {}
+    # N = 2^20, the reference's method
+    n4 = 1 << 20
+    x, v, m = ic.two_spheres(n4, 4)
+    sim = L.Sim(x, v, m, 1.0)
+    out["n2e20"] = {"workload": "two Plummer spheres N=2^20 (seed 4), eps=1e-3, leaf=1",
+                    "plain": bh_case(L, sim, n4, 3, 0.5)}
+    sim.close()
+    log(f"BH 2^20 plain: {out['n2e20']['plain']['particles_per_s']:.3e} particles/s")
+
+    # N = 2^21: plain vs chosen at p99 <= 1e-3
+    n5 = 1 << 21
+    xt, vt, mt = ic.two_spheres(n5, 7)
+    x, v, m = ic.two_spheres(n5, 5)
+    sim = L.Sim(x, v, m, 1.0)
+    res = {"workload": "two Plummer spheres N=2^21 (seed 5), eps=1e-3, leaf=1; learned correction "
+                       "trained on 10^6 samples of seed 7", "p99_target": P99_TARGET}
+    res["plain"] = bh_case(L, sim, n5, 3, 0.5)
+    log(f"BH 2^21 plain theta 0.5: p99 {res['plain']['p99_rel']:.2e}")
+    cands = []
+    for th in (0.45, 0.4, 0.35, 0.3, 0.25, 0.2):
+        r = bh_case(L, sim, n5, 3, th, reps=3)
+        if r["p99_rel"] <= P99_TARGET:
+            res["plain_at_target"] = r
+            cands.append(r)
+            break
+    for th in (0.6, 0.55, 0.5, 0.45, 0.4):
+        r = bh_case(L, sim, n5, 19, th, reps=3)
+        if r["p99_rel"] <= P99_TARGET:
+            res["quadrupole_at_target"] = r
+            cands.append(r)
+            break
+    log("BH 2^21 monopole / quadrupole at target done")
+
+    # learned correction, the specified coarse walk (theta = 0.8), spot check on
+    t0 = time.perf_counter()
+    loss = L.learned_train(xt, mt, eps=EPS, theta=0.8, leaf=1, samples=1000000, epochs=10, batch=4096,
+                           lr=3e-3, seed=5)
+    train_s = time.perf_counter() - t0
+    rep = L.learned_training_report()
+    L.learned_set_policy(1.0, 0, 0.5)
+    raw = bh_case(L, sim, n5, 18, 0.8, reps=3)
+    L.learned_set_policy(P99_TARGET, 1024, 0.5)
+    checked = bh_case(L, sim, n5, 18, 0.8, reps=3, label="learned+spot-check")
+    spot_p99, fell_back = L.learned_status()
+    res["learned_theta0.8"] = {"train_s": train_s, "train_loss": loss, "held_out": rep,
+                               "without_check": raw, "with_check": checked,
+                               "spot_check_p99": spot_p99, "fell_back_to_theta0.5": fell_back}
+    log(f"BH 2^21 learned 0.8: p99 {raw['p99_rel']:.2e}, fell back {fell_back}")
+    # learned correction at the coarsest theta whose spot check passes
+    for th in (0.55, 0.5, 0.45, 0.4):
+        L.learned_train(xt, mt, eps=EPS, theta=th, leaf=1, samples=1000000, epochs=10, batch=4096, lr=3e-3,
+                        seed=5)
+        L.learned_set_policy(P99_TARGET, 1024, 0.5)
+        r = bh_case(L, sim, n5, 18, th, reps=3, label="learned+spot-check")
+        sp, fb = L.learned_status()
+        r["spot_check_p99"], r["fell_back"] = sp, fb
+        r["held_out"] = L.learned_training_report()
+        if not fb and r["p99_rel"] <= P99_TARGET:
+            res["learned_at_target"] = r
+            cands.append(r)
+            break
+    L.learned_set_policy(P99_TARGET, 1024, 0.5)
+    sim.close()
+    if cands:
+        best = max(cands, key=lambda r: r["particles_per_s"])
+        res["chosen"] = {"method": best["method"], "theta": best["theta"],
+                         "particles_per_s": best["particles_per_s"], "p99_rel": best["p99_rel"]}
+        if "plain_at_target" in res:
+            res["chosen_vs_plain_at_equal_p99"] = best["particles_per_s"] / res["plain_at_target"]["particles_per_s"]
+        res["chosen_vs_plain_theta0.5"] = best["particles_per_s"] / res["plain"]["particles_per_s"]
+    out["n2e21"] = res
+    log(f"BH 2^21 chosen: {res.get('chosen')}")
+    return out
+
+
+def run_whfast(L, ic, steps):
+    off, X, V, M = ic.ensemble(4096, 3)
+    P = int(off[-1])
+    counts = np.diff(off)
+    T = counts - 9
+    dt = 0.01
+    L.timing_enable(True)
+    L.timing_reset()
+    t0 = time.perf_counter()
+    L.whfast_ensemble(off, X, V, M, 1.0, dt, steps * dt, method=2)
+    wall = time.perf_counter() - t0
+    k_ms, k_calls = L.timing_query("whfast_steps")
+    i_ms, _ = L.timing_query("whfast_init")
+    L.timing_enable(False)
+    # 20 flops per pair force (massive-massive and massive-test) + 150 flops
+    # per Kepler drift, per system-step
+    flops = steps * float(np.sum(20.0 * (9 * 8 + 9 * T) + 150.0 * (counts - 1)))
+    return {"workload": f"4096 systems (1 star + 8 planets + U{{0..2000}} test particles, seed 3), "
+                        f"{P} particles, WHFast dt=0.01, {steps} steps, massless form",
+            "particles": P, "steps": steps,
+            "particle_steps_per_s": P * steps / (k_ms / 1e3), "kernel_ms": k_ms, "init_ms": i_ms,
+            "wall_s_incl_transfers": wall,
+            "fp64_tflops": flops / (k_ms / 1e3) / 1e12, "fp64_frac": flops / (k_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+            "full_1e4_steps_s_est": k_ms / 1e3 * 1e4 / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--whfast-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--no-whfast", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="run the FP32 shape autotuner first (default: the library's default shape)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "decomposition is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): this build runs on a single "
+              "MI355X; the multi-GPU decomposition is described, not built", file=sys.stderr)
         sys.exit(2)
 
     from grav_sim_amd import ic
@@ -101,6 +264,7 @@ def main():
     achieved = FLOPS_PER_INTERACTION * inter / kavg / 1e12
     med, p99, mx = sim.accel_error(p, samples=4096)
     sim.close()
+    log(f"direct: {value:.4e} interactions/s, kernel {kavg * 1e3:.3f} ms, {achieved:.1f} TFLOP/s")
 
     out = {
         "metric": "interactions/s + BH particles/s at fixed accel error vs FP64; plain vs chosen",
@@ -118,7 +282,7 @@ def main():
         "config": {"workload": "FP32 split-precision direct sum + Leapfrog, Plummer N=131072, seed 2, "
                                "eps=1e-3, dt=1e-3", "N": N_DIRECT, "fp32_shape": tune},
         "roofline": {"bound": "fp32-valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": DIRECT_F32_TRAFFIC_BYTES,
                      "kernel": "direct_f32", "kernel_avg_ms": kavg * 1e3,
                      "flops_per_launch": FLOPS_PER_INTERACTION * inter},
         "accuracy": {"vs": "FP64 GPU direct sum, 4096 sampled targets", "median_rel": med, "p99_rel": p99,
@@ -127,39 +291,11 @@ def main():
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "cu": info.cu_count,
                    "clock_khz": info.clock_khz},
     }
-
     if not args.no_bh:
-        xb, vb, mb = ic.two_spheres(N_BH, 4)
-        pb = L.param(3, theta=0.5, leaf=1)
-        simb = L.Sim(xb, vb, mb, 1.0)
-        simb.acceleration(pb, copy=False)
-        L.synchronize()
-        L.timing_enable(True)
-        L.timing_reset()
-        reps = 5
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            simb.acceleration(pb, copy=False)
-        L.synchronize()
-        bwall = (time.perf_counter() - t0) / reps
-        wk_ms, wk_calls = L.timing_query("bh_walk")
-        bd_ms, bd_calls = L.timing_query("tree_build")
-        so_ms, so_calls = L.timing_query("tree_sort")
-        L.timing_enable(False)
-        nint = L.last_walk_interactions()
-        bmed, bp99, bmx = simb.accel_error(pb, samples=4096)
-        simb.close()
-        walk_s = wk_ms / max(wk_calls, 1) / 1e3
-        out["bh"] = {
-            "workload": "two Plummer spheres N=2^20 (seed 4), theta=0.5, leaf=1, monopole",
-            "plain": {"particles_per_s": N_BH / bwall, "ms_per_eval": bwall * 1e3,
-                      "walk_ms": walk_s * 1e3, "build_ms": bd_ms / max(bd_calls, 1),
-                      "sort_ms": so_ms / max(so_calls, 1),
-                      "interactions": nint,
-                      "walk_tflops": FLOPS_PER_INTERACTION * nint / walk_s / 1e12,
-                      "median_rel": bmed, "p99_rel": bp99, "max_rel": bmx},
-        }
-
+        out["bh"] = run_bh(L, ic)
+    if not args.no_whfast:
+        out["whfast"] = run_whfast(L, ic, args.whfast_steps)
+        log(f"whfast: {out['whfast']['particle_steps_per_s']:.3e} particle-steps/s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, m, args.cpu_sample)
     print(json.dumps(out), flush=True)
