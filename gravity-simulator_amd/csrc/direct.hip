@@ -217,7 +217,7 @@ __global__ __launch_bounds__(BS) void k_direct_f32(const double *__restrict__ x,
 // every lane converts its FP64 target position to the same tile frame once
 // per tile, then streams the tile's sources through wave-uniform (scalar)
 // loads, so the sources sit in SGPRs and no LDS or barrier is involved.
-constexpr int ST_TILE = 128;
+constexpr int ST_TILE = 256;
 
 __global__ void k_f32_sources(const double *__restrict__ x, const double *__restrict__ m, int n,
                               int n_pad, float4 *__restrict__ S, double4 *__restrict__ C)
@@ -240,7 +240,7 @@ __global__ void k_f32_sources(const double *__restrict__ x, const double *__rest
 }
 
 template <int IPT, bool SOFT>
-__global__ __launch_bounds__(256) void k_direct_f32s(const double *__restrict__ x, int n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IPT <= 2 ? 8 : 4))) void k_direct_f32s(const double *__restrict__ x, int n,
                                                      const float4 *__restrict__ S,
                                                      const double4 *__restrict__ C, float eps2,
                                                      int tiles_per_chunk, int ntiles,
@@ -275,7 +275,11 @@ __global__ __launch_bounds__(256) void k_direct_f32s(const double *__restrict__ 
             px[r] = py[r] = pz[r] = 0.0f;
         }
         const float4 *__restrict__ Sj = S + static_cast<size_t>(J) * ST_TILE;
-#pragma unroll 16
+        // unrolled to 16 sources per step with one target per lane; with
+        // several targets per lane 4 sources per step keep the registers
+        // within 8 waves per SIMD
+        constexpr int UNROLL = IPT == 1 ? 16 : 4;
+#pragma unroll UNROLL
         for (int jj = 0; jj < ST_TILE; jj++) {
             const float4 s = Sj[jj];
 #pragma unroll
@@ -331,14 +335,15 @@ Fp32Config &fp32_config()
 }
 
 // Source chunks per evaluation when the configuration leaves it to us:
-// about 2048 sources per chunk (64 chunks at N = 2^17: the best measured
-// balance on MI355X between enough workgroups to fill 256 CUs and the cost
-// of the FP64 partial sums written per chunk and reduced afterwards).
+// about 1024 sources per chunk (128 chunks at N = 2^17).  Measured on
+// MI355X at N = 2^17 with 256-source tiles (kernel only): 2048 per chunk
+// 4.36 ms, 1024 4.35 ms, 512 4.34 ms; each halving doubles the FP64 partial
+// sums (402 MB at 1024) that the reduction reads back (~0.09 ms).
 static int resolve_split(int n, const Fp32Config &cfg, int cu_count)
 {
     (void)cu_count;
     if (cfg.jsplit > 0) return cfg.jsplit;
-    const int per_chunk = 2048;
+    const int per_chunk = 1024;
     const int split = (n + per_chunk - 1) / per_chunk;
     return split < 1 ? 1 : split;
 }

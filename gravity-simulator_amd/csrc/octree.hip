@@ -803,6 +803,8 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
         unsigned long long h = 0;
         GS_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
         GS_HIP(hipStreamSynchronize(c->stream));
+        if (h >> 62)
+            return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut walk stack overflow");
         *interactions = static_cast<long long>(h);
     }
     return make_success_error_status();
