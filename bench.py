@@ -49,10 +49,12 @@ FLOPS_PER_INTERACTION = 20.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 P99_TARGET = 1e-3
-# HBM traffic of one direct_f32 launch at N = 2^17 (64 source chunks):
-# rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), copy-kernel calibrated,
+# HBM bytes of one direct_f32 launch at N = 2^17 (128 source chunks):
+# rocprofv3 --pmc FETCH_SIZE 18119 KiB x 2.0 + WRITE_SIZE 393216 KiB x 1.0
+# (factors from the copy kernel: 256 MiB in / out reads 131083 / 262144 KiB);
+# 402.7 MB of it are the FP64 per-chunk partial sums.
 # profiles/direct_f32_plummer_n131072.md
-DIRECT_F32_TRAFFIC_BYTES = None
+DIRECT_F32_TRAFFIC_BYTES = (2.0 * 18119.4 + 393216.0) * 1024
 
 
 def log(msg):
@@ -115,7 +117,7 @@ def bh_case(L, sim, n, method, theta, reps=5, label=None):
     return r
 
 
-def run_bh(L, ic):
+def run_bh(L, ic, learned=True):
     out = {}
     # N = 2^20, the reference's method
     n4 = 1 << 20
@@ -150,6 +152,10 @@ def run_bh(L, ic):
             break
     log("BH 2^21 monopole / quadrupole at target done")
 
+    if not learned:
+        sim.close()
+        out["n2e21"] = res
+        return out
     # learned correction, the specified coarse walk (theta = 0.8), spot check on
     t0 = time.perf_counter()
     loss = L.learned_train(xt, mt, eps=EPS, theta=0.8, leaf=1, samples=1000000, epochs=10, batch=4096,
@@ -228,6 +234,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--no-whfast", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="Barnes-Hut without the learned correction")
     ap.add_argument("--autotune", action="store_true",
                     help="run the FP32 shape autotuner first (default: the library's default shape)")
     args = ap.parse_args()
@@ -292,7 +299,7 @@ def main():
                    "clock_khz": info.clock_khz},
     }
     if not args.no_bh:
-        out["bh"] = run_bh(L, ic)
+        out["bh"] = run_bh(L, ic, learned=not args.no_learned)
     if not args.no_whfast:
         out["whfast"] = run_whfast(L, ic, args.whfast_steps)
         log(f"whfast: {out['whfast']['particle_steps_per_s']:.3e} particle-steps/s")

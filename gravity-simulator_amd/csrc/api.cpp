@@ -430,8 +430,12 @@ ErrorStatus gs_sim_accel_error(GsSim *s, const AccelerationParam *p, int num_sam
     GS_HIP(hipMemcpyAsync(dt, targets.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
     const bool skip = p->method == ACCELERATION_METHOD_MASSLESS ||
                       p->method == ACCELERATION_METHOD_MASSLESS_FP32;
-    GS_HIP(launch_direct_f64_targets(c, s->x, s->m, n, s->G, p->softening_length, skip,
-                                     static_cast<int *>(dt), ns, static_cast<double *>(dr)));
+    const size_t pb = direct_f64_sampled_bytes(n, ns);
+    void *part;
+    GS_TRY(slot(c, SLOT_PARTIAL, pb, &part));
+    GS_HIP(launch_direct_f64_sampled(c, s->x, s->m, n, s->G, p->softening_length, skip,
+                                     static_cast<int *>(dt), ns, static_cast<double *>(dr), part,
+                                     pb));
     GS_HIP(hipMemcpyAsync(a.data(), s->a, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipMemcpyAsync(ref.data(), dr, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
                           c->stream));

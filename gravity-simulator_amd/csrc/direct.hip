@@ -108,6 +108,104 @@ hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *
     return hipGetLastError();
 }
 
+// FP64 sums of one source chunk [blockIdx.y * chunk, +chunk) for a list of
+// targets, the same per-term arithmetic as k_direct_f64; partial sums per
+// chunk (without G) for k_reduce_partials.  Used for the sampled error
+// reference, where a few thousand targets alone would occupy a handful of
+// CUs for the whole source range.
+template <bool SKIP_MASSLESS>
+__global__ __launch_bounds__(F64_BLOCK) void k_direct_f64_part(
+    const double *__restrict__ x, const double *__restrict__ m, int n, double eps2,
+    const int *__restrict__ targets, int num_targets, int chunk, double *__restrict__ partial)
+{
+    __shared__ double sx[F64_BLOCK], sy[F64_BLOCK], sz[F64_BLOCK], sm[F64_BLOCK];
+    const int t = blockIdx.x * F64_BLOCK + threadIdx.x;
+    const bool live = t < num_targets;
+    const int k = live ? targets[t] : -1;
+    double xk0 = 0.0, xk1 = 0.0, xk2 = 0.0;
+    if (live) {
+        xk0 = x[3 * k + 0];
+        xk1 = x[3 * k + 1];
+        xk2 = x[3 * k + 2];
+    }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+    for (int base = j0; base < j1; base += F64_BLOCK) {
+        const int j = base + threadIdx.x;
+        if (j < j1) {
+            sx[threadIdx.x] = x[3 * j + 0];
+            sy[threadIdx.x] = x[3 * j + 1];
+            sz[threadIdx.x] = x[3 * j + 2];
+            sm[threadIdx.x] = m[j];
+        }
+        __syncthreads();
+        const int cnt = min(F64_BLOCK, j1 - base);
+        if (live) {
+            for (int jj = 0; jj < cnt; jj++) {
+                const double ml = sm[jj];
+                if (base + jj == k) continue;
+                if (SKIP_MASSLESS && ml == 0.0) continue;
+                const double R0 = sx[jj] - xk0;
+                const double R1 = sy[jj] - xk1;
+                const double R2 = sz[jj] - xk2;
+                const double r = __builtin_sqrt(R0 * R0 + R1 * R1 + R2 * R2 + eps2);
+                const double tv = 1.0 / (r * r * r);
+                a0 += (tv * R0) * ml;
+                a1 += (tv * R1) * ml;
+                a2 += (tv * R2) * ml;
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double *o = partial + (static_cast<size_t>(blockIdx.y) * num_targets + t) * 3;
+        o[0] = a0;
+        o[1] = a1;
+        o[2] = a2;
+    }
+}
+
+__global__ void k_reduce_partials(const double *__restrict__ partial, int nsplit, int n3,
+                                  double G, double *__restrict__ a);
+
+static int f64_part_split(int n, int nt)
+{
+    const int wgs = (nt + F64_BLOCK - 1) / F64_BLOCK;
+    int split = (2048 + wgs - 1) / wgs;
+    const int max_split = (n + F64_BLOCK - 1) / F64_BLOCK;  // at least one source tile each
+    if (split > max_split) split = max_split;
+    return split < 1 ? 1 : split;
+}
+
+size_t direct_f64_sampled_bytes(int n, int nt)
+{
+    return static_cast<size_t>(f64_part_split(n, nt)) * 3 * static_cast<size_t>(nt) * sizeof(double);
+}
+
+hipError_t launch_direct_f64_sampled(Context *c, const double *x, const double *m, int n, double G,
+                                     double eps, bool skip_massless, const int *targets, int nt,
+                                     double *out, void *buf, size_t bytes)
+{
+    if (nt <= 0) return hipSuccess;
+    if (bytes < direct_f64_sampled_bytes(n, nt)) return hipErrorInvalidValue;
+    const int split = f64_part_split(n, nt);
+    const int chunk = (n + split - 1) / split;
+    const int nsplit = (n + chunk - 1) / chunk;
+    double *partial = static_cast<double *>(buf);
+    dim3 grid((nt + F64_BLOCK - 1) / F64_BLOCK, nsplit);
+    if (skip_massless)
+        hipLaunchKernelGGL(k_direct_f64_part<true>, grid, dim3(F64_BLOCK), 0, c->stream, x, m, n,
+                           eps * eps, targets, nt, chunk, partial);
+    else
+        hipLaunchKernelGGL(k_direct_f64_part<false>, grid, dim3(F64_BLOCK), 0, c->stream, x, m, n,
+                           eps * eps, targets, nt, chunk, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((3 * nt + 255) / 256), dim3(256), 0, c->stream,
+                       partial, nsplit, 3 * nt, G, out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ FP32 split
 // Kahan step: fold tile partial p into (s, comp).  Contraction is off for
 // this file, so the compensation survives as written.

@@ -122,6 +122,12 @@ hipError_t launch_direct_f32_targets(Context *c, const double *x, const double *
 size_t direct_f32_targets_bytes(int n, int nt);
 // out[t] = src[idx[t]] for rows of 3 doubles.
 hipError_t launch_gather_rows3(Context *c, const double *src, const int *idx, int nt, double *out);
+// FP64 sums for the listed targets split over source chunks (the sampled
+// error reference: same terms, chunked summation order).
+hipError_t launch_direct_f64_sampled(Context *c, const double *x, const double *m, int n, double G,
+                                     double eps, bool skip_massless, const int *targets, int nt,
+                                     double *out, void *buf, size_t bytes);
+size_t direct_f64_sampled_bytes(int n, int nt);
 // FP64 reference-order sums for the listed targets.
 hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *m, int n,
                                      double G, double eps, bool skip_massless, const int *targets,
