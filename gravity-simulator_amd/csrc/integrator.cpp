@@ -2,7 +2,7 @@
 // GPU (src/grav_sim.c:61-138, src/integrator.c:191-279).  Leapfrog runs
 // device-resident with the reference's operation order
 // (src/integrator.c:894-1121); WHFast runs the GPU WHFast
-// (src/integrator_whfast.c:200-407).
+// (src/integrator_whfast.c:200-407); IAS15 the GPU IAS15 (ias15.hip).
 #include <cmath>
 #include <cstdio>
 
@@ -11,6 +11,8 @@
 namespace gs {
 ErrorStatus whfast_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
                        SimulationStatus *status, Settings *settings, double tf);
+ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
+                      SimulationStatus *status, Settings *settings, double tf);
 }
 
 using namespace gs;
@@ -44,6 +46,28 @@ static ErrorStatus leapfrog_gpu(System *system, IntegratorParam *ip, Acceleratio
 
 extern "C" {
 
+ErrorStatus ias15(System *system, IntegratorParam *ip, AccelerationParam *ap, OutputParam *op,
+                  SimulationStatus *status, Settings *settings, const double tf)
+{
+    if (!system || !ip || !ap || !op || !status || !settings)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    if (op->method != OUTPUT_METHOD_DISABLED)
+        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
+                                "Snapshot output is not provided by the GPU library");
+    return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, status, settings, tf));
+}
+
+ErrorStatus whfast(System *system, IntegratorParam *ip, AccelerationParam *ap, OutputParam *op,
+                   SimulationStatus *status, Settings *settings, const double tf)
+{
+    if (!system || !ip || !ap || !op || !status || !settings)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    if (op->method != OUTPUT_METHOD_DISABLED)
+        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
+                                "Snapshot output is not provided by the GPU library");
+    return WRAP_TRACEBACK(whfast_gpu(system, ip, ap, status, settings, tf));
+}
+
 ErrorStatus integrator_launch_simulation(System *system, IntegratorParam *ip,
                                          AccelerationParam *ap, OutputParam *op,
                                          SimulationStatus *status, Settings *settings,
@@ -59,6 +83,8 @@ ErrorStatus integrator_launch_simulation(System *system, IntegratorParam *ip,
             return WRAP_TRACEBACK(leapfrog_gpu(system, ip, ap, status, settings, tf));
         case INTEGRATOR_WHFAST:
             return WRAP_TRACEBACK(whfast_gpu(system, ip, ap, status, settings, tf));
+        case INTEGRATOR_IAS15:
+            return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, status, settings, tf));
         case INTEGRATOR_EULER:
         case INTEGRATOR_EULER_CROMER:
         case INTEGRATOR_RK4:
@@ -66,7 +92,6 @@ ErrorStatus integrator_launch_simulation(System *system, IntegratorParam *ip,
         case INTEGRATOR_DOPRI:
         case INTEGRATOR_DVERK:
         case INTEGRATOR_RKF78:
-        case INTEGRATOR_IAS15:
             return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
                                         "Integrator %d is not provided by the GPU library; "
                                         "call acceleration() from the host integrator",

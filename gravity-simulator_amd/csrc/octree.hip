@@ -693,6 +693,9 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
                        const double *box_center, double box_width, DeviceTree **out)
 {
     DeviceTree *t = &g_tree;
+    // an unfinalised parameter (-1) means the reference's default of one
+    // particle per leaf (src/acceleration.c:73-129)
+    if (leaf_cap < 1) leaf_cap = 1;
     GS_TRY(tree_sort(c, x, n, box_center, box_width, t));
     t->leaf_cap = leaf_cap;
     void *p;

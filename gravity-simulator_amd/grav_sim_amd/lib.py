@@ -40,7 +40,7 @@ EXPORTS = [
     "gs_set_fp32_config", "gs_copy_calibrate", "gs_whfast_ensemble",
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
     "gs_learned_reset", "gs_learned_forward", "gs_learned_gradients", "gs_learned_features",
-    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report",
+    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "ias15", "whfast",
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
     "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
 ]
@@ -119,6 +119,10 @@ def load(path=None):
         "gs_learned_set_policy": (ES, [C.c_double, C.c_int, C.c_double]),
         "gs_learned_status": (ES, [PD, PI]),
         "gs_learned_training_report": (ES, [PD, PD, PD, PD, PD]),
+        "ias15": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
+                       C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
+        "whfast": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
+                        C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -313,7 +317,7 @@ class Sim:
 
 
 def launch_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, theta=1.0, leaf=-1,
-                      remove_invalid=True):
+                      remove_invalid=True, tolerance=-1.0, initial_dt=-1.0):
     """launch_simulation with output disabled; returns (ids, x, v, m, steps)."""
     L = load()
     x, v, m = _c(x).copy(), _c(v).copy(), _c(m).copy()
@@ -323,6 +327,8 @@ def launch_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, theta=1
     ip.integrator = integrator
     ip.dt = dt
     ip.whfast_remove_invalid_particles = remove_invalid
+    ip.tolerance = tolerance
+    ip.initial_dt = initial_dt
     ap = param(method, theta, eps, leaf)
     op = L.get_new_output_param()
     st = _abi.SimulationStatus()

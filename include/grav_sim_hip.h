@@ -292,13 +292,30 @@ ErrorStatus finalize_integration_param(IntegratorParam *GS_RESTRICT integration_
 /* replaces src/integrator.h:62-70 (src/integrator.c:191-279).  LEAPFROG runs
  * device-resident on the GPU (src/integrator.c:894-1121 semantics, identical
  * FP64 op order); WHFAST runs the GPU WHFast (src/integrator_whfast.c:200-407
- * semantics).  The other integrators return GRAV_NOT_IMPLEMENTED_ERROR. */
+ * semantics); IAS15 runs the device-resident IAS15 below.  The other
+ * integrators return GRAV_NOT_IMPLEMENTED_ERROR. */
 ErrorStatus integrator_launch_simulation(System *GS_RESTRICT system,
                                          IntegratorParam *GS_RESTRICT integrator_param,
                                          AccelerationParam *GS_RESTRICT acceleration_param,
                                          OutputParam *GS_RESTRICT output_param,
                                          SimulationStatus *GS_RESTRICT simulation_status,
                                          Settings *GS_RESTRICT settings, const double tf);
+
+/* replaces src/integrator.h:108-116 (src/integrator_ias15.c:229-615):
+ * IAS15 with the reference's predictor-corrector, step-size control and
+ * compensated sums, every per-component update an FP64 GPU kernel in the
+ * reference's operation order and the forces from acceleration_param on the
+ * device (bitwise equal to the reference with the FP64 direct sums).  Output
+ * must be OUTPUT_METHOD_DISABLED. */
+ErrorStatus ias15(System *system, IntegratorParam *integrator_param,
+                  AccelerationParam *acceleration_param, OutputParam *output_param,
+                  SimulationStatus *simulation_status, Settings *settings, const double tf);
+
+/* replaces src/integrator.h:131-139 (src/integrator_whfast.c:200-407): the GPU
+ * WHFast on one system (gs_whfast_ensemble batches many). */
+ErrorStatus whfast(System *system, IntegratorParam *integrator_param,
+                   AccelerationParam *acceleration_param, OutputParam *output_param,
+                   SimulationStatus *simulation_status, Settings *settings, const double tf);
 
 /* replaces src/grav_sim.h:47-55 (src/grav_sim.c:61-138) */
 ErrorStatus launch_simulation(System *GS_RESTRICT system,

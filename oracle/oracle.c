@@ -6,6 +6,7 @@
  * reference expression it restates, so results are bitwise reproducible.
  */
 #include "oracle.h"
+#include "ias15_coeffs.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -965,4 +966,252 @@ int orc_whfast(int *np, int32_t *ids, double *x, double *v, double *m, double G,
     free(eta);
     free(rm);
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* IAS15 (src/integrator_ias15.c:229-615 loop, :702-1382 pieces)       */
+/* ------------------------------------------------------------------ */
+
+/* state arrays are [slot][n*3] blocks */
+#define IAS_AT(arr, slot, q, n3) ((arr)[(size_t)(slot) * (n3) + (q)])
+
+static double orc_abs_max(const double *a, size_t len)
+{
+    double mx = fabs(a[0]);
+    for (size_t q = 1; q < len; q++)
+        mx = fmax(mx, fabs(a[q]));
+    return mx;
+}
+
+/* position and velocity at the spacing h (src/integrator_ias15.c:808-919):
+ * Horner forms of x0 + dt h (v0 + dt h (a0 + h (b0/3 + ... )) / 2), with
+ * the compensation term added before the increment */
+static void ias_predict(double *x1, double *v1, size_t n3, const double *x, const double *v,
+                        const double *a, const double *b, double h, double dt,
+                        const double *xc, const double *vc)
+{
+    for (size_t q = 0; q < n3; q++)
+    {
+        const double b0 = IAS_AT(b, 0, q, n3), b1 = IAS_AT(b, 1, q, n3), b2 = IAS_AT(b, 2, q, n3),
+                     b3 = IAS_AT(b, 3, q, n3), b4 = IAS_AT(b, 4, q, n3), b5 = IAS_AT(b, 5, q, n3),
+                     b6 = IAS_AT(b, 6, q, n3);
+        double px = b5 / 28.0 + h * b6 / 36.0;
+        px = b4 / 21.0 + h * px;
+        px = b3 / 15.0 + h * px;
+        px = b2 / 10.0 + h * px;
+        px = b1 / 6.0 + h * px;
+        px = b0 / 3.0 + h * px;
+        px = a[q] + h * px;
+        x1[q] = x[q];
+        x1[q] += xc[q] + dt * h * (v[q] + dt * h * px / 2.0);
+
+        double pv = b5 / 7.0 + h * b6 / 8.0;
+        pv = b4 / 6.0 + h * pv;
+        pv = b3 / 5.0 + h * pv;
+        pv = b2 / 4.0 + h * pv;
+        pv = b1 / 3.0 + h * pv;
+        pv = b0 / 2.0 + h * pv;
+        v1[q] = v[q];
+        v1[q] += vc[q] + dt * h * (a[q] + h * pv);
+    }
+}
+
+/* g and b after the force at spacing i (src/integrator_ias15.c:1117-1274
+ * and :985-1115): g_k for k < i by the divided-difference recurrence, then
+ * b_k for k < i from all seven g */
+static void ias_update_gb(double *g, double *b, size_t n3, const double *F, int i,
+                          const Ias15Coeffs *k)
+{
+    for (size_t q = 0; q < n3; q++)
+    {
+        for (int j = 0; j < i && j < 7; j++)
+        {
+            double t = (IAS_AT(F, j + 1, q, n3) - IAS_AT(F, 0, q, n3)) * k->r[j + 1][0];
+            for (int l = 1; l <= j; l++)
+                t = (t - IAS_AT(g, l - 1, q, n3)) * k->r[j + 1][l];
+            IAS_AT(g, j, q, n3) = t;
+        }
+        for (int j = 0; j < i && j < 7; j++)
+        {
+            double s = k->c[j][j] * IAS_AT(g, j, q, n3);
+            for (int l = j + 1; l < 7; l++)
+                s = s + k->c[l][j] * IAS_AT(g, l, q, n3);
+            IAS_AT(b, j, q, n3) = s;
+        }
+    }
+}
+
+/* the step's end point with compensated sums (src/integrator_ias15.c:921-983) */
+static void ias_step(double *x1, double *v1, double *txc, double *tvc, size_t n3, const double *x,
+                     const double *v, const double *a, const double *b, double dt)
+{
+    for (size_t q = 0; q < n3; q++)
+    {
+        const double b0 = IAS_AT(b, 0, q, n3), b1 = IAS_AT(b, 1, q, n3), b2 = IAS_AT(b, 2, q, n3),
+                     b3 = IAS_AT(b, 3, q, n3), b4 = IAS_AT(b, 4, q, n3), b5 = IAS_AT(b, 5, q, n3),
+                     b6 = IAS_AT(b, 6, q, n3);
+        const double sx = a[q] + b0 / 3.0 + b1 / 6.0 + b2 / 10.0 + b3 / 15.0 + b4 / 21.0 +
+                          b5 / 28.0 + b6 / 36.0;
+        txc[q] += dt * (v[q] + dt * sx / 2.0);
+        x1[q] = x[q] + txc[q];
+        txc[q] += (x[q] - x1[q]);
+
+        const double sv = a[q] + b0 / 2.0 + b1 / 3.0 + b2 / 4.0 + b3 / 5.0 + b4 / 6.0 +
+                          b5 / 7.0 + b6 / 8.0;
+        tvc[q] += dt * sv;
+        v1[q] = v[q] + tvc[q];
+        tvc[q] += (v[q] - v1[q]);
+    }
+}
+
+/* b predicted for the next step from the ratio of step sizes, plus the
+ * correction learned from the last prediction (src/integrator_ias15.c:1276-1382) */
+static void ias_refine(double *b, double *e, size_t n3, double dt, double dt_new, int refine)
+{
+    const double q1 = dt_new / dt;
+    const double q2 = q1 * q1, q3 = q2 * q1, q4 = q3 * q1, q5 = q4 * q1, q6 = q5 * q1,
+                 q7 = q6 * q1;
+    for (size_t q = 0; q < n3; q++)
+    {
+        double db[7];
+        for (int j = 0; j < 7; j++)
+            db[j] = refine ? IAS_AT(b, j, q, n3) - IAS_AT(e, j, q, n3) : 0.0;
+        const double b0 = IAS_AT(b, 0, q, n3), b1 = IAS_AT(b, 1, q, n3), b2 = IAS_AT(b, 2, q, n3),
+                     b3 = IAS_AT(b, 3, q, n3), b4 = IAS_AT(b, 4, q, n3), b5 = IAS_AT(b, 5, q, n3),
+                     b6 = IAS_AT(b, 6, q, n3);
+        double en[7];
+        en[0] = q1 * (b6 * 7.0 + b5 * 6.0 + b4 * 5.0 + b3 * 4.0 + b2 * 3.0 + b1 * 2.0 + b0);
+        en[1] = q2 * (b6 * 21.0 + b5 * 15.0 + b4 * 10.0 + b3 * 6.0 + b2 * 3.0 + b1);
+        en[2] = q3 * (b6 * 35.0 + b5 * 20.0 + b4 * 10.0 + b3 * 4.0 + b2);
+        en[3] = q4 * (b6 * 35.0 + b5 * 15.0 + b4 * 5.0 + b3);
+        en[4] = q5 * (b6 * 21.0 + b5 * 6.0 + b4);
+        en[5] = q6 * (b6 * 7.0 + b5);
+        en[6] = q7 * b6;
+        for (int j = 0; j < 7; j++)
+        {
+            IAS_AT(e, j, q, n3) = en[j];
+            IAS_AT(b, j, q, n3) = en[j] + db[j];
+        }
+    }
+}
+
+/* Initial step size (src/integrator_ias15.c:702-806 ias15_initial_dt) */
+static double ias_initial_dt(int n, const double *x, const double *v, const double *m, double G,
+                             int method, double eps, const double *a)
+{
+    const size_t n3 = (size_t)n * 3;
+    double *x1 = malloc(n3 * sizeof(double));
+    double *a1 = malloc(n3 * sizeof(double));
+    const double d0 = orc_abs_max(x, n3), d1 = orc_abs_max(a, n3);
+    const double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+    for (size_t q = 0; q < n3; q++)
+        x1[q] = x[q] + dt0 * v[q];
+    accel(a1, n, x1, m, G, method, eps);
+    for (size_t q = 0; q < n3; q++)
+        a1[q] = a1[q] - a[q];
+    const double d2 = orc_abs_max(a1, n3) / dt0;
+    double dt1;
+    if (fmax(d1, d2) <= 1e-15)
+        dt1 = fmax(1e-6, dt0 * 1e-3);
+    else
+        dt1 = pow((0.01 / fmax(d1, d2)), 1.0 / (1.0 + (double)15));
+    free(x1);
+    free(a1);
+    return fmin(100.0 * dt0, dt1);
+}
+
+int64_t orc_ias15(double *x, double *v, int n, const double *m, double G, int method, double eps,
+                  double tol, double initial_dt, double tf)
+{
+    Ias15Coeffs K;
+    ias15_coeffs(&K);
+    const size_t n3 = (size_t)n * 3;
+    double *a = malloc(n3 * sizeof(double));
+    double *x1 = calloc(n3, sizeof(double)), *v1 = calloc(n3, sizeof(double));
+    double *a1 = calloc(n3, sizeof(double));
+    double *F = calloc(8 * n3, sizeof(double));
+    double *g = calloc(7 * n3, sizeof(double)), *b = calloc(7 * n3, sizeof(double));
+    double *b0 = calloc(7 * n3, sizeof(double)), *e = calloc(7 * n3, sizeof(double));
+    double *xc = calloc(n3, sizeof(double)), *vc = calloc(n3, sizeof(double));
+    double *txc = calloc(n3, sizeof(double)), *tvc = calloc(n3, sizeof(double));
+    double *d7 = calloc(n3, sizeof(double));
+    const double safety = 0.25, exponent = 1.0 / 7.0, tol_pc = 1e-16;
+
+    accel(a, n, x, m, G, method, eps);
+    double dt;
+    if (initial_dt > 0.0)
+        dt = initial_dt;
+    else
+    {
+        dt = ias_initial_dt(n, x, v, m, G, method, eps, a);
+        if (dt > tf)
+            dt = tf;
+    }
+    double t = 0.0;
+    int64_t steps = 0;
+    int refine = 0;
+    while (t < tf)
+    {
+        for (int it = 0; it < 12; it++)
+        {
+            for (int i = 0; i < 8; i++)
+            {
+                ias_predict(x1, v1, n3, x, v, a, b, K.h[i], dt, xc, vc);
+                accel(F + (size_t)i * n3, n, x1, m, G, method, eps);
+                ias_update_gb(g, b, n3, F, i, &K);
+            }
+            for (size_t q = 0; q < n3; q++)
+                d7[q] = IAS_AT(b, 6, q, n3) - IAS_AT(b0, 6, q, n3);
+            memcpy(b0, b, 7 * n3 * sizeof(double));
+            if (orc_abs_max(d7, n3) / orc_abs_max(F + 7 * n3, n3) < tol_pc)
+                break;
+        }
+        memcpy(txc, xc, n3 * sizeof(double));
+        memcpy(tvc, vc, n3 * sizeof(double));
+        ias_step(x1, v1, txc, tvc, n3, x, v, a, b, dt);
+        accel(a1, n, x1, m, G, method, eps);
+
+        const double err_b7 = orc_abs_max(b + 6 * n3, n3) / orc_abs_max(a1, n3);
+        double err = pow((err_b7 / tol), exponent);
+        if (err < 1e-10)
+            err = 1e-10;
+        const double dt_new = dt / err;
+        if (err <= 1.0 || dt == tf * 1e-12)
+        {
+            steps++;
+            t += dt;
+            ias_refine(b, e, n3, dt, dt_new, refine);
+            refine = 1;
+            memcpy(xc, txc, n3 * sizeof(double));
+            memcpy(vc, tvc, n3 * sizeof(double));
+            memcpy(x, x1, n3 * sizeof(double));
+            memcpy(v, v1, n3 * sizeof(double));
+            memcpy(a, a1, n3 * sizeof(double));
+        }
+        if (dt_new > (dt / safety))
+            dt /= safety;
+        else if (dt_new < dt * safety)
+            dt *= safety;
+        else
+            dt = dt_new;
+        if (dt_new < tf * 1e-12)
+            dt = tf * 1e-12;
+        if ((t < tf) && (t + dt > tf))
+            dt = tf - t;
+    }
+    free(a);
+    free(x1);
+    free(v1);
+    free(a1);
+    free(F);
+    free(g);
+    free(b);
+    free(b0);
+    free(e);
+    free(xc);
+    free(vc);
+    free(txc);
+    free(tvc);
+    free(d7);
+    return steps;
 }

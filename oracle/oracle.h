@@ -119,6 +119,14 @@ void orc_stumpff(double z, double c[4]);
 int orc_whfast(int *n, int32_t *ids, double *x, double *v, double *m, double G, int method,
                double eps, double dt, double tf, int remove_invalid);
 
+/* IAS15 (src/integrator_ias15.c:229-1382): Gauss-Radau predictor-corrector
+ * with adaptive steps and compensated sums, the reference's operation order,
+ * using orc_pairwise (method 1) or orc_massless (2).  initial_dt <= 0 picks
+ * the reference's automatic initial step.  x, v updated in place; returns the
+ * number of accepted steps. */
+int64_t orc_ias15(double *x, double *v, int n, const double *m, double G, int method, double eps,
+                  double tol, double initial_dt, double tf);
+
 #ifdef __cplusplus
 }
 #endif

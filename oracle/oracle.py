@@ -73,6 +73,8 @@ def lib():
         L.orc_stumpff.argtypes = [d, PD]
         L.orc_whfast.argtypes = [C.POINTER(i), PI, PD, PD, PD, d, i, d, d, d, i]
         L.orc_whfast.restype = i
+        L.orc_ias15.argtypes = [PD, PD, i, PD, d, i, d, d, d, d]
+        L.orc_ias15.restype = i64
         _lib = L
     return _lib
 
@@ -253,6 +255,14 @@ def whfast(x, v, m, G, dt, tf, eps=0.0, method=2, remove_invalid=True, ids=None)
     return ids[:k].copy(), x[:k].copy(), v[:k].copy(), m[:k].copy()
 
 
+def ias15(x, v, m, G, tol, tf, eps=0.0, method=1, initial_dt=-1.0):
+    """Returns (x, v, accepted steps)."""
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m)
+    steps = lib().orc_ias15(_abi.dptr(x), _abi.dptr(v), len(m), _abi.dptr(m), G, method, eps, tol,
+                            initial_dt, tf)
+    return x, v, steps
+
+
 # ---------------------------------------------------------------- reference build
 def ref_acceleration(x, m, method=1, G=1.0, eps=0.0, theta=1.0, leaf=-1):
     R = ref()
@@ -315,7 +325,8 @@ def ref_octree(x, m, leaf_cap=1):
     return out
 
 
-def ref_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, remove_invalid=True):
+def ref_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, remove_invalid=True, tolerance=-1.0,
+                   initial_dt=-1.0):
     """Reference launch_simulation with output disabled; returns (ids, x, v, m, steps)."""
     R = ref()
     x, v, m = _c(x).copy(), _c(v).copy(), _c(m).copy()
@@ -325,6 +336,8 @@ def ref_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, remove_inv
     ip.integrator = integrator
     ip.dt = dt
     ip.whfast_remove_invalid_particles = remove_invalid
+    ip.tolerance = tolerance
+    ip.initial_dt = initial_dt
     ap = R.get_new_acceleration_param()
     ap.method = method
     ap.softening_length = eps

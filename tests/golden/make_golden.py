@@ -24,6 +24,11 @@ exported functions on seeded inputs and stores inputs and outputs as data:
   f6_whfast.npz     8 systems of the ensemble (seed 3), WHFast massless,
                     dt=0.01, 10^4 steps: final ids, x, v
                     (src/integrator_whfast.c:200-407)
+  f8_ias15.npz      IAS15 (src/integrator_ias15.c:229-1382), automatic initial
+                    step: the Pythagorean three-body problem (masses 3, 4, 5
+                    over the default G, src/system.c:888-930), tolerance 1e-9,
+                    t = 70; and N=64 Plummer (seed 8), eps=1e-2, tolerance
+                    1e-9, t = 2: final x, v and step counts
 """
 import hashlib
 import os
@@ -106,8 +111,31 @@ def f6():
     np.savez_compressed(os.path.join(HERE, "f6_whfast.npz"), **out)
 
 
+G_DEFAULT = 0.000295912208284119496676630  # src/system.c:34
+
+
+def pyth3():
+    x = np.array([[1.0, 3.0, 0.0], [-2.0, -1.0, 0.0], [1.0, -1.0, 0.0]])
+    v = np.zeros((3, 3))
+    m = np.array([3.0, 4.0, 5.0]) / G_DEFAULT
+    return x, v, m
+
+
+def f8():
+    out = {}
+    x, v, m = pyth3()
+    ids, xf, vf, mf, steps = O.ref_simulation(x, v, m, G_DEFAULT, 9, -1.0, 70.0, method=1, tolerance=1e-9)
+    out.update(p3_x=x, p3_v=v, p3_m=m, p3_G=G_DEFAULT, p3_x_final=xf, p3_v_final=vf, p3_steps=steps)
+    print("pyth-3-body steps", steps, flush=True)
+    x, v, m = ic.plummer(64, 8)
+    ids, xf, vf, mf, steps = O.ref_simulation(x, v, m, 1.0, 9, -1.0, 2.0, method=1, eps=1e-2, tolerance=1e-9)
+    out.update(pl_x=x, pl_v=v, pl_m=m, pl_x_final=xf, pl_v_final=vf, pl_steps=steps)
+    print("plummer-64 steps", steps, flush=True)
+    np.savez_compressed(os.path.join(HERE, "f8_ias15.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6"]
+    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f8"]
     for w in which:
-        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6}[w]()
+        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f8": f8}[w]()
         print("wrote", w, flush=True)

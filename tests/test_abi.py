@@ -175,3 +175,28 @@ def test_python_api_symbols_and_host_helpers():
     lib.compute_linear_momentum_python.argtypes = [_abi.PD, _abi.PD, C.c_int, C.c_int]
     lib.compute_linear_momentum_python(_abi.dptr(lm), _abi.dptr(np.ascontiguousarray(sol)), 2, 5)
     assert np.allclose(lm, (sol[..., 0] * sol[..., 4:7].sum(-1)).sum(-1), rtol=1e-14)
+
+
+def test_ias15_refuses_fp32_forces():
+    """IAS15's corrector and step control need FP64 forces: the FP32 methods
+    are refused before any device use (no spin with dt -> 1e-12 tf)."""
+    import numpy as np
+
+    lib = L.load()
+    x = np.zeros((2, 3))
+    x[1, 0] = 1.0
+    v = np.zeros((2, 3))
+    m = np.ones(2)
+    s = _abi.make_system(x, v, m, 1.0)
+    ip = lib.get_new_integrator_param()
+    ip.integrator = 9
+    ip.tolerance = 1e-9
+    ap = L.param(16)
+    op = lib.get_new_output_param()
+    st_ = _abi.SimulationStatus()
+    se = lib.get_new_settings()
+    se.verbose = 0
+    st = lib.ias15(C.byref(s), C.byref(ip), C.byref(ap), C.byref(op), C.byref(st_), C.byref(se), 1.0)
+    assert st.return_code == _abi.GRAV_VALUE_ERROR
+    assert "FP64" in L.tb_text(st)
+    lib.free_traceback(C.byref(st))
