@@ -198,6 +198,28 @@ def run_bh(L, ic, learned=True):
     return out
 
 
+def run_small_leapfrog(L, ic):
+    """FP64 direct sum + Leapfrog at N=1024 (Plummer seed 1, eps=0, dt=1e-3,
+    1000 steps): launch-bound, so the captured-graph replay matters."""
+    x, v, m = ic.plummer(1024, 1)
+    p = L.param(1)
+    out = {"workload": "FP64 pairwise + Leapfrog, Plummer N=1024 (seed 1), eps=0, dt=1e-3, 1000 steps"}
+    for mode in ("graph", "eager"):
+        sim = L.Sim(x, v, m, 1.0)
+        sim.leapfrog_begin(p, 1e-3)
+        L.timing_enable(mode == "eager")  # timing on -> every kernel launched with its own events
+        L.synchronize()
+        t0 = time.perf_counter()
+        sim.leapfrog_steps(p, 1e-3, 1000)
+        L.synchronize()
+        wall = time.perf_counter() - t0
+        L.timing_enable(False)
+        sim.close()
+        out[f"{mode}_ms_per_step"] = wall
+    out["interactions_per_s_graph"] = 1024 * 1023 / (out["graph_ms_per_step"] / 1000.0)
+    return out
+
+
 def run_whfast(L, ic, steps):
     off, X, V, M = ic.ensemble(4096, 3)
     P = int(off[-1])
@@ -300,6 +322,9 @@ def main():
     }
     if not args.no_bh:
         out["bh"] = run_bh(L, ic, learned=not args.no_learned)
+    out["leapfrog_n1024"] = run_small_leapfrog(L, ic)
+    log(f"leapfrog N=1024: graph {out['leapfrog_n1024']['graph_ms_per_step']:.4f} ms/step, "
+        f"eager {out['leapfrog_n1024']['eager_ms_per_step']:.4f} ms/step")
     if not args.no_whfast:
         out["whfast"] = run_whfast(L, ic, args.whfast_steps)
         log(f"whfast: {out['whfast']['particle_steps_per_s']:.3e} particle-steps/s")

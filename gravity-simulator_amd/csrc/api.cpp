@@ -103,7 +103,81 @@ struct GsSim {
     double G = 1.0;
     double *x = nullptr, *v = nullptr, *m = nullptr, *a = nullptr;
     double *xe = nullptr, *ve = nullptr;
+    // captured Leapfrog step (drift, forces, kick) and what it was captured for
+    hipGraphExec_t lf_exec = nullptr;
+    struct Key {
+        int method = -1;
+        double eps = 0.0, dt = 0.0;
+        Fp32Config cfg;
+        void *partial = nullptr;
+        bool operator==(const Key &o) const
+        {
+            return method == o.method && eps == o.eps && dt == o.dt && cfg.kind == o.cfg.kind &&
+                   cfg.block == o.cfg.block && cfg.ipt == o.cfg.ipt && cfg.jsplit == o.cfg.jsplit &&
+                   partial == o.partial;
+        }
+    } lf_key;
 };
+
+namespace gs {
+
+// Methods whose force evaluation is a fixed sequence of kernel launches on
+// the stream (no host synchronisation, allocation or memset), so a Leapfrog
+// step can be captured into a HIP graph and replayed.
+static bool graph_method(int method)
+{
+    return method == ACCELERATION_METHOD_PAIRWISE || method == ACCELERATION_METHOD_MASSLESS ||
+           method == ACCELERATION_METHOD_PAIRWISE_FP32 ||
+           method == ACCELERATION_METHOD_MASSLESS_FP32;
+}
+
+static ErrorStatus lf_step(Context *c, GsSim *s, const AccelerationParam *p, double dt)
+{
+    GS_HIP(launch_lf_drift(c, s->x, s->xe, s->v, s->n, dt));
+    GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
+    GS_HIP(launch_lf_kick(c, s->v, s->ve, s->a, s->n, dt));
+    return make_success_error_status();
+}
+
+// Capture one step into s->lf_exec unless the cached graph was captured for
+// the same parameters and buffers.  Called after an eager step, so every
+// buffer the step needs already exists and nothing allocates while capturing.
+static ErrorStatus lf_graph(Context *c, GsSim *s, const AccelerationParam *p, double dt)
+{
+    GsSim::Key key;
+    key.method = p->method;
+    key.eps = p->softening_length;
+    key.dt = dt;
+    key.cfg = fp32_config();
+    key.partial = c->slots[SLOT_PARTIAL].ptr;
+    if (s->lf_exec && key == s->lf_key) return make_success_error_status();
+    if (s->lf_exec) {
+        GS_HIP(hipGraphExecDestroy(s->lf_exec));
+        s->lf_exec = nullptr;
+    }
+    GS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    ErrorStatus st = lf_step(c, s, p, dt);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (st.return_code != GRAV_SUCCESS) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    GS_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&s->lf_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GS_HIP(ei);
+    if (key.partial != c->slots[SLOT_PARTIAL].ptr) {
+        // a buffer moved while capturing: the graph would point at freed memory
+        (void)hipGraphExecDestroy(s->lf_exec);
+        s->lf_exec = nullptr;
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Leapfrog graph capture saw a buffer reallocation");
+    }
+    s->lf_key = key;
+    return make_success_error_status();
+}
+
+}  // namespace gs
 
 extern "C" {
 
@@ -300,6 +374,7 @@ ErrorStatus gs_sim_destroy(GsSim *s)
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         (void)hipStreamSynchronize(c->stream);
     }
+    if (s->lf_exec) (void)hipGraphExecDestroy(s->lf_exec);
     (void)hipFree(s->x);
     (void)hipFree(s->v);
     (void)hipFree(s->m);
@@ -371,11 +446,17 @@ ErrorStatus gs_sim_leapfrog_steps(GsSim *s, const AccelerationParam *p, double d
     Context *c;
     GS_TRY(context(&c));
     std::lock_guard<std::recursive_mutex> lk(c->mu);
-    for (int64 i = 0; i < k; i++) {
-        GS_HIP(launch_lf_drift(c, s->x, s->xe, s->v, s->n, dt));
-        GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
-        GS_HIP(launch_lf_kick(c, s->v, s->ve, s->a, s->n, dt));
+    // with per-kernel timing on, every step runs eagerly so that its kernels
+    // carry their own events
+    const bool graph = k >= 3 && graph_method(p->method) && !c->timing;
+    int64 i = 0;
+    if (graph) {
+        GS_TRY(lf_step(c, s, p, dt));  // allocates what the step needs
+        i = 1;
+        GS_TRY(lf_graph(c, s, p, dt));
+        for (; i < k; i++) GS_HIP(hipGraphLaunch(s->lf_exec, c->stream));
     }
+    for (; i < k; i++) GS_TRY(lf_step(c, s, p, dt));
     return make_success_error_status();
 }
 

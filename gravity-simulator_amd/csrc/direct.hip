@@ -24,6 +24,16 @@ namespace gs {
 constexpr int F64_BLOCK = 256;
 
 // ------------------------------------------------------------------ FP64
+// Bitwise reference sums: every target adds its terms in source-index order,
+// term = fl(fl(G / fl(fl(r*r)*r) * fl(x_l - x_k)) * m_l), l != k
+// (src/acceleration.c:206-229 as restated per target).  A workgroup takes
+// F64_TB targets; per tile of 256 sources every thread computes its source's
+// terms for all F64_TB targets into LDS (the expensive sqrt and division run
+// on all lanes), then one thread per (target, component) adds the tile's
+// terms in index order.  The additions are the reference's, in the
+// reference's order, so the result is bit for bit the serial one.
+constexpr int F64_TB = 8;
+
 template <bool SKIP_MASSLESS>
 __global__ __launch_bounds__(F64_BLOCK) void k_direct_f64(const double *__restrict__ x,
                                                           const double *__restrict__ m, int n,
@@ -32,55 +42,75 @@ __global__ __launch_bounds__(F64_BLOCK) void k_direct_f64(const double *__restri
                                                           int num_targets,
                                                           double *__restrict__ a)
 {
-    __shared__ double sx[F64_BLOCK], sy[F64_BLOCK], sz[F64_BLOCK], sm[F64_BLOCK];
-    const int t = blockIdx.x * F64_BLOCK + threadIdx.x;
-    const bool live = t < num_targets;
-    const int k = live ? (targets ? targets[t] : t) : -1;
-    double xk0 = 0.0, xk1 = 0.0, xk2 = 0.0;
-    if (live) {
-        xk0 = x[3 * k + 0];
-        xk1 = x[3 * k + 1];
-        xk2 = x[3 * k + 2];
+    __shared__ double term[F64_TB][3][F64_BLOCK];
+    __shared__ unsigned long long skip[F64_TB][F64_BLOCK / 64];  // one bit per source
+    __shared__ double tx[F64_TB][3];
+    __shared__ int tk[F64_TB];
+    const int tid = threadIdx.x;
+    const int t0 = blockIdx.x * F64_TB;
+    if (tid < F64_TB) {
+        const int t = t0 + tid;
+        const int k = t < num_targets ? (targets ? targets[t] : t) : -1;
+        tk[tid] = k;
+        tx[tid][0] = k >= 0 ? x[3 * k + 0] : 0.0;
+        tx[tid][1] = k >= 0 ? x[3 * k + 1] : 0.0;
+        tx[tid][2] = k >= 0 ? x[3 * k + 2] : 0.0;
     }
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    __syncthreads();
+    // accumulator of (target tid / 3, component tid % 3) for tid < 3 F64_TB
+    double acc = 0.0;
+    const int at = tid / 3, ac = tid % 3;
     for (int base = 0; base < n; base += F64_BLOCK) {
-        const int j = base + threadIdx.x;
-        if (j < n) {
-            sx[threadIdx.x] = x[3 * j + 0];
-            sy[threadIdx.x] = x[3 * j + 1];
-            sz[threadIdx.x] = x[3 * j + 2];
-            sm[threadIdx.x] = m[j];
+        const int j = base + tid;
+        const bool src = j < n;
+        double xj0 = 0.0, xj1 = 0.0, xj2 = 0.0, mj = 0.0;
+        if (src) {
+            xj0 = x[3 * j + 0];
+            xj1 = x[3 * j + 1];
+            xj2 = x[3 * j + 2];
+            mj = m[j];
         }
-        __syncthreads();
-        const int cnt = min(F64_BLOCK, n - base);
-        if (live) {
-            for (int jj = 0; jj < cnt; jj++) {
-                const double ml = sm[jj];
-                if (base + jj == k) continue;
-                if (SKIP_MASSLESS && ml == 0.0) continue;
-                const double R0 = sx[jj] - xk0;
-                const double R1 = sy[jj] - xk1;
-                const double R2 = sz[jj] - xk2;
+#pragma unroll 2
+        for (int tt = 0; tt < F64_TB; tt++) {
+            const int k = tk[tt];
+            const bool sk = !src || k < 0 || j == k || (SKIP_MASSLESS && mj == 0.0);
+            const unsigned long long bits = __ballot(sk);
+            if ((tid & 63) == 0) skip[tt][tid >> 6] = bits;
+            if (!sk) {
+                const double R0 = xj0 - tx[tt][0];
+                const double R1 = xj1 - tx[tt][1];
+                const double R2 = xj2 - tx[tt][2];
                 const double r = __builtin_sqrt(R0 * R0 + R1 * R1 + R2 * R2 + eps2);
                 const double tv = G / (r * r * r);
-                a0 += (tv * R0) * ml;
-                a1 += (tv * R1) * ml;
-                a2 += (tv * R2) * ml;
+                term[tt][0][tid] = (tv * R0) * mj;
+                term[tt][1][tid] = (tv * R1) * mj;
+                term[tt][2][tid] = (tv * R2) * mj;
+            }
+        }
+        __syncthreads();
+        if (tid < 3 * F64_TB) {
+            // in index order; a skipped source (self, massless, past n) keeps
+            // acc unchanged through a select, so the loads and adds pipeline
+            const double *tp = term[at][ac];
+            for (int w = 0; w < F64_BLOCK / 64; w++) {
+                const unsigned long long bits = skip[at][w];
+#pragma unroll 16
+                for (int jj = 0; jj < 64; jj++) {
+                    const double t = tp[64 * w + jj];
+                    const double s = acc + t;
+                    acc = ((bits >> jj) & 1ull) ? acc : s;
+                }
             }
         }
         __syncthreads();
     }
-    if (live) {
-        a[3 * t + 0] = a0;
-        a[3 * t + 1] = a1;
-        a[3 * t + 2] = a2;
-    }
+    if (tid < 3 * F64_TB && t0 + at < num_targets) a[3 * (t0 + at) + ac] = acc;
 }
 
 hipError_t launch_direct_f64(Context *c, const double *x, const double *m, int n, double G,
                              double eps, bool skip_massless, double *a)
 {
-    const int grid = (n + F64_BLOCK - 1) / F64_BLOCK;
+    const int grid = (n + F64_TB - 1) / F64_TB;
     const int rec = timing_begin(c, "direct_f64");
     if (skip_massless)
         hipLaunchKernelGGL(k_direct_f64<true>, dim3(grid), dim3(F64_BLOCK), 0, c->stream, x, m,
@@ -98,7 +128,7 @@ hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *
                                      double G, double eps, bool skip_massless,
                                      const int *targets, int num_targets, double *out)
 {
-    const int grid = (num_targets + F64_BLOCK - 1) / F64_BLOCK;
+    const int grid = (num_targets + F64_TB - 1) / F64_TB;
     if (skip_massless)
         hipLaunchKernelGGL(k_direct_f64<true>, dim3(grid), dim3(F64_BLOCK), 0, c->stream, x, m,
                            n, G, eps * eps, targets, num_targets, out);

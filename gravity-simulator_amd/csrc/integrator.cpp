@@ -3,6 +3,7 @@
 // device-resident with the reference's operation order
 // (src/integrator.c:894-1121); WHFast runs the GPU WHFast
 // (src/integrator_whfast.c:200-407); IAS15 the GPU IAS15 (ias15.hip).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -30,13 +31,25 @@ static ErrorStatus leapfrog_gpu(System *system, IntegratorParam *ip, Acceleratio
     status->t = 0.0;
     status->dt = dt;
     status->num_steps = 0;
-    while (st.return_code == GRAV_SUCCESS && status->num_steps < total) {
+    // steps 0 .. total-2 end at (i+1) dt <= (total-1) dt < tf, so only the last
+    // one can be clamped (src/integrator.c:1001-1006): the others run in
+    // chunks (a replayed HIP graph for the direct-sum methods), checking the
+    // exit flag between chunks
+    const int64 chunk = 1024;
+    while (st.return_code == GRAV_SUCCESS && status->num_steps < total - 1) {
+        const int64 k = std::min(chunk, total - 1 - status->num_steps);
+        st = gs_sim_leapfrog_steps(sim, ap, dt, k);
+        status->num_steps += k;
+        status->t = status->num_steps * dt;
+        if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
+    }
+    const bool exited = settings->is_exit_ptr && *settings->is_exit_ptr;
+    if (st.return_code == GRAV_SUCCESS && !exited && status->num_steps < total) {
         if (status->t + dt > tf) dt = tf - status->t;
         status->dt = dt;
         st = gs_sim_leapfrog_steps(sim, ap, dt, 1);
         status->num_steps++;
         status->t = status->num_steps * dt;
-        if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
     }
     if (st.return_code == GRAV_SUCCESS) st = gs_sim_leapfrog_end(sim, dt);
     if (st.return_code == GRAV_SUCCESS) st = gs_sim_download(sim, system->x, system->v);
