@@ -12,8 +12,9 @@
 // index) in LDS -> permute jx, jv, m, ids -> eta (cumulative mass over the
 // massive particles; massless ones add exactly 0) -> universal-variable
 // Kepler drift per particle -> removal of unconverged particles (ascending
-// order) -> Jacobi->Cartesian positions (a sequential recurrence, one lane
-// per component) -> massless-form accelerations and kick.  Cartesian
+// order) -> Jacobi->Cartesian positions (the recurrence walked over the
+// massive particles and the heads of the massless runs, then filled in
+// parallel) -> massless-form accelerations and kick.  Cartesian
 // velocities are only needed at the end and are reconstructed there.
 #include <cmath>
 #include <vector>
@@ -179,6 +180,20 @@ struct WfShared {
     int primary;
 };
 
+// LDS reused between the phases of a step: removal flags while drifting,
+// then the Jacobi -> Cartesian quotients.
+constexpr int J2C_K = 3;  // explicit quotients per massless run before its fixed point
+union WfScratch {
+    int flags[WF_MAX_N];
+    struct {
+        double exp[WF_MAX_MASSIVE][J2C_K][3];  // run below massive ordinal g: first quotients
+        double mq[WF_MAX_MASSIVE][3];          // quotient at massive ordinal g
+        int nexp[WF_MAX_MASSIVE][3];           // quotients stored for the run of gap g
+        int fallback[3];                       // component d needs the full serial chain
+    } j;
+};
+static_assert(sizeof(WfScratch) == sizeof(int) * WF_MAX_N, "the quotients fit in the flags");
+
 // Sort (distance, index) ascending, lexicographically, for n <= WF_MAX_N.
 __device__ void bitonic_sort(WfShared &S, int n)
 {
@@ -211,8 +226,52 @@ __device__ void bitonic_sort(WfShared &S, int n)
     }
 }
 
+// Odd-even transposition passes tried before the bitonic sort.
+constexpr int WF_OET_PASSES = 24;
+
+// Sort the (distance, index) pairs in S.key / S.idx (idx = 0..n-1 on entry)
+// ascending, lexicographically.  Each step arrives in the previous step's
+// order, so the distances are nearly sorted: odd-even transposition passes
+// finish in a few passes; if they do not within WF_OET_PASSES, the bitonic
+// sort takes the partly sorted pairs.  The pairs are distinct, so both give
+// the same order.  NaN distances go straight to the bitonic sort.
+__device__ void sort_pairs(WfShared &S, int n)
+{
+    int unsorted = 0, nan = 0;
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+        const double k = S.key[i];
+        if (k != k) nan = 1;
+        if (i + 1 < n && k > S.key[i + 1]) unsorted = 1;
+    }
+    if (__syncthreads_or(nan)) {
+        bitonic_sort(S, n);
+        return;
+    }
+    if (!__syncthreads_or(unsorted)) return;
+    bool prev = true;
+    for (int pass = 0; pass < WF_OET_PASSES; pass++) {
+        int sw = 0;
+        for (int i = (pass & 1) + 2 * static_cast<int>(threadIdx.x); i + 1 < n; i += 2 * WF_BLOCK) {
+            const double ki = S.key[i], kl = S.key[i + 1];
+            const short ii = S.idx[i], il = S.idx[i + 1];
+            if (ki > kl || (ki == kl && ii > il)) {
+                S.key[i] = kl;
+                S.key[i + 1] = ki;
+                S.idx[i] = il;
+                S.idx[i + 1] = ii;
+                sw = 1;
+            }
+        }
+        const bool any = __syncthreads_or(sw);
+        if (!any && !prev) return;  // both parities left every pair in order
+        prev = any;
+    }
+    bitonic_sort(S, n);
+}
+
 // distances from the particle whose id is 0 (src/system.c:1232-1277), then
-// sort and permute (pos, vel, m, ids) into the alternate buffers and back.
+// sort and permute (pos, vel, m, ids) through the alternate buffers; only
+// the particles that changed place move.
 __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, double *m,
                                  int *ids, double *pos2, double *vel2, double *m2, int *ids2)
 {
@@ -237,9 +296,10 @@ __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, d
         S.idx[i] = static_cast<short>(i);
     }
     __syncthreads();
-    bitonic_sort(S, n);
+    sort_pairs(S, n);
     for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
         const int s = S.idx[i];
+        if (s == i) continue;
         pos2[3 * i] = pos[3 * s];
         pos2[3 * i + 1] = pos[3 * s + 1];
         pos2[3 * i + 2] = pos[3 * s + 2];
@@ -251,6 +311,7 @@ __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, d
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+        if (S.idx[i] == i) continue;
         pos[3 * i] = pos2[3 * i];
         pos[3 * i + 1] = pos2[3 * i + 1];
         pos[3 * i + 2] = pos2[3 * i + 2];
@@ -349,6 +410,101 @@ __device__ void j2c_component(int n, int d, const double *jq, double *q, const d
         c = c2;
     }
     q[d] = c / m[0];
+}
+
+// Jacobi -> Cartesian for the whole workgroup.  The recurrence above only
+// needs jq at the massive particles: inside a run of massless particles
+// (finite jq, c != 0) the quotient is c / eta and c becomes
+// eta (c / eta), which reaches a fixed point within a few particles.  Lane d
+// therefore walks the massive particles and the head of each run (at most
+// J2C_K quotients per run) and stores the quotients; then every thread forms
+// q = jq + quotient for its particles in parallel.  Components where a run
+// has a non-finite jq, meets c == 0 or has no fixed point within J2C_K take
+// the serial recurrence; both give the same bits.  Called by all threads.
+__device__ void j2c_block(int n, const double *jq, double *q, const double *m, const WfShared &S,
+                          WfScratch &X)
+{
+    const int tid = threadIdx.x;
+    const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
+    if (tid < 3) X.j.fallback[tid] = 0;
+    __syncthreads();
+    // a non-finite jq in a run changes the run's quotients
+    int bad = 0;
+    for (int i = 1 + tid; i < n; i += WF_BLOCK) {
+        if (m[i] != 0.0) continue;  // (runs hold massless particles only)
+        for (int d = 0; d < 3; d++)
+            if (!isfinite(jq[3 * i + d])) bad |= 1 << d;
+    }
+    if (bad)
+        for (int d = 0; d < 3; d++)
+            if (bad & (1 << d)) X.j.fallback[d] = 1;
+    __syncthreads();
+    if (tid < 3) {
+        const int d = tid;
+        const double *eta = S.key;
+        bool fail = X.j.fallback[d] != 0;
+        double c = eta[n - 1] * jq[d];
+        for (int g = nm - 1; g >= 0 && !fail; g--) {
+            const int lo = S.mlist[g];
+            const int hi = (g + 1 < nm ? S.mlist[g + 1] : n) - 1;
+            int k = 0;
+            for (int i = hi; i > lo; i--) {
+                if (k == J2C_K || c == 0.0) {
+                    fail = true;
+                    break;
+                }
+                // (c - 0 * jq) / eta with jq finite and c != 0 is c / eta
+                const double cn = c / eta[i];
+                X.j.exp[g][k][d] = cn;
+                k++;
+                const double c2 = eta[i - 1] * cn;
+                if (c2 == c) break;  // the rest of the run repeats this quotient
+                c = c2;
+            }
+            X.j.nexp[g][d] = k;
+            if (!fail && g >= 1) {
+                const double mi = m[lo], jqi = jq[3 * lo + d];
+                const double cn = (c - mi * jqi) / eta[lo];
+                X.j.mq[g][d] = cn;
+                c = eta[lo - 1] * cn;
+            }
+        }
+        if (fail) {
+            X.j.fallback[d] = 1;
+            j2c_component(n, d, jq, q, m, S);
+        } else {
+            q[d] = c / m[0];
+        }
+    }
+    __syncthreads();
+    const bool f0 = X.j.fallback[0], f1 = X.j.fallback[1], f2 = X.j.fallback[2];
+    for (int i = 1 + tid; i < n; i += WF_BLOCK) {
+        // gap: the largest massive ordinal at or below i
+        int lo = 0, hi = nm - 1, g = 0;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            if (S.mlist[mid] <= i) {
+                g = mid;
+                lo = mid + 1;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const bool event = S.mlist[g] == i;
+        const int top = (g + 1 < nm ? S.mlist[g + 1] : n) - 1;
+        for (int d = 0; d < 3; d++) {
+            if (d == 0 ? f0 : (d == 1 ? f1 : f2)) continue;
+            double cn;
+            if (event) {
+                cn = X.j.mq[g][d];
+            } else {
+                const int r = top - i, k = X.j.nexp[g][d];
+                cn = X.j.exp[g][r < k ? r : k - 1][d];
+            }
+            q[3 * i + d] = jq[3 * i + d] + cn;
+        }
+    }
+    __syncthreads();
 }
 
 // Cartesian -> Jacobi (src/integrator_whfast.c:682-724), one lane per
@@ -565,7 +721,8 @@ __global__ __launch_bounds__(WF_BLOCK) void k_whfast_init(WfArgs A)
 __global__ __launch_bounds__(WF_BLOCK) void k_whfast_steps(WfArgs A)
 {
     __shared__ WfShared S;
-    __shared__ int flags[WF_MAX_N];
+    __shared__ WfScratch X;
+    int *flags = X.flags;
     const WfSys sy = A.sys[blockIdx.x];
     const long long o = sy.off;
     int n = A.count[blockIdx.x];
@@ -598,14 +755,13 @@ __global__ __launch_bounds__(WF_BLOCK) void k_whfast_steps(WfArgs A)
             massive_list(S, n, m, A.method);
             fill_eta(S, n);
         }
-        if (threadIdx.x < 3) j2c_component(n, threadIdx.x, jx, x, m, S);
+        __syncthreads();  // the removal flags share their LDS with the quotients
+        j2c_block(n, jx, x, m, S, X);
         // Cartesian velocities only matter for the state returned to the
         // caller: rebuild them (from the drifted, not yet kicked, Jacobi
         // velocities, as the reference's last jacobi_to_cartesian does) on
-        // the last step of each launch, in a second wavefront
-        if (s == A.step0 + A.nsteps - 1 && threadIdx.x >= 64 && threadIdx.x < 67)
-            j2c_component(n, threadIdx.x - 64, jv, A.v + 3 * o, m, S);
-        __syncthreads();
+        // the last step of each launch
+        if (s == A.step0 + A.nsteps - 1) j2c_block(n, jv, A.v + 3 * o, m, S, X);
         accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, dt, A.method);
     }
     if (threadIdx.x == 0) A.count[blockIdx.x] = n;

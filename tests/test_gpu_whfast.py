@@ -44,6 +44,25 @@ def test_whfast_matches_oracle_bitwise():
         assert np.array_equal(ids, io) and np.array_equal(x, xo) and np.array_equal(v, vo), s
 
 
+def test_whfast_planar_systems_match_oracle_bitwise():
+    """Planar systems: every z is 0, so the z component of the Jacobi ->
+    Cartesian recurrence carries c == 0 and takes the serial path, while x and
+    y take the run-quotient path; both must give the oracle's bits.  Planets
+    at a ~ logU[0.4, 30] land between the test particles (a in [2, 3.5]) after
+    the distance sort, so the massless runs have many lengths."""
+    off, X, V, M = ic.ensemble(6, 31, max_test=400)
+    X, V = X.copy(), V.copy()
+    X[:, 2] = 0.0
+    V[:, 2] = 0.0
+    res = L.whfast_ensemble(off, X, V, M, 1.0, 0.01, 3.0, method=2)
+    for s in range(6):
+        sl = slice(off[s], off[s + 1])
+        io, xo, vo, mo = O.whfast(X[sl], V[sl], M[sl], 1.0, 0.01, 3.0, method=2)
+        ids, x, v, m = res[s]
+        assert np.array_equal(ids, io) and np.array_equal(x, xo) and np.array_equal(v, vo), s
+        assert np.all(x[:, 2] == 0.0)
+
+
 def test_whfast_pairwise_form_and_clamped_last_step():
     off, X, V, M = ic.ensemble(2, 23, num_planets=5, max_test=20)
     res = L.whfast_ensemble(off, X, V, M, 1.0, 0.07, 3.0, method=1, eps=0.01)
