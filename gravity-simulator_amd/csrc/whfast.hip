@@ -16,7 +16,9 @@
 // massive particles and the heads of the massless runs, then filled in
 // parallel) -> massless-form accelerations and kick.  Cartesian
 // velocities are only needed at the end and are reconstructed there.
+#include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <vector>
 
 #include "gs_internal.h"
@@ -52,6 +54,22 @@ __device__ __forceinline__ double norm3(double a, double b, double c)
     return __builtin_sqrt(a * a + b * b + c * c);
 }
 
+// z / b for the constant divisors of the Stumpff series, correctly rounded
+// with one multiply and two FMAs instead of a full division (Markstein's
+// theorem: with y = RN(1/b) and q = RN(z y) within an ulp of z / b, the
+// remainder r = z - b q is exact and RN(q + r y) = RN(z / b)).  Quotients
+// near the subnormal range, where the remainder can be inexact, take the
+// division itself.
+__device__ __forceinline__ double div_const(double z, double b, double y)
+{
+    if (!(fabs(z) >= 0x1p-900)) return z / b;
+    const double q = z * y;
+    const double r = __builtin_fma(-q, b, z);
+    return __builtin_fma(r, y, q);
+}
+
+#define DIVC(z, b) div_const((z), (b), 1.0 / (b))
+
 // src/integrator_whfast.c:774-815
 __device__ void stumpff(double z, double &c0, double &c1, double &c2, double &c3)
 {
@@ -60,18 +78,21 @@ __device__ void stumpff(double z, double &c0, double &c1, double &c2, double &c3
         z /= 4.0;
         k++;
     }
-    double t3 = (1.0 - z / 20.0 *
-                           (1.0 - z / 42.0 *
-                                      (1.0 - z / 72.0 *
-                                                 (1.0 - z / 110.0 *
-                                                            (1.0 - z / 156.0 * (1.0 - z / 210.0)))))) /
-                6.0;
-    double t2 = (1.0 - z / 12.0 *
-                           (1.0 - z / 30.0 *
-                                      (1.0 - z / 56.0 *
-                                                 (1.0 - z / 90.0 *
-                                                            (1.0 - z / 132.0 * (1.0 - z / 182.0)))))) /
-                2.0;
+    // the reference's nested series, innermost factor first
+    double t3 = 1.0 - DIVC(z, 210.0);
+    t3 = 1.0 - DIVC(z, 156.0) * t3;
+    t3 = 1.0 - DIVC(z, 110.0) * t3;
+    t3 = 1.0 - DIVC(z, 72.0) * t3;
+    t3 = 1.0 - DIVC(z, 42.0) * t3;
+    t3 = 1.0 - DIVC(z, 20.0) * t3;
+    t3 = DIVC(t3, 6.0);
+    double t2 = 1.0 - DIVC(z, 182.0);
+    t2 = 1.0 - DIVC(z, 132.0) * t2;
+    t2 = 1.0 - DIVC(z, 90.0) * t2;
+    t2 = 1.0 - DIVC(z, 56.0) * t2;
+    t2 = 1.0 - DIVC(z, 30.0) * t2;
+    t2 = 1.0 - DIVC(z, 12.0) * t2;
+    t2 = t2 / 2.0;
     double t1 = 1.0 - z * t3;
     double t0 = 1.0 - z * t2;
     while (k > 0) {
@@ -174,11 +195,14 @@ struct WfShared {
     // and massive particle p (pairs with one member inside, one outside)
     double strad_l[WF_MAX_MASSIVE + 1][3];
     double strad_m[WF_MAX_MASSIVE][3];
+    unsigned mbits[WF_MAX_N / 32];  // massive flags, one bit per particle
     int nm;
     int n;
     int any_remove;
+    int bad;  // bit d: some jq component d is not finite
     int primary;
 };
+static_assert(WF_MAX_N == 64 * 32, "one 32-bit mask word per lane of one wavefront");
 
 // LDS reused between the phases of a step: removal flags while drifting,
 // then the Jacobi -> Cartesian quotients.
@@ -188,7 +212,9 @@ union WfScratch {
     struct {
         double exp[WF_MAX_MASSIVE][J2C_K][3];  // run below massive ordinal g: first quotients
         double mq[WF_MAX_MASSIVE][3];          // quotient at massive ordinal g
-        int nexp[WF_MAX_MASSIVE][3];           // quotients stored for the run of gap g
+        double mjq[WF_MAX_MASSIVE][3];         // jq at massive ordinal g
+        double q0[3];                          // Cartesian coordinate of particle 0
+        unsigned char nexp[WF_MAX_MASSIVE][3];  // quotients stored for the run of gap g
         int fallback[3];                       // component d needs the full serial chain
     } j;
 };
@@ -329,20 +355,33 @@ __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, d
 // (src/integrator_whfast.c:1266-1279; massless particles add exactly 0).
 __device__ void massive_list(WfShared &S, int n, const double *m, int method)
 {
-    // ordinals of massive particles, ascending: block scan over chunks
-    if (threadIdx.x == 0) S.nm = 0;
+    // ordinals of massive particles, ascending: bit mask, then one wavefront
+    // turns the mask words into the list (lane w owns particles 32w..32w+31)
+    for (int w = threadIdx.x; w < WF_MAX_N / 32; w += WF_BLOCK) S.mbits[w] = 0u;
     __syncthreads();
-    for (int base = 0; base < n; base += WF_BLOCK) {
-        const int i = base + threadIdx.x;
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK)
         // particle 0 (the primary) always anchors the list
-        const int f = (i < n && (i == 0 || method == 1 || m[i] != 0.0)) ? 1 : 0;
-        int tot;
-        const int pos = block_scan_excl(f, &tot, S.scan_buf);
-        if (f && S.nm + pos < WF_MAX_MASSIVE) S.mlist[S.nm + pos] = i;
-        __syncthreads();
-        if (threadIdx.x == 0) S.nm += tot;
-        __syncthreads();
+        if (i == 0 || method == 1 || m[i] != 0.0) atomicOr(&S.mbits[i >> 5], 1u << (i & 31));
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        unsigned b = S.mbits[lane];
+        const int cnt = __popc(b);
+        int incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        int pos = incl - cnt;
+        while (b) {
+            const int bit = __ffs(b) - 1;
+            b &= b - 1u;
+            if (pos < WF_MAX_MASSIVE) S.mlist[pos] = 32 * lane + bit;
+            pos++;
+        }
+        if (lane == 63) S.nm = incl;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
         const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
         double e = 0.0;
@@ -412,38 +451,59 @@ __device__ void j2c_component(int n, int d, const double *jq, double *q, const d
     q[d] = c / m[0];
 }
 
-// Jacobi -> Cartesian for the whole workgroup.  The recurrence above only
-// needs jq at the massive particles: inside a run of massless particles
-// (finite jq, c != 0) the quotient is c / eta and c becomes
+// Largest massive ordinal g with mlist[g] <= i (i >= 0).
+__device__ __forceinline__ int massive_floor(const WfShared &S, int nm, int i)
+{
+    int lo = 0, hi = nm - 1, g = 0;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (S.mlist[mid] <= i) {
+            g = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return g;
+}
+
+// Quotient of the recurrence for the massless particle i in the run above
+// massive ordinal g (its first J2C_K quotients, then the fixed point).
+__device__ __forceinline__ double run_quotient(const WfShared &S, const WfScratch &X, int nm, int n,
+                                               int g, int i, int d)
+{
+    const int top = (g + 1 < nm ? S.mlist[g + 1] : n) - 1;
+    const int r = top - i, k = X.j.nexp[g][d];
+    return X.j.exp[g][r < k ? r : k - 1][d];
+}
+
+// Jacobi -> Cartesian for the whole workgroup, first half.  The recurrence
+// above only needs jq at the massive particles: inside a run of massless
+// particles (finite jq, c != 0) the quotient is c / eta and c becomes
 // eta (c / eta), which reaches a fixed point within a few particles.  Lane d
 // therefore walks the massive particles and the head of each run (at most
-// J2C_K quotients per run) and stores the quotients; then every thread forms
-// q = jq + quotient for its particles in parallel.  Components where a run
-// has a non-finite jq, meets c == 0 or has no fixed point within J2C_K take
-// the serial recurrence; both give the same bits.  Called by all threads.
-__device__ void j2c_block(int n, const double *jq, double *q, const double *m, const WfShared &S,
-                          WfScratch &X)
+// J2C_K quotients per run) and stores the quotients, with the coordinate of
+// particle 0; then q = jq + quotient for every other particle, in parallel
+// (j2c_fill, or fused into the kick).  Components with bit d of `bad` set (a
+// non-finite jq), or whose walk meets c == 0 or no fixed point within J2C_K,
+// are marked in X.j.fallback for the serial recurrence; both give the same
+// bits.  Called by all threads.
+__device__ void j2c_quotients(int n, const double *jq, const WfShared &S, WfScratch &X, int bad)
 {
     const int tid = threadIdx.x;
     const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
-    if (tid < 3) X.j.fallback[tid] = 0;
-    __syncthreads();
-    // a non-finite jq in a run changes the run's quotients
-    int bad = 0;
-    for (int i = 1 + tid; i < n; i += WF_BLOCK) {
-        if (m[i] != 0.0) continue;  // (runs hold massless particles only)
-        for (int d = 0; d < 3; d++)
-            if (!isfinite(jq[3 * i + d])) bad |= 1 << d;
+    for (int q = tid; q < nm; q += WF_BLOCK) {
+        const int j = S.mlist[q];
+        X.j.mjq[q][0] = jq[3 * j];
+        X.j.mjq[q][1] = jq[3 * j + 1];
+        X.j.mjq[q][2] = jq[3 * j + 2];
     }
-    if (bad)
-        for (int d = 0; d < 3; d++)
-            if (bad & (1 << d)) X.j.fallback[d] = 1;
     __syncthreads();
     if (tid < 3) {
         const int d = tid;
         const double *eta = S.key;
-        bool fail = X.j.fallback[d] != 0;
-        double c = eta[n - 1] * jq[d];
+        bool fail = ((bad >> d) & 1) != 0;
+        double c = eta[n - 1] * X.j.mjq[0][d];  // (ordinal 0 is particle 0)
         for (int g = nm - 1; g >= 0 && !fail; g--) {
             const int lo = S.mlist[g];
             const int hi = (g + 1 < nm ? S.mlist[g + 1] : n) - 1;
@@ -461,50 +521,61 @@ __device__ void j2c_block(int n, const double *jq, double *q, const double *m, c
                 if (c2 == c) break;  // the rest of the run repeats this quotient
                 c = c2;
             }
-            X.j.nexp[g][d] = k;
+            X.j.nexp[g][d] = static_cast<unsigned char>(k);
             if (!fail && g >= 1) {
-                const double mi = m[lo], jqi = jq[3 * lo + d];
-                const double cn = (c - mi * jqi) / eta[lo];
+                const double cn = (c - S.mm[g] * X.j.mjq[g][d]) / eta[lo];
                 X.j.mq[g][d] = cn;
                 c = eta[lo - 1] * cn;
             }
         }
-        if (fail) {
-            X.j.fallback[d] = 1;
-            j2c_component(n, d, jq, q, m, S);
-        } else {
-            q[d] = c / m[0];
-        }
+        X.j.fallback[d] = fail ? 1 : 0;
+        if (!fail) X.j.q0[d] = c / S.mm[0];
     }
     __syncthreads();
+}
+
+// Second half: q for every particle into global memory; the serial
+// recurrence for the components marked in X.j.fallback.
+__device__ void j2c_fill(int n, const double *jq, double *q, const double *m, const WfShared &S,
+                         const WfScratch &X)
+{
+    const int tid = threadIdx.x;
+    const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
     const bool f0 = X.j.fallback[0], f1 = X.j.fallback[1], f2 = X.j.fallback[2];
+    if (tid < 3) {
+        if (X.j.fallback[tid])
+            j2c_component(n, tid, jq, q, m, S);
+        else
+            q[tid] = X.j.q0[tid];
+    }
     for (int i = 1 + tid; i < n; i += WF_BLOCK) {
-        // gap: the largest massive ordinal at or below i
-        int lo = 0, hi = nm - 1, g = 0;
-        while (lo <= hi) {
-            const int mid = (lo + hi) >> 1;
-            if (S.mlist[mid] <= i) {
-                g = mid;
-                lo = mid + 1;
-            } else {
-                hi = mid - 1;
-            }
-        }
+        const int g = massive_floor(S, nm, i);
         const bool event = S.mlist[g] == i;
-        const int top = (g + 1 < nm ? S.mlist[g + 1] : n) - 1;
         for (int d = 0; d < 3; d++) {
             if (d == 0 ? f0 : (d == 1 ? f1 : f2)) continue;
-            double cn;
-            if (event) {
-                cn = X.j.mq[g][d];
-            } else {
-                const int r = top - i, k = X.j.nexp[g][d];
-                cn = X.j.exp[g][r < k ? r : k - 1][d];
-            }
+            const double cn = event ? X.j.mq[g][d] : run_quotient(S, X, nm, n, g, i, d);
             q[3 * i + d] = jq[3 * i + d] + cn;
         }
     }
     __syncthreads();
+}
+
+// Both halves, with the non-finite check over the massless particles.
+__device__ void j2c_block(int n, const double *jq, double *q, const double *m, WfShared &S,
+                          WfScratch &X)
+{
+    if (threadIdx.x == 0) S.bad = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
+        if (m[i] != 0.0) continue;  // (runs hold massless particles only)
+        for (int d = 0; d < 3; d++)
+            if (!isfinite(jq[3 * i + d])) bad |= 1 << d;
+    }
+    if (bad) atomicOr(&S.bad, bad);
+    __syncthreads();
+    j2c_quotients(n, jq, S, X, S.bad);
+    j2c_fill(n, jq, q, m, S, X);
 }
 
 // Cartesian -> Jacobi (src/integrator_whfast.c:682-724), one lane per
@@ -525,21 +596,24 @@ __device__ void c2j_component(int n, int d, const double *q, double *jq, const d
 // Interaction accelerations and kick for particle i >= 1 (massless form,
 // src/integrator_whfast.c:1022-1258; the pairwise form is the same with
 // every particle in the massive list).
-__device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const double *x,
-                           const double *jx, const double *m, double G, double eps3,
-                           double *jv, double dt)
+// xi, x0: Cartesian positions of particle i and of particle 0; jxi: its
+// Jacobi position; ginside: for a massless particle, the number of massive
+// particles inside it.
+__device__ __forceinline__ void accel_kick_v(const WfShared &S, int i, bool massive, int p,
+                                             const double *xi, const double *x0,
+                                             const double *jxi, int ginside, double m0, double G,
+                                             double eps3, double *jv, double dt)
 {
     double a[3];
-    const double tv0 = x[3 * i] - x[0], tv1 = x[3 * i + 1] - x[1], tv2 = x[3 * i + 2] - x[2];
+    const double tv0 = xi[0] - x0[0], tv1 = xi[1] - x0[1], tv2 = xi[2] - x0[2];
     const double c0 = cube_plus(tv0, tv1, tv2, eps3);
-    const double cj = cube_plus(jx[3 * i], jx[3 * i + 1], jx[3 * i + 2], eps3);
+    const double cj = cube_plus(jxi[0], jxi[1], jxi[2], eps3);
     const double ei = S.key[i], ei1 = S.key[i - 1];
     const double tv[3] = {tv0, tv1, tv2};
     if (massive) {
-        for (int d = 0; d < 3; d++)
-            a[d] = G * m[0] * ei / ei1 * (jx[3 * i + d] / cj - tv[d] / c0);
+        for (int d = 0; d < 3; d++) a[d] = G * m0 * ei / ei1 * (jxi[d] / cj - tv[d] / c0);
     } else {
-        for (int d = 0; d < 3; d++) a[d] = G * m[0] * (jx[3 * i + d] / cj - tv[d] / c0);
+        for (int d = 0; d < 3; d++) a[d] = G * m0 * (jxi[d] / cj - tv[d] / c0);
     }
     const int nm = S.nm;
     double aux0 = 0.0, aux1 = 0.0, aux2 = 0.0;
@@ -547,8 +621,7 @@ __device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const 
     for (int q = 1; q < nm; q++) {
         const int j = S.mlist[q];
         if (massive ? q >= p : j >= i) break;
-        const double t0 = x[3 * i] - S.mx[q][0], t1 = x[3 * i + 1] - S.mx[q][1],
-                     t2 = x[3 * i + 2] - S.mx[q][2];
+        const double t0 = xi[0] - S.mx[q][0], t1 = xi[1] - S.mx[q][1], t2 = xi[2] - S.mx[q][2];
         const double cc = cube_plus(t0, t1, t2, eps3);
         aux0 += G * S.mm[q] * t0 / cc;
         aux1 += G * S.mm[q] * t1 / cc;
@@ -568,8 +641,7 @@ __device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const 
     for (int q = 1; q < nm; q++) {
         const int j = S.mlist[q];
         if (massive ? q <= p : j <= i) continue;
-        const double t0 = S.mx[q][0] - x[3 * i], t1 = S.mx[q][1] - x[3 * i + 1],
-                     t2 = S.mx[q][2] - x[3 * i + 2];
+        const double t0 = S.mx[q][0] - xi[0], t1 = S.mx[q][1] - xi[1], t2 = S.mx[q][2] - xi[2];
         const double cc = cube_plus(t0, t1, t2, eps3);
         aux0 += G * S.mm[q] * t0 / cc;
         aux1 += G * S.mm[q] * t1 / cc;
@@ -586,11 +658,9 @@ __device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const 
         aux1 = S.strad_m[p][1];
         aux2 = S.strad_m[p][2];
     } else {
-        int g = 0;  // massive particles inside i
-        while (g < nm && S.mlist[g] < i) g++;
-        aux0 = S.strad_l[g][0];
-        aux1 = S.strad_l[g][1];
-        aux2 = S.strad_l[g][2];
+        aux0 = S.strad_l[ginside][0];
+        aux1 = S.strad_l[ginside][1];
+        aux2 = S.strad_l[ginside][2];
     }
     a[0] -= aux0 / ei1;
     a[1] -= aux1 / ei1;
@@ -598,6 +668,17 @@ __device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const 
     jv[3 * i] += a[0] * dt;
     jv[3 * i + 1] += a[1] * dt;
     jv[3 * i + 2] += a[2] * dt;
+}
+
+// The same with the positions read from global memory.
+__device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const double *x,
+                           const double *jx, const double *m, double G, double eps3,
+                           double *jv, double dt)
+{
+    int g = 0;  // massive particles inside i
+    if (!massive)
+        while (g < S.nm && S.mlist[g] < i) g++;
+    accel_kick_v(S, i, massive, p, &x[3 * i], x, &jx[3 * i], g, m[0], G, eps3, jv, dt);
 }
 
 __device__ void load_massive_positions(WfShared &S, const double *x)
@@ -653,6 +734,49 @@ __device__ void accel_kick_all(WfShared &S, int n, const double *x, const double
     if (method == 2) {
         for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK)
             if (m[i] == 0.0) accel_kick(S, i, false, -1, x, jx, m, G, eps3, jv, dt);
+    }
+    __syncthreads();
+}
+
+// Positions from the quotients fused with the kick: the massive particles'
+// positions go to LDS, every massless particle forms its position in
+// registers; global positions are written only when `write_x` (they are
+// an output, not an input, of the next step).  Requires no component marked
+// for the serial recurrence.
+__device__ void kick_from_quotients(WfShared &S, const WfScratch &X, int n, double *x,
+                                    const double *jx, double G, double eps3, double *jv, double dt,
+                                    int method, bool write_x)
+{
+    const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
+    for (int q = threadIdx.x; q < nm; q += WF_BLOCK) {
+        const int j = S.mlist[q];
+        for (int d = 0; d < 3; d++) {
+            const double v = q == 0 ? X.j.q0[d] : X.j.mjq[q][d] + X.j.mq[q][d];
+            S.mx[q][d] = v;
+            if (write_x) x[3 * j + d] = v;
+        }
+    }
+    __syncthreads();
+    straddle_sums(S, G, eps3);
+    const double m0 = S.mm[0];
+    for (int p = 1 + static_cast<int>(threadIdx.x); p < nm; p += WF_BLOCK) {
+        const int i = S.mlist[p];
+        accel_kick_v(S, i, true, p, S.mx[p], S.mx[0], &jx[3 * i], 0, m0, G, eps3, jv, dt);
+    }
+    if (method == 2) {
+        for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
+            const int g = massive_floor(S, nm, i);
+            if (S.mlist[g] == i) continue;
+            const double jxi[3] = {jx[3 * i], jx[3 * i + 1], jx[3 * i + 2]};
+            double xi[3];
+            for (int d = 0; d < 3; d++) xi[d] = jxi[d] + run_quotient(S, X, nm, n, g, i, d);
+            if (write_x) {
+                x[3 * i] = xi[0];
+                x[3 * i + 1] = xi[1];
+                x[3 * i + 2] = xi[2];
+            }
+            accel_kick_v(S, i, false, -1, xi, S.mx[0], jxi, g + 1, m0, G, eps3, jv, dt);
+        }
     }
     __syncthreads();
 }
@@ -718,7 +842,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_whfast_init(WfArgs A)
 }
 
 // `nsteps` steps of the main loop (src/integrator_whfast.c:290-378).
-__global__ __launch_bounds__(WF_BLOCK) void k_whfast_steps(WfArgs A)
+__global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_whfast_steps(WfArgs A)
 {
     __shared__ WfShared S;
     __shared__ WfScratch X;
@@ -733,13 +857,27 @@ __global__ __launch_bounds__(WF_BLOCK) void k_whfast_steps(WfArgs A)
     const double eps3 = A.eps * A.eps * A.eps;
     for (long long s = A.step0; s < A.step0 + A.nsteps; s++) {
         const double dt = (s == A.total_steps - 1) ? A.dt_last : A.dt;
+        const bool last = s == A.step0 + A.nsteps - 1;
         sort_by_distance(S, n, jx, jv, m, ids, jx2, jv2, m2, ids2);
-        massive_list(S, n, m, A.method);
+        // while no massive particle changes place, the massive list, masses
+        // and cumulative masses stand
+        int moved = s == A.step0 ? 1 : 0;
+        const int nmc = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
+        if (!moved)
+            for (int q = threadIdx.x; q < nmc; q += WF_BLOCK) {
+                const int j = S.mlist[q];
+                if (S.idx[j] != j) moved = 1;
+            }
+        if (__syncthreads_or(moved)) massive_list(S, n, m, A.method);
         fill_eta(S, n);
-        if (threadIdx.x == 0) S.any_remove = 0;
+        if (threadIdx.x == 0) {
+            S.any_remove = 0;
+            S.bad = 0;
+        }
         for (int i = threadIdx.x; i < n; i += WF_BLOCK) flags[i] = 0;
         __syncthreads();
         const double gm0 = A.G * m[0];
+        int bad = 0;
         for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
             const double gm = gm0 * S.key[i] / S.key[i - 1];
             const int f = kepler_drift(&jx[3 * i], &jv[3 * i], gm, dt);
@@ -748,21 +886,33 @@ __global__ __launch_bounds__(WF_BLOCK) void k_whfast_steps(WfArgs A)
                 flags[i] = 1;
                 S.any_remove = 1;
             }
+            for (int d = 0; d < 3; d++)
+                if (!isfinite(jx[3 * i + d])) bad |= 1 << d;
         }
+        if (bad) atomicOr(&S.bad, bad);
         __syncthreads();
+        // (S.bad is reused for the velocities below)
+        const int pos_bad = S.bad;
         if (S.any_remove) {
             n = compact(S, n, flags, jx, jv, m, ids, jx2, jv2, m2, ids2);
             massive_list(S, n, m, A.method);
             fill_eta(S, n);
         }
         __syncthreads();  // the removal flags share their LDS with the quotients
-        j2c_block(n, jx, x, m, S, X);
         // Cartesian velocities only matter for the state returned to the
         // caller: rebuild them (from the drifted, not yet kicked, Jacobi
         // velocities, as the reference's last jacobi_to_cartesian does) on
         // the last step of each launch
-        if (s == A.step0 + A.nsteps - 1) j2c_block(n, jv, A.v + 3 * o, m, S, X);
-        accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, dt, A.method);
+        if (last) j2c_block(n, jv, A.v + 3 * o, m, S, X);
+        // (pos_bad covers every drifted particle: a superset of the runs,
+        // and compaction only removes particles)
+        j2c_quotients(n, jx, S, X, pos_bad);
+        if (X.j.fallback[0] | X.j.fallback[1] | X.j.fallback[2]) {
+            j2c_fill(n, jx, x, m, S, X);
+            accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, dt, A.method);
+        } else {
+            kick_from_quotients(S, X, n, x, jx, A.G, eps3, jv, dt, A.method, last);
+        }
     }
     if (threadIdx.x == 0) A.count[blockIdx.x] = n;
 }
@@ -805,6 +955,18 @@ ErrorStatus whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, d
     const long long P = offsets[num_systems] - offsets[0];
     const long long base = offsets[0];
     for (auto &s : sys) s.off -= base;
+    // workgroup b integrates system order[b]: largest systems first, so the
+    // uneven systems pack the CUs with a short tail (per-workgroup counts and
+    // status follow the same order and are mapped back below)
+    std::vector<int> order(num_systems);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return sys[a].cap > sys[b].cap; });
+    {
+        std::vector<WfSys> by(num_systems);
+        for (int b = 0; b < num_systems; b++) by[b] = sys[order[b]];
+        sys.swap(by);
+    }
     // one allocation: 7 x 3P doubles (x, v, jx, jv, jx2, jv2) + 2P doubles (m, m2)
     // + 2P ints (ids, ids2) + per-system arrays
     const size_t b3 = sizeof(double) * 3 * P;
@@ -890,7 +1052,7 @@ ErrorStatus whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, d
     if (!e) e = hipStreamSynchronize(c->stream);
     (void)hipFree(buf);
     if (e) return hip_error(e, "GPU WHFast", __FILE__, __LINE__, __func__);
-    for (int s = 0; s < num_systems; s++) out_counts[s] = cnt[s];
+    for (int b = 0; b < num_systems; b++) out_counts[order[b]] = cnt[b];
     if (steps_done) *steps_done = done;
     return st;
 }
