@@ -42,7 +42,8 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
         case ACCELERATION_METHOD_BARNES_HUT:
         case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE: {
             DeviceTree *t;
-            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, &t));
+            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0,
+                              p->method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE, &t));
             GS_TRY(tree_walk(c, t, x, G, eps, p->opening_angle, a, &g_last_walk_interactions,
                              p->method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE ? 2 : 1));
             return make_success_error_status();
@@ -216,7 +217,7 @@ ErrorStatus construct_octree(LinearOctree *octree, const System *system,
     GS_HIP(hipMemcpyAsync(dm, system->m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     DeviceTree *t;
     GS_TRY(tree_build(c, static_cast<double *>(dx), static_cast<double *>(dm), n,
-                      p->max_num_particles_per_leaf, box_center, box_width, &t));
+                      p->max_num_particles_per_leaf, box_center, box_width, false, &t));
     return tree_to_host(c, t, octree);
 }
 

@@ -149,9 +149,11 @@ hipError_t launch_lf_sync(Context *c, double *v, const double *a, int n, double 
 struct DeviceTree;  // defined in octree.hip
 
 // Build the tree of (x, m) (device arrays, AoS) into the context's buffers.
-// box_center/box_width: NULL/<=0 -> bounding box.
+// box_center/box_width: NULL/<=0 -> bounding box.  with_quad: also the second
+// moments, which the quadrupole and feature walks need.
 ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
-                       const double *box_center, double box_width, DeviceTree **out);
+                       const double *box_center, double box_width, bool with_quad,
+                       DeviceTree **out);
 // Walk; writes a (device, AoS, original order).  order 1: monopoles (the
 // reference's), 2: with quadrupole terms.  feat != NULL: monopole walk that
 // also writes 8 FP32 features per particle for the learned correction.

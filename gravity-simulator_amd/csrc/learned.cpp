@@ -77,7 +77,7 @@ static ErrorStatus coarse_walk(Context *c, const double *x, const double *m, int
                                float **F_out, DeviceTree **t_out, float **wf_out = nullptr)
 {
     DeviceTree *t;
-    GS_TRY(tree_build(c, x, m, n, leaf, nullptr, -1.0, &t));
+    GS_TRY(tree_build(c, x, m, n, leaf, nullptr, -1.0, true, &t));
     void *wf, *F;
     GS_TRY(slot(c, SLOT_FEAT, sizeof(float) * MLP_WALK_FEAT * static_cast<size_t>(n), &wf));
     GS_TRY(slot(c, SLOT_MLP_X, sizeof(float) * MLP_F * static_cast<size_t>(n), &F));

@@ -64,22 +64,43 @@ __global__ __launch_bounds__(BB_BLOCK) void k_bbox_partial(const double *__restr
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[0..2] = centre, out[3] = width (src/linear_octree.c:138-145)
-__global__ void k_bbox_final(const double *__restrict__ part, int nparts, double *__restrict__ out)
+// out[0..2] = centre, out[3] = width (src/linear_octree.c:138-145); one
+// workgroup of BB_BLOCK threads folds the partial boxes
+__global__ __launch_bounds__(BB_BLOCK) void k_bbox_final(const double *__restrict__ part,
+                                                         int nparts, double *__restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double red[6][BB_BLOCK];
     double lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
         lo[d] = part[d];
         hi[d] = part[3 + d];
     }
-    for (int p = 1; p < nparts; p++)
+    for (int p = threadIdx.x; p < nparts; p += BB_BLOCK)
         for (int d = 0; d < 3; d++) {
             lo[d] = fmin(lo[d], part[6 * p + d]);
             hi[d] = fmax(hi[d], part[6 * p + 3 + d]);
         }
-    for (int d = 0; d < 3; d++) out[d] = (hi[d] + lo[d]) / 2.0;
-    out[3] = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    for (int d = 0; d < 3; d++) {
+        red[d][threadIdx.x] = lo[d];
+        red[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int s = BB_BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; d++) {
+                red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+                red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; d++) {
+            lo[d] = red[d][0];
+            hi[d] = red[3 + d][0];
+        }
+        for (int d = 0; d < 3; d++) out[d] = (hi[d] + lo[d]) / 2.0;
+        out[3] = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    }
 }
 
 // ------------------------------------------------------------------ Morton
@@ -216,25 +237,38 @@ __global__ void k_gather_particles(const double *__restrict__ x, const double *_
 
 // ------------------------------------------------------------------ build
 // Child octant ranges of a node of depth d (children at depth d+1) by binary
-// search on its sorted key range (src/linear_octree.c:342-393).
+// search on its sorted key range (src/linear_octree.c:342-393).  The eight
+// searches (first position whose octant digit is > o) are independent and
+// branch-free with a common trip count, so their key loads overlap instead
+// of forming one chain of dependent loads.
 __device__ void octant_bounds(const long long *__restrict__ keys, int first, int count,
                               int child_depth, int bounds[9])
 {
     const int shift = 3 * (MAXL - child_depth);
-    int lo = first;
-    const int end = first + count;
+    int base[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) base[o] = first;
     bounds[0] = first;
-    for (int o = 0; o < 8; o++) {
-        // first position in [lo, end) whose octant digit is > o
-        int a = lo, b = end;
-        while (a < b) {
-            const int mid = (a + b) >> 1;
-            const int oct = static_cast<int>((keys[mid] >> shift) & 7);
-            if (oct <= o) a = mid + 1;
-            else b = mid;
+    if (count <= 0) {
+#pragma unroll
+        for (int o = 0; o < 8; o++) bounds[o + 1] = first;
+        return;
+    }
+    // invariant: the answer for octant o lies in [base[o], base[o] + len]
+    int len = count;
+    while (len > 1) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+            const int oct = static_cast<int>((keys[base[o] + half - 1] >> shift) & 7);
+            base[o] = oct <= o ? base[o] + half : base[o];
         }
-        bounds[o + 1] = a;
-        lo = a;
+        len -= half;
+    }
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+        const int oct = static_cast<int>((keys[base[o]] >> shift) & 7);
+        bounds[o + 1] = base[o] + (oct <= o ? 1 : 0);
     }
 }
 
@@ -292,7 +326,9 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
     nd.nchild = c - c0;
 }
 
-// moments: mom[node] = (sum m, sum m x, sum m y, sum m z)
+// moments: mom[node] = (sum m, sum m x, sum m y, sum m z); with QUAD also
+// the second central moments (only the quadrupole and feature walks read them)
+template <bool QUAD>
 __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
                             const double4 *__restrict__ P, double4 *__restrict__ mom,
                             double *__restrict__ quad)
@@ -332,6 +368,7 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         nd.cy = p.y;
         nd.cz = p.z;
     }
+    if (!QUAD) return;
     // second central moments Q_ab = sum m (x - com)_a (x - com)_b
     // (xx yy zz xy xz yz): leaves from their particles, internal nodes from
     // their children shifted to this centre of mass (parallel axes)
@@ -655,7 +692,7 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
         int nb = (n + BB_BLOCK - 1) / BB_BLOCK;
         if (nb > 1024) nb = 1024;
         hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(BB_BLOCK), 0, c->stream, x, n, bpart);
-        hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, bpart, nb, box);
+        hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_BLOCK), 0, c->stream, bpart, nb, box);
     }
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, n, box, k0,
                        i0);
@@ -690,9 +727,11 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
 }
 
 ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
-                       const double *box_center, double box_width, DeviceTree **out)
+                       const double *box_center, double box_width, bool with_quad,
+                       DeviceTree **out)
 {
     DeviceTree *t = &g_tree;
+    t->has_quad = false;
     // an unfinalised parameter (-1) means the reference's default of one
     // particle per leaf (src/acceleration.c:73-129)
     if (leaf_cap < 1) leaf_cap = 1;
@@ -766,10 +805,15 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
     for (int d = levels - 1; d >= 0; d--) {
         const int lb = t->level_begin[d], le = t->level_end[d];
         if (le <= lb) continue;
-        hipLaunchKernelGGL(k_multipole, dim3((le - lb + 255) / 256), dim3(256), 0, c->stream,
-                           t->nodes, lb, le, t->P, t->mom, t->quad);
+        if (with_quad)
+            hipLaunchKernelGGL(k_multipole<true>, dim3((le - lb + 255) / 256), dim3(256), 0,
+                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
+        else
+            hipLaunchKernelGGL(k_multipole<false>, dim3((le - lb + 255) / 256), dim3(256), 0,
+                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
     }
     GS_HIP(hipGetLastError());
+    t->has_quad = with_quad;
     timing_end(c, rec);
     *out = t;
     return make_success_error_status();
@@ -779,6 +823,9 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
                       double theta, double *a, long long *interactions, int order, float *feat)
 {
     (void)x;
+    if ((order == 2 || feat) && !t->has_quad)
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Quadrupole or feature walk of a tree built without "
+                                              "second moments");
     WalkArgs args;
     args.w2 = t->width * t->width;
     args.theta2 = theta * theta;
