@@ -166,7 +166,10 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const long long *__restric
 
 // Stable scatter: wave w owns tile positions [w*512, (w+1)*512), visited in
 // 8 rounds of 64 consecutive keys; within a round, peers with equal digits
-// are found with 8 ballots and ranked by lane order.
+// are found with 8 ballots and ranked by lane order.  The tile is then
+// reordered by digit in LDS, so that consecutive threads write consecutive
+// addresses of each digit's run instead of scattering single keys.
+static_assert(RS_BLOCK == RS_BINS, "one thread per digit for the tile offsets");
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__restrict__ kin,
                                                          const int *__restrict__ iin, int n,
                                                          int shift,
@@ -176,6 +179,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__rest
                                                          int *__restrict__ iout)
 {
     __shared__ unsigned wcount[RS_WAVES][RS_BINS];
+    __shared__ unsigned toff[RS_BINS];   // first tile position of each digit
+    __shared__ unsigned goff[RS_BINS];   // first output position of each digit
+    __shared__ unsigned wsum[RS_WAVES];
+    __shared__ long long skey[RS_TILE];
+    __shared__ int sval[RS_TILE];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (int b = threadIdx.x; b < RS_WAVES * RS_BINS; b += RS_BLOCK)
@@ -211,15 +219,41 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__rest
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
+    // tile offsets of the digits: exclusive scan of the tile histogram
+    {
+        const int d = threadIdx.x;
+        unsigned tot = 0;
+        for (int w = 0; w < RS_WAVES; w++) tot += wcount[w][d];
+        unsigned incl = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        goff[d] = offs[d * ntiles + blockIdx.x];
+        __syncthreads();
+        unsigned add = 0;
+        for (int w = 0; w < wave; w++) add += wsum[w];
+        toff[d] = add + incl - tot;
+    }
+    __syncthreads();
     for (int r = 0; r < RS_ROUNDS; r++) {
         const int d = dig[r];
         if (d < 0) continue;
-        unsigned off = offs[d * ntiles + blockIdx.x];
-        for (int w = 0; w < wave; w++) off += wcount[w][d];
-        const unsigned dst = off + rank[r];
+        unsigned lp = toff[d] + rank[r];
+        for (int w = 0; w < wave; w++) lp += wcount[w][d];
+        skey[lp] = key[r];
+        sval[lp] = val[r];
+    }
+    __syncthreads();
+    const int tile_n = min(RS_TILE, n - blockIdx.x * RS_TILE);
+    for (int p = threadIdx.x; p < tile_n; p += RS_BLOCK) {
+        const long long k = skey[p];
+        const int d = static_cast<int>((k >> shift) & (RS_BINS - 1));
+        const unsigned dst = goff[d] + (static_cast<unsigned>(p) - toff[d]);
         if (dst < static_cast<unsigned>(n)) {
-            kout[dst] = key[r];
-            iout[dst] = val[r];
+            kout[dst] = k;
+            iout[dst] = sval[p];
         }
     }
 }
