@@ -559,57 +559,69 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         __builtin_amdgcn_wave_barrier();
         const NodeRec nd = nodes[node];
         const bool act = (M & me) != 0;
+        // The VALU executes an instruction for the whole wavefront whatever
+        // its lane mask, so the terms are formed branch-free for every lane
+        // and lanes that do not take them add an exact zero (the running sums
+        // start at +0 and a sum of products that cancels is +0, never -0, so
+        // adding fma(-0, R, a) = a leaves every bit unchanged).
         if (nd.nchild == 0 && nd.count == 1) {
             // single-particle leaf: its monopole is the particle itself
-            if (act && nd.first != s) {
+            const bool use = act && nd.first != s;
+            if (__ballot(use) != 0) {
                 const double4 p = P[nd.first];
-                pp_force(xi0, xi1, xi2, p.x, p.y, p.z, p.w, args.eps2, a0, a1, a2);
-                cnt++;
-                npp++;
-            }
-            continue;
-        }
-        bool open = false;
-        if (act) {
-            const bool included = (key >> (3 * (MAXL - nd.depth))) == nd.prefix;
-            const double R0 = xi0 - nd.cx, R1 = xi1 - nd.cy, R2 = xi2 - nd.cz;
-            const double d2 = R0 * R0 + R1 * R1 + R2 * R2;
-            // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
-            const double l2 = __builtin_amdgcn_ldexp(args.w2, -2 * nd.depth);
-            if (!included && l2 < args.theta2 * d2) {
-                const float ri = __builtin_amdgcn_rsqf(static_cast<float>(d2 + args.eps2));
-                const float mr3 = static_cast<float>(nd.mass) * (ri * ri * ri);
-                const double w = static_cast<double>(mr3);
+                const double R0 = xi0 - p.x, R1 = xi1 - p.y, R2 = xi2 - p.z;
+                const double r2 = R0 * R0 + R1 * R1 + R2 * R2 + args.eps2;
+                const float ri = __builtin_amdgcn_rsqf(static_cast<float>(r2));
+                const float mr3 = static_cast<float>(p.w) * (ri * ri * ri);
+                const double w = static_cast<double>(use ? mr3 : 0.0f);
                 a0 = __builtin_fma(-w, R0, a0);
                 a1 = __builtin_fma(-w, R1, a1);
                 a2 = __builtin_fma(-w, R2, a2);
-                cnt++;
-                if (QUAD || FEAT) {
-                    float o0, o1, o2;
-                    quad_term(quad + 8 * static_cast<size_t>(node), R0, R1, R2, ri, o0, o1, o2);
-                    if (QUAD) {
-                        a0 += static_cast<double>(o0);
-                        a1 += static_cast<double>(o1);
-                        a2 += static_cast<double>(o2);
-                    } else {
+                cnt += use ? 1u : 0u;
+                npp += use ? 1u : 0u;
+            }
+            continue;
+        }
+        const bool included = (key >> (3 * (MAXL - nd.depth))) == nd.prefix;
+        const double R0 = xi0 - nd.cx, R1 = xi1 - nd.cy, R2 = xi2 - nd.cz;
+        const double d2 = R0 * R0 + R1 * R1 + R2 * R2;
+        // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
+        const double l2 = __builtin_amdgcn_ldexp(args.w2, -2 * nd.depth);
+        const bool accept = act && !included && l2 < args.theta2 * d2;
+        const bool open = act && !accept;
+        if (__ballot(accept) != 0) {
+            const float ri = __builtin_amdgcn_rsqf(static_cast<float>(d2 + args.eps2));
+            const float mr3 = static_cast<float>(nd.mass) * (ri * ri * ri);
+            const double w = static_cast<double>(accept ? mr3 : 0.0f);
+            a0 = __builtin_fma(-w, R0, a0);
+            a1 = __builtin_fma(-w, R1, a1);
+            a2 = __builtin_fma(-w, R2, a2);
+            cnt += accept ? 1u : 0u;
+            if (QUAD || FEAT) {
+                float o0, o1, o2;
+                quad_term(quad + 8 * static_cast<size_t>(node), R0, R1, R2, ri, o0, o1, o2);
+                if (QUAD) {
+                    a0 += static_cast<double>(accept ? o0 : 0.0f);
+                    a1 += static_cast<double>(accept ? o1 : 0.0f);
+                    a2 += static_cast<double>(accept ? o2 : 0.0f);
+                } else {
+                    // M l^2 / d^4 and M l^3 / d^5 (l: cell side, d: distance to the COM)
+                    const float lr = static_cast<float>(__builtin_sqrt(l2)) * ri;
+                    const float mr2 = mr3 * static_cast<float>(__builtin_sqrt(d2 + args.eps2));
+                    const float lr3 = lr * lr * lr;
+                    const float k3 = mr3 * lr3;
+                    if (accept) {
                         fq0 += o0;
                         fq1 += o1;
                         fq2 += o2;
-                        // M l^2 / d^4 and M l^3 / d^5 (l: cell side, d: distance to the COM)
-                        const float lr = static_cast<float>(__builtin_sqrt(l2)) * ri;
-                        const float mr2 = mr3 * static_cast<float>(__builtin_sqrt(d2 + args.eps2));
-                        const float lr3 = lr * lr * lr;
                         fs2 = __builtin_fmaf(mr2, lr * lr, fs2);
                         fs3 = __builtin_fmaf(mr2, lr3, fs3);
-                        const float k3 = mr3 * lr3;
                         fv0 = __builtin_fmaf(k3, static_cast<float>(R0), fv0);
                         fv1 = __builtin_fmaf(k3, static_cast<float>(R1), fv1);
                         fv2 = __builtin_fmaf(k3, static_cast<float>(R2), fv2);
                         nacc++;
                     }
                 }
-            } else {
-                open = true;
             }
         }
         const unsigned long long O = __ballot(open);
@@ -625,13 +637,12 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                 }
             }
         } else if (sp + nd.nchild <= WK_STACK) {
-            for (int c = nd.nchild - 1; c >= 0; c--) {
-                if (lane == 0) {
-                    snode[sp] = nd.child0 + c;
-                    smask[sp] = O;
-                }
-                sp++;
+            // children pushed last-to-first (child0 on top), one lane each
+            if (lane < nd.nchild) {
+                snode[sp + lane] = nd.child0 + nd.nchild - 1 - lane;
+                smask[sp + lane] = O;
             }
+            sp += nd.nchild;
             __builtin_amdgcn_wave_barrier();
         } else if (lane == 0 && n_int) {
             // cannot happen for depth <= 21 (at most 7 pending siblings per
