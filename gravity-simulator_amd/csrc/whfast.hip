@@ -30,6 +30,8 @@ namespace gs {
 constexpr int WF_BLOCK = 256;
 constexpr int WF_MAX_N = 2048;      // particles per system (LDS sort capacity)
 constexpr int WF_MAX_MASSIVE = 64;  // massive particles per system
+constexpr int WF_PAIR_NM = 16;      // massive particles up to which pair terms are tabulated
+constexpr int WF_PAIRS = WF_PAIR_NM * (WF_PAIR_NM - 1) / 2;
 
 struct WfSys {
     long long off;  // first particle in the flat arrays
@@ -195,6 +197,7 @@ struct WfShared {
     // and massive particle p (pairs with one member inside, one outside)
     double strad_l[WF_MAX_MASSIVE + 1][3];
     double strad_m[WF_MAX_MASSIVE][3];
+    double pterm[WF_PAIRS][3];  // G m_q m_r (x_r - x_q) / (|x_r - x_q|^3 + eps^3), q < r
     unsigned mbits[WF_MAX_N / 32];  // massive flags, one bit per particle
     int nm;
     int n;
@@ -699,6 +702,45 @@ __device__ void load_massive_positions(WfShared &S, const double *x)
 __device__ void straddle_sums(WfShared &S, double G, double eps3)
 {
     const int nm = S.nm;
+    if (nm <= WF_PAIR_NM) {
+        // every pair term once, in parallel; each sum below then adds its
+        // terms in the same order as the loop further down
+        const int np = nm * (nm - 1) / 2;
+        for (int k = threadIdx.x; k < np; k += WF_BLOCK) {
+            int q = 0, rem = k;
+            while (rem >= nm - 1 - q) {
+                rem -= nm - 1 - q;
+                q++;
+            }
+            const int r = q + 1 + rem;
+            const double t0 = S.mx[r][0] - S.mx[q][0], t1 = S.mx[r][1] - S.mx[q][1],
+                         t2 = S.mx[r][2] - S.mx[q][2];
+            const double cc = cube_plus(t0, t1, t2, eps3);
+            S.pterm[k][0] = G * S.mm[q] * S.mm[r] * t0 / cc;
+            S.pterm[k][1] = G * S.mm[q] * S.mm[r] * t1 / cc;
+            S.pterm[k][2] = G * S.mm[q] * S.mm[r] * t2 / cc;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < 2 * nm; t += WF_BLOCK) {
+            const bool massive = t > nm;
+            const int g = massive ? t - nm : t;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int q = 0; q < g; q++) {
+                const int base = q * (2 * nm - q - 1) / 2 - q - 1;  // + r: pair (q, r)
+                for (int r = massive ? g + 1 : g; r < nm; r++) {
+                    a0 += S.pterm[base + r][0];
+                    a1 += S.pterm[base + r][1];
+                    a2 += S.pterm[base + r][2];
+                }
+            }
+            double *dst = massive ? S.strad_m[g] : S.strad_l[g];
+            dst[0] = a0;
+            dst[1] = a1;
+            dst[2] = a2;
+        }
+        __syncthreads();
+        return;
+    }
     // t = 0..nm: gap g = t; t = nm+1..2nm-1: massive ordinal p = t - nm
     for (int t = threadIdx.x; t < 2 * nm; t += WF_BLOCK) {
         const bool massive = t > nm;
