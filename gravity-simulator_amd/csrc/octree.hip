@@ -567,7 +567,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         if (nd.nchild == 0 && nd.count == 1) {
             // single-particle leaf: its monopole is the particle itself
             const bool use = act && nd.first != s;
-            if (__ballot(use) != 0) {
+            {
                 const double4 p = P[nd.first];
                 const double R0 = xi0 - p.x, R1 = xi1 - p.y, R2 = xi2 - p.z;
                 const double r2 = R0 * R0 + R1 * R1 + R2 * R2 + args.eps2;
@@ -589,7 +589,9 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         const double l2 = __builtin_amdgcn_ldexp(args.w2, -2 * nd.depth);
         const bool accept = act && !included && l2 < args.theta2 * d2;
         const bool open = act && !accept;
-        if (__ballot(accept) != 0) {
+        // (no branch around the terms: a visit where no lane accepts is rare
+        // enough that computing them anyway is cheaper than merging the sums)
+        {
             const float ri = __builtin_amdgcn_rsqf(static_cast<float>(d2 + args.eps2));
             const float mr3 = static_cast<float>(nd.mass) * (ri * ri * ri);
             const double w = static_cast<double>(accept ? mr3 : 0.0f);
