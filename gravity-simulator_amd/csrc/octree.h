@@ -8,14 +8,15 @@ namespace gs {
 // One node, 64 bytes, read by the walk with wave-uniform (scalar) loads.
 struct NodeRec {
     double cx, cy, cz;  // centre of mass
-    double mass;
+    double l2;          // squared cell side, w^2 4^-depth (w: box width)
     long long prefix;   // Morton prefix at `depth` (3*depth bits)
     int depth;
     int count;          // particles in the node
     int first;          // first particle, sorted position
     int child0;         // first child (children are contiguous), -1 for leaves
     int nchild;         // number of children, 0 for leaves
-    int pad;
+    float fmass;        // mass rounded to FP32 (the walk's monopole weight);
+                        // the FP64 mass is mom[node].x
 };
 static_assert(sizeof(NodeRec) == 64, "NodeRec must stay 64 bytes");
 

@@ -326,7 +326,7 @@ __global__ void k_count_children(NodeRec *__restrict__ nodes, int lb, int le, in
     cnt[t] = c;
 }
 
-__global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int leaf_cap,
+__global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int leaf_cap, double w2,
                                 const long long *__restrict__ keys,
                                 const unsigned *__restrict__ offs, int node_cap)
 {
@@ -352,7 +352,9 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
         ch.nchild = 0;
         ch.child0 = -1;
         ch.cx = ch.cy = ch.cz = 0.0;
-        ch.mass = 0.0;
+        // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
+        ch.l2 = __builtin_amdgcn_ldexp(w2, -2 * ch.depth);
+        ch.fmass = 0.0f;
         if (c < node_cap) nodes[c] = ch;
         c++;
     }
@@ -390,7 +392,7 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         }
     }
     mom[node] = make_double4(M, X, Y, Z);
-    nd.mass = M;
+    nd.fmass = static_cast<float>(M);
     if (M != 0.0) {
         nd.cx = X / M;
         nd.cy = Y / M;
@@ -423,7 +425,7 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
             const NodeRec &ch = nodes[c];
             const double d0 = ch.cx - c0, d1 = ch.cy - c1, d2 = ch.cz - c2;
-            const double Mc = ch.mass;
+            const double Mc = mom[c].x;
             const double *qc = quad + 8 * static_cast<size_t>(c);
             q0 += qc[0] + Mc * d0 * d0;
             q1 += qc[1] + Mc * d1 * d1;
@@ -450,7 +452,6 @@ constexpr int WK_WAVES = WK_BLOCK / 64;
 constexpr int WK_STACK = 192;  // >= 7 pending siblings x 21 levels + 8
 
 struct WalkArgs {
-    double w2;  // squared box width
     double theta2;
     double eps2;
     double G;
@@ -528,11 +529,11 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         key = skeys[s];
     }
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    unsigned cnt = 0;
+    unsigned cnt = 0;             // per-lane interactions of the rare opened-leaf path
+    unsigned long long wcnt = 0;  // the wavefront's other interactions (uniform)
     float fq0 = 0.0f, fq1 = 0.0f, fq2 = 0.0f, fs2 = 0.0f, fs3 = 0.0f;
     float fv0 = 0.0f, fv1 = 0.0f, fv2 = 0.0f;
     unsigned nacc = 0, npp = 0;
-    const unsigned long long me = 1ull << lane;
     int *snode = stk_node[wave];
     unsigned long long *smask = stk_mask[wave];
 
@@ -558,7 +559,9 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
             static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(smask[sp])));
         __builtin_amdgcn_wave_barrier();
         const NodeRec nd = nodes[node];
-        const bool act = (M & me) != 0;
+        // (M is uniform: the lane's bit as a per-lane predicate, no VALU work)
+        const bool act = __builtin_amdgcn_inverse_ballot_w64(M);
+        unsigned long long O = 0;  // lanes that open this node
         // The VALU executes an instruction for the whole wavefront whatever
         // its lane mask, so the terms are formed branch-free for every lane
         // and lanes that do not take them add an exact zero (the running sums
@@ -567,6 +570,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         if (nd.nchild == 0 && nd.count == 1) {
             // single-particle leaf: its monopole is the particle itself
             const bool use = act && nd.first != s;
+            wcnt += static_cast<unsigned>(__popcll(__ballot(use)));
             {
                 const double4 p = P[nd.first];
                 const double R0 = xi0 - p.x, R1 = xi1 - p.y, R2 = xi2 - p.z;
@@ -577,28 +581,27 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                 a0 = __builtin_fma(-w, R0, a0);
                 a1 = __builtin_fma(-w, R1, a1);
                 a2 = __builtin_fma(-w, R2, a2);
-                cnt += use ? 1u : 0u;
-                npp += use ? 1u : 0u;
+                if (FEAT) npp += use ? 1u : 0u;
             }
             continue;
         }
         const bool included = (key >> (3 * (MAXL - nd.depth))) == nd.prefix;
         const double R0 = xi0 - nd.cx, R1 = xi1 - nd.cy, R2 = xi2 - nd.cz;
         const double d2 = R0 * R0 + R1 * R1 + R2 * R2;
-        // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
-        const double l2 = __builtin_amdgcn_ldexp(args.w2, -2 * nd.depth);
+        const double l2 = nd.l2;
         const bool accept = act && !included && l2 < args.theta2 * d2;
-        const bool open = act && !accept;
+        const unsigned long long AM = __ballot(accept);
+        wcnt += static_cast<unsigned>(__popcll(AM));
+        O = M & ~AM;  // act && !accept
         // (no branch around the terms: a visit where no lane accepts is rare
         // enough that computing them anyway is cheaper than merging the sums)
         {
             const float ri = __builtin_amdgcn_rsqf(static_cast<float>(d2 + args.eps2));
-            const float mr3 = static_cast<float>(nd.mass) * (ri * ri * ri);
+            const float mr3 = nd.fmass * (ri * ri * ri);
             const double w = static_cast<double>(accept ? mr3 : 0.0f);
             a0 = __builtin_fma(-w, R0, a0);
             a1 = __builtin_fma(-w, R1, a1);
             a2 = __builtin_fma(-w, R2, a2);
-            cnt += accept ? 1u : 0u;
             if (QUAD || FEAT) {
                 float o0, o1, o2;
                 quad_term(quad + 8 * static_cast<size_t>(node), R0, R1, R2, ri, o0, o1, o2);
@@ -626,10 +629,9 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                 }
             }
         }
-        const unsigned long long O = __ballot(open);
         if (O == 0) continue;
         if (nd.nchild == 0) {
-            const bool mine = (O & me) != 0;
+            const bool mine = __builtin_amdgcn_inverse_ballot_w64(O);
             for (int t = nd.first; t < nd.first + nd.count; t++) {
                 const double4 p = P[t];
                 if (mine && t != s) {
@@ -667,7 +669,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
     // interaction count: one atomic per wave
     unsigned long long tot = cnt;
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    if (lane == 0 && n_int) atomicAdd(n_int, tot);
+    if (lane == 0 && n_int) atomicAdd(n_int, tot + wcnt);
 }
 
 // ------------------------------------------------------------------ host side
@@ -800,6 +802,7 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
     root.first = 0;
     root.count = n;
     root.child0 = -1;
+    root.l2 = t->width * t->width;
     GS_HIP(hipMemcpyAsync(t->nodes, &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
     t->level_begin[0] = 0;
     t->level_end[0] = 1;
@@ -830,7 +833,7 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
             GS_HIP(scan_exclusive(c->stream, t->tmp, t->tmp, np, t->scan_tmp, total_dev));
         }
         hipLaunchKernelGGL(k_emit_children, dim3(grid), dim3(256), 0, c->stream, t->nodes, lb, le,
-                           leaf_cap, t->skeys, t->tmp, t->node_cap);
+                           leaf_cap, t->width * t->width, t->skeys, t->tmp, t->node_cap);
         GS_HIP(hipGetLastError());
         t->level_begin[d] = le;
         t->level_end[d] = le + nc;
@@ -843,8 +846,8 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
         const int lb = t->level_begin[d], le = t->level_end[d];
         if (le > lb && d == MAXL) {
             hipLaunchKernelGGL(k_emit_children, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, leaf_cap, t->skeys, t->tmp,
-                               t->node_cap);
+                               c->stream, t->nodes, lb, le, leaf_cap, t->width * t->width,
+                               t->skeys, t->tmp, t->node_cap);
         }
     }
     t->num_levels = levels;
@@ -874,7 +877,6 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Quadrupole or feature walk of a tree built without "
                                               "second moments");
     WalkArgs args;
-    args.w2 = t->width * t->width;
     args.theta2 = theta * theta;
     args.eps2 = eps * eps;
     args.G = G;
@@ -911,7 +913,10 @@ ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out)
 {
     const int n = t->n, nn = t->num_nodes;
     std::vector<NodeRec> h(nn);
+    std::vector<double4> hm(nn);
     GS_HIP(hipMemcpyAsync(h.data(), t->nodes, sizeof(NodeRec) * nn, hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipMemcpyAsync(hm.data(), t->mom, sizeof(double4) * nn, hipMemcpyDeviceToHost,
                           c->stream));
     *out = get_new_linear_octree();
     out->box_width = t->width;
@@ -945,7 +950,7 @@ ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out)
         out->tree_num_internal_children[i] = h[i].nchild;
         out->tree_first_particle_sorted_idx[i] = h[i].first;
         out->tree_first_internal_children_idx[i] = h[i].child0;
-        out->tree_mass[i] = h[i].mass;
+        out->tree_mass[i] = hm[i].x;
         out->tree_center_of_mass_x[i] = h[i].cx;
         out->tree_center_of_mass_y[i] = h[i].cy;
         out->tree_center_of_mass_z[i] = h[i].cz;
