@@ -446,6 +446,183 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
     qn[7] = 0.0;
 }
 
+// ------------------------------------------------------------------ build, all levels at once
+// The same node set and numbering as the level-by-level build, found from
+// the sorted keys alone.  L_i = number of leading octal digits particle i
+// shares with particle i-1 (L_0 = L_n = -1): particle i starts a cell at
+// every depth d > L_i.  The cell of depth d containing i holds at most
+// leaf_cap particles exactly when some a <= i < b with b - a <= leaf_cap has
+// L_a < d and L_b < d, so the first such depth is
+//   D_i = min over those (a, b) of max(L_a, L_b) + 1     (clamped to 1..21:
+// depth 0 is always subdivided, depth 21 never).  The nodes are the pairs
+// (i, d) with L_i < d <= D_i: level d in order of first particle i, internal
+// when d < D_i, its first child being (i, d + 1).
+constexpr int FB_BLOCK = 256;
+constexpr int FB_PER = 1;                   // particles per thread
+constexpr int FB_TILE = FB_BLOCK * FB_PER;  // particles per workgroup
+constexpr int FB_DEPTHS = MAXL + 1;
+constexpr int FB_WAVES = FB_BLOCK / 64;
+constexpr int FB_MAX_LEAF = 16;             // window search cost grows as leaf_cap^2
+
+__device__ __forceinline__ int shared_digits(long long a, long long b)
+{
+    const unsigned long long x = static_cast<unsigned long long>(a ^ b);
+    if (x == 0) return MAXL;
+    const int h = 63 - __clzll(x);  // highest differing bit (keys use bits 0..62)
+    return MAXL - 1 - h / 3;
+}
+
+__global__ void k_fb_digits(const long long *__restrict__ keys, int n, signed char *__restrict__ L)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    L[i] = static_cast<signed char>((i == 0 || i == n) ? -1 : shared_digits(keys[i - 1], keys[i]));
+}
+
+__global__ void k_fb_leafdepth(const signed char *__restrict__ L, int n, int leaf_cap,
+                               signed char *__restrict__ D)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int best = MAXL + 1;
+    const int a0 = i - leaf_cap + 1 > 0 ? i - leaf_cap + 1 : 0;
+    for (int a = a0; a <= i; a++) {
+        const int la = L[a];
+        const int bmax = a + leaf_cap < n ? a + leaf_cap : n;
+        for (int b = i + 1; b <= bmax; b++) {
+            const int lb = L[b];
+            const int v = (la > lb ? la : lb) + 1;
+            best = v < best ? v : best;
+        }
+    }
+    D[i] = static_cast<signed char>(best < 1 ? 1 : (best > MAXL ? MAXL : best));
+}
+
+__device__ __forceinline__ bool fb_node(int l, int dd, int d) { return l < d && d <= dd; }
+
+// nodes per (depth, tile), depth-major for the scan
+__global__ __launch_bounds__(FB_BLOCK) void k_fb_count(const signed char *__restrict__ L,
+                                                       const signed char *__restrict__ D, int n,
+                                                       int ntiles, unsigned *__restrict__ cnt)
+{
+    __shared__ unsigned c[FB_DEPTHS];
+    if (threadIdx.x < FB_DEPTHS) c[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int r = 0; r < FB_PER; r++) {
+        const int i = blockIdx.x * FB_TILE + r * FB_BLOCK + threadIdx.x;
+        const int l = i < n ? L[i] : MAXL + 1;
+        const int dd = i < n ? D[i] : -1;
+        for (int d = 0; d < FB_DEPTHS; d++) {
+            const unsigned long long m = __ballot(fb_node(l, dd, d));
+            if (lane == 0 && m) atomicAdd(&c[d], static_cast<unsigned>(__popcll(m)));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < FB_DEPTHS) cnt[threadIdx.x * ntiles + blockIdx.x] = c[threadIdx.x];
+}
+
+// level offsets (first node of each depth) and the node count
+__global__ void k_fb_levels(const unsigned *__restrict__ scanned, int ntiles,
+                            const unsigned *__restrict__ total, unsigned *__restrict__ out)
+{
+    const int d = threadIdx.x;
+    if (d < FB_DEPTHS) out[d] = scanned[d * ntiles];
+    if (d == FB_DEPTHS) out[d] = *total;
+}
+
+// first position after i whose depth-d prefix differs from i's (galloping)
+__device__ int cell_end(const long long *__restrict__ keys, int n, int i, int shift)
+{
+    const long long pre = keys[i] >> shift;
+    int lo = i + 1, hi = n, step = 1;  // keys[j] share the prefix for i <= j < lo
+    for (;;) {
+        const int j = i + step;
+        if (j >= n) break;
+        if ((keys[j] >> shift) != pre) {
+            hi = j;
+            break;
+        }
+        lo = j + 1;
+        step <<= 1;
+    }
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((keys[mid] >> shift) == pre) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restrict__ keys,
+                                                      const signed char *__restrict__ L,
+                                                      const signed char *__restrict__ D, int n,
+                                                      int ntiles,
+                                                      const unsigned *__restrict__ scanned,
+                                                      double w2, NodeRec *__restrict__ nodes)
+{
+    __shared__ unsigned base[FB_DEPTHS][FB_PER][FB_WAVES];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int l[FB_PER], dd[FB_PER];
+    for (int r = 0; r < FB_PER; r++) {
+        const int i = blockIdx.x * FB_TILE + r * FB_BLOCK + threadIdx.x;
+        l[r] = i < n ? L[i] : MAXL + 1;
+        dd[r] = i < n ? D[i] : -1;
+        for (int d = 0; d < FB_DEPTHS; d++) {
+            const unsigned long long m = __ballot(fb_node(l[r], dd[r], d));
+            if (lane == 0) base[d][r][wave] = static_cast<unsigned>(__popcll(m));
+        }
+    }
+    __syncthreads();
+    // exclusive offsets within the tile, in particle order (r, wave, lane)
+    if (threadIdx.x < FB_DEPTHS) {
+        const int d = threadIdx.x;
+        unsigned acc = scanned[d * ntiles + blockIdx.x];
+        for (int r = 0; r < FB_PER; r++)
+            for (int w = 0; w < FB_WAVES; w++) {
+                const unsigned c = base[d][r][w];
+                base[d][r][w] = acc;
+                acc += c;
+            }
+    }
+    __syncthreads();
+    for (int r = 0; r < FB_PER; r++) {
+        const int i = blockIdx.x * FB_TILE + r * FB_BLOCK + threadIdx.x;
+        int below = -1;  // index of node (i, d + 1)
+        for (int d = FB_DEPTHS - 1; d >= 0; d--) {
+            const bool is = fb_node(l[r], dd[r], d);
+            const unsigned long long m = __ballot(is);
+            if (!is) continue;
+            const int idx = static_cast<int>(base[d][r][wave] + __popcll(m & lt));
+            const int shift = 3 * (MAXL - d);
+            NodeRec nd;
+            nd.cx = nd.cy = nd.cz = 0.0;
+            // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
+            nd.l2 = __builtin_amdgcn_ldexp(w2, -2 * d);
+            nd.prefix = keys[i] >> shift;
+            nd.depth = d;
+            nd.first = i;
+            nd.count = (d == 0 ? n : cell_end(keys, n, i, shift)) - i;
+            nd.fmass = 0.0f;
+            if (d < dd[r]) {
+                int b[9];
+                octant_bounds(keys, i, nd.count, d + 1, b);
+                int nc = 0;
+                for (int o = 0; o < 8; o++) nc += b[o + 1] > b[o] ? 1 : 0;
+                nd.child0 = below;
+                nd.nchild = nc;
+            } else {
+                nd.child0 = -1;
+                nd.nchild = 0;
+            }
+            nodes[idx] = nd;
+            below = idx;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ walk
 constexpr int WK_BLOCK = 256;
 constexpr int WK_WAVES = WK_BLOCK / 64;
@@ -775,6 +952,79 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
     return make_success_error_status();
 }
 
+// Moments bottom-up, one launch per level (deepest first).
+static ErrorStatus tree_moments(Context *c, DeviceTree *t, bool with_quad)
+{
+    for (int d = t->num_levels - 1; d >= 0; d--) {
+        const int lb = t->level_begin[d], le = t->level_end[d];
+        if (le <= lb) continue;
+        if (with_quad)
+            hipLaunchKernelGGL(k_multipole<true>, dim3((le - lb + 255) / 256), dim3(256), 0,
+                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
+        else
+            hipLaunchKernelGGL(k_multipole<false>, dim3((le - lb + 255) / 256), dim3(256), 0,
+                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
+    }
+    GS_HIP(hipGetLastError());
+    return make_success_error_status();
+}
+
+// All levels at once (leaf_cap <= FB_MAX_LEAF): digits and leaf depths per
+// particle, node counts per (depth, tile), one scan and the emission; one
+// host round trip (the level sizes), then the moments level by level.
+static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_cap, bool with_quad)
+{
+    const int ntiles = (n + FB_TILE - 1) / FB_TILE;
+    const long long hn = static_cast<long long>(FB_DEPTHS) * ntiles;
+    if (hn > scan_max_elements())
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Too many particles for the octree: %d", n);
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    const size_t bL = up(static_cast<size_t>(n) + 1), bD = up(n),
+                 bC = up(sizeof(unsigned) * static_cast<size_t>(hn)), bV = up(sizeof(unsigned) * 32);
+    void *p;
+    GS_TRY(slot(c, SLOT_LEVELS, bL + bD + bC + bV, &p));
+    signed char *Ld = static_cast<signed char *>(p);
+    signed char *Dd = Ld + bL;
+    unsigned *cnt = reinterpret_cast<unsigned *>(Dd + bD);
+    unsigned *lev = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(cnt) + bC);
+    GS_TRY(slot(c, SLOT_HIST_SCAN, scan_temp_bytes(hn) + 64, &p));
+    unsigned *stmp = static_cast<unsigned *>(p);
+    GS_TRY(slot(c, SLOT_MISC, 256, &p));
+    unsigned *total_dev = static_cast<unsigned *>(p);
+    void *hp;
+    GS_TRY(pinned(c, 256, &hp));
+    unsigned *lev_host = static_cast<unsigned *>(hp);
+
+    hipLaunchKernelGGL(k_fb_digits, dim3((n + 1 + 255) / 256), dim3(256), 0, c->stream, t->skeys,
+                       n, Ld);
+    hipLaunchKernelGGL(k_fb_leafdepth, dim3((n + 255) / 256), dim3(256), 0, c->stream, Ld, n,
+                       leaf_cap, Dd);
+    hipLaunchKernelGGL(k_fb_count, dim3(ntiles), dim3(FB_BLOCK), 0, c->stream, Ld, Dd, n, ntiles,
+                       cnt);
+    GS_HIP(hipGetLastError());
+    GS_HIP(scan_exclusive(c->stream, cnt, cnt, hn, stmp, total_dev));
+    hipLaunchKernelGGL(k_fb_levels, dim3(1), dim3(64), 0, c->stream, cnt, ntiles, total_dev, lev);
+    GS_HIP(hipMemcpyAsync(lev_host, lev, sizeof(unsigned) * (FB_DEPTHS + 1), hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    const int nn = static_cast<int>(lev_host[FB_DEPTHS]);
+    GS_TRY(ensure_nodes(c, t, nn));
+    hipLaunchKernelGGL(k_fb_emit, dim3(ntiles), dim3(FB_BLOCK), 0, c->stream, t->skeys, Ld, Dd, n,
+                       ntiles, cnt, t->width * t->width, t->nodes);
+    GS_HIP(hipGetLastError());
+    int levels = 0;
+    for (int d = 0; d < FB_DEPTHS; d++) {
+        const int b = static_cast<int>(lev_host[d]);
+        const int e = d + 1 < FB_DEPTHS ? static_cast<int>(lev_host[d + 1]) : nn;
+        t->level_begin[d] = b;
+        t->level_end[d] = e;
+        if (e > b) levels = d + 1;
+    }
+    t->num_levels = levels;
+    t->num_nodes = nn;
+    return tree_moments(c, t, with_quad);
+}
+
 ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
                        const double *box_center, double box_width, bool with_quad,
                        DeviceTree **out)
@@ -794,6 +1044,13 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
     GS_HIP(hipGetLastError());
 
     const int rec = timing_begin(c, "tree_build");
+    if (leaf_cap <= FB_MAX_LEAF) {
+        GS_TRY(tree_build_flat(c, t, n, leaf_cap, with_quad));
+        t->has_quad = with_quad;
+        timing_end(c, rec);
+        *out = t;
+        return make_success_error_status();
+    }
     GS_TRY(ensure_nodes(c, t, 2 * n + 64));
     NodeRec root;
     memset(&root, 0, sizeof(root));
@@ -852,16 +1109,7 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
     }
     t->num_levels = levels;
     t->num_nodes = t->level_end[levels - 1];
-    for (int d = levels - 1; d >= 0; d--) {
-        const int lb = t->level_begin[d], le = t->level_end[d];
-        if (le <= lb) continue;
-        if (with_quad)
-            hipLaunchKernelGGL(k_multipole<true>, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
-        else
-            hipLaunchKernelGGL(k_multipole<false>, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
-    }
+    GS_TRY(tree_moments(c, t, with_quad));
     GS_HIP(hipGetLastError());
     t->has_quad = with_quad;
     timing_end(c, rec);

@@ -81,6 +81,29 @@ def test_tree_identical_to_oracle(golden, cap):
         assert np.array_equal(g[k], t[k]), k
 
 
+@pytest.mark.parametrize("n,cap", [(1, 1), (2, 1), (3, 2), (700, 1), (700, 16), (700, 17), (5000, 3),
+                                   (5000, 40)])
+def test_tree_identical_to_oracle_ragged_and_duplicates(n, cap):
+    """Node arrays and monopoles bitwise equal to the oracle's for tiny inputs,
+    repeated positions (equal keys down to depth 21) and leaf capacities on
+    both sides of the all-levels build's limit (16)."""
+    rng = np.random.default_rng(n + cap)
+    x = rng.standard_normal((n, 3))
+    if n >= 700:
+        x[100:140] = x[100]  # 40 particles at one position
+        x[300:303] = x[7]
+    m = rng.uniform(0.5, 1.5, n) / n
+    g = L.construct_octree(x, m, leaf=cap)
+    t = O.build_tree(x, m, cap)
+    assert g["num_nodes"] == len(t["depth"])
+    for k in ("count", "first", "nchild"):
+        assert np.array_equal(g[k], t[k]), k
+    leaf = g["nchild"] == 0
+    assert np.array_equal(g["child0"][~leaf], t["child0"][~leaf])
+    for k in ("mass", "comx", "comy", "comz"):
+        assert np.array_equal(g[k], t[k]), k
+
+
 def _walk_case(n, seed, theta, eps):
     x, v, m = ic.two_spheres(n, seed)
     t = O.build_tree(x, m, 1)
