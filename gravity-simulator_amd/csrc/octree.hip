@@ -8,9 +8,11 @@
 //   radix sort     stable LSD, 8-bit digits, 8 passes (histogram, scan,
 //                  wave-ranked scatter): the same permutation as the
 //                  reference's stable 9-bit LSD sort
-//   build          level-synchronous: children of the internal nodes of one
-//                  depth are found by binary search on the sorted keys,
-//                  counted, scanned and emitted contiguously (level order)
+//   build          leaf capacity <= 16: every level at once from the shared
+//                  octal digits of neighbouring keys (count per depth and
+//                  tile, one scan, emission in level order); larger leaves:
+//                  level-synchronous (children of one depth found by binary
+//                  search on the sorted keys, counted, scanned, emitted)
 //   multipoles     bottom-up per level; leaves sum their particles in sorted
 //                  order, internal nodes sum their children (FP64)
 //   k_walk         one wavefront walks for 64 Morton-consecutive targets with

@@ -8,8 +8,9 @@
 // bitwise equal to the reference's (with the particle-0 acceleration slot the
 // reference never writes taken as zero).
 //
-// Per step:  distances |jx_i - jx_primary| -> bitonic sort of (distance,
-// index) in LDS -> permute jx, jv, m, ids -> eta (cumulative mass over the
+// Per step:  distances |jx_i - jx_primary| -> sort of (distance, index) in
+// LDS from the previous order (odd-even passes, bitonic fallback) -> move
+// the particles that changed place -> eta (cumulative mass over the
 // massive particles; massless ones add exactly 0) -> universal-variable
 // Kepler drift per particle -> removal of unconverged particles (ascending
 // order) -> Jacobi->Cartesian positions (the recurrence walked over the
