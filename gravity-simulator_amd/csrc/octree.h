@@ -37,6 +37,8 @@ struct DeviceTree {
     NodeRec *nodes = nullptr;       // device: level-ordered nodes
     double4 *mom = nullptr;         // device: (M, sum m x, sum m y, sum m z)
     double *quad = nullptr;         // device: 8 per node (xx yy zz xy xz yz, 0, 0)
+    float *quadf = nullptr;         // device: 8 per node, the walk's FP32 copy
+                                    // (xx yy zz xy xz yz, 1.5 tr, 0)
     bool has_quad = false;          // quad holds this tree's second moments
     unsigned *tmp = nullptr;        // device: per-parent child counts
     unsigned *scan_tmp = nullptr;   // device: scan scratch

@@ -369,7 +369,7 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
 template <bool QUAD>
 __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
                             const double4 *__restrict__ P, double4 *__restrict__ mom,
-                            double *__restrict__ quad)
+                            double *__restrict__ quad, float *__restrict__ quadf)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= le - lb) return;
@@ -446,6 +446,14 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
     qn[5] = q5;
     qn[6] = 0.0;
     qn[7] = 0.0;
+    // the walk's FP32 copy: the six moments rounded to FP32 and 1.5 tr(Q)
+    // formed from them, as the quadrupole term uses them
+    float4 *qf = reinterpret_cast<float4 *>(quadf + 8 * static_cast<size_t>(node));
+    const float fxx = static_cast<float>(q0), fyy = static_cast<float>(q1),
+                fzz = static_cast<float>(q2);
+    qf[0] = make_float4(fxx, fyy, fzz, static_cast<float>(q3));
+    qf[1] = make_float4(static_cast<float>(q4), static_cast<float>(q5), 1.5f * (fxx + fyy + fzz),
+                        0.0f);
 }
 
 // ------------------------------------------------------------------ build, all levels at once
@@ -651,22 +659,20 @@ __device__ __forceinline__ void pp_force(double xi0, double xi1, double xi2, dou
 
 // Quadrupole term of an accepted node (R = x_i - com, r2 = |R|^2 + eps^2),
 // without the factor G: 3 Q R / r^5 - (15/2) (R.Q.R) R / r^7 + (3/2) tr(Q) R / r^5.
-__device__ __forceinline__ void quad_term(const double *__restrict__ q, double R0, double R1,
+__device__ __forceinline__ void quad_term(const float *__restrict__ q, double R0, double R1,
                                           double R2, float ri, float &o0, float &o1, float &o2)
 {
     const float r0 = static_cast<float>(R0), r1 = static_cast<float>(R1),
                 r2 = static_cast<float>(R2);
-    const float qxx = static_cast<float>(q[0]), qyy = static_cast<float>(q[1]),
-                qzz = static_cast<float>(q[2]), qxy = static_cast<float>(q[3]),
-                qxz = static_cast<float>(q[4]), qyz = static_cast<float>(q[5]);
+    // (q: the moments in FP32 and 1.5 tr(Q), written by k_multipole)
+    const float qxx = q[0], qyy = q[1], qzz = q[2], qxy = q[3], qxz = q[4], qyz = q[5];
     const float QR0 = __builtin_fmaf(qxz, r2, __builtin_fmaf(qxy, r1, qxx * r0));
     const float QR1 = __builtin_fmaf(qyz, r2, __builtin_fmaf(qyy, r1, qxy * r0));
     const float QR2 = __builtin_fmaf(qzz, r2, __builtin_fmaf(qyz, r1, qxz * r0));
     const float RQR = __builtin_fmaf(r2, QR2, __builtin_fmaf(r1, QR1, r0 * QR0));
-    const float tr = qxx + qyy + qzz;
     const float ir2 = ri * ri;
     const float ir5 = ir2 * ir2 * ri;
-    const float sc = ir5 * __builtin_fmaf(-7.5f * RQR, ir2, 1.5f * tr);
+    const float sc = ir5 * __builtin_fmaf(-7.5f * RQR, ir2, q[6]);
     const float k3 = 3.0f * ir5;
     o0 = __builtin_fmaf(k3, QR0, sc * r0);
     o1 = __builtin_fmaf(k3, QR1, sc * r1);
@@ -686,7 +692,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                                                    const int *__restrict__ sidx, int n,
                                                    WalkArgs args, double *__restrict__ a,
                                                    unsigned long long *__restrict__ n_int,
-                                                   const double *__restrict__ quad,
+                                                   const float *__restrict__ quad,
                                                    float *__restrict__ feat)
 {
     __shared__ int stk_node[WK_WAVES][WK_STACK];
@@ -875,6 +881,10 @@ static ErrorStatus ensure_nodes(Context *c, DeviceTree *t, int need)
     GS_HIP(hipMalloc(reinterpret_cast<void **>(&nq), sizeof(double) * 8 * cap));
     if (t->quad) GS_HIP(hipFree(t->quad));
     t->quad = nq;
+    float *nqf = nullptr;
+    GS_HIP(hipMalloc(reinterpret_cast<void **>(&nqf), sizeof(float) * 8 * cap));
+    if (t->quadf) GS_HIP(hipFree(t->quadf));
+    t->quadf = nqf;
     unsigned *tc = nullptr;
     GS_HIP(hipMalloc(reinterpret_cast<void **>(&tc), sizeof(unsigned) * (cap + 1)));
     if (t->tmp) GS_HIP(hipFree(t->tmp));
@@ -962,10 +972,10 @@ static ErrorStatus tree_moments(Context *c, DeviceTree *t, bool with_quad)
         if (le <= lb) continue;
         if (with_quad)
             hipLaunchKernelGGL(k_multipole<true>, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
+                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad, t->quadf);
         else
             hipLaunchKernelGGL(k_multipole<false>, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad);
+                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad, t->quadf);
     }
     GS_HIP(hipGetLastError());
     return make_success_error_status();
@@ -1139,13 +1149,13 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
     const int rec = timing_begin(c, "bh_walk");
     if (feat)
         hipLaunchKernelGGL((k_walk<false, true>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
-                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quad, feat);
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, feat);
     else if (order >= 2)
         hipLaunchKernelGGL((k_walk<true, false>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
-                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quad, feat);
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, feat);
     else
         hipLaunchKernelGGL((k_walk<false, false>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
-                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quad, feat);
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, feat);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     if (interactions) {
