@@ -14,6 +14,12 @@ ErrorStatus whfast_gpu(System *system, IntegratorParam *ip, AccelerationParam *a
                        SimulationStatus *status, Settings *settings, double tf);
 ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
                       SimulationStatus *status, Settings *settings, double tf);
+ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
+                      SimulationStatus *status, Settings *settings, double tf, bool cromer);
+ErrorStatus rk4_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
+                    SimulationStatus *status, Settings *settings, double tf);
+ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
+                            SimulationStatus *status, Settings *settings, double tf);
 }
 
 using namespace gs;
@@ -99,16 +105,16 @@ ErrorStatus integrator_launch_simulation(System *system, IntegratorParam *ip,
         case INTEGRATOR_IAS15:
             return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, status, settings, tf));
         case INTEGRATOR_EULER:
+            return WRAP_TRACEBACK(euler_gpu(system, ip, ap, status, settings, tf, false));
         case INTEGRATOR_EULER_CROMER:
+            return WRAP_TRACEBACK(euler_gpu(system, ip, ap, status, settings, tf, true));
         case INTEGRATOR_RK4:
+            return WRAP_TRACEBACK(rk4_gpu(system, ip, ap, status, settings, tf));
         case INTEGRATOR_RKF45:
         case INTEGRATOR_DOPRI:
         case INTEGRATOR_DVERK:
         case INTEGRATOR_RKF78:
-            return WRAP_RAISE_ERROR_FMT(GRAV_NOT_IMPLEMENTED_ERROR,
-                                        "Integrator %d is not provided by the GPU library; "
-                                        "call acceleration() from the host integrator",
-                                        ip->integrator);
+            return WRAP_TRACEBACK(rk_embedded_gpu(system, ip, ap, status, settings, tf));
         default:
             return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "Invalid integrator");
     }

@@ -7,6 +7,7 @@
  */
 #include "oracle.h"
 #include "ias15_coeffs.h"
+#include "rk_tableaus.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -1213,5 +1214,292 @@ int64_t orc_ias15(double *x, double *v, int n, const double *m, double G, int me
     free(txc);
     free(tvc);
     free(d7);
+    return steps;
+}
+
+/* ------------------------------------------------------------------ */
+/* Euler, Euler-Cromer, RK4 (src/integrator.c:276-892)                */
+/* ------------------------------------------------------------------ */
+
+/* src/integrator.c:276-512 (cromer = 0) and :514-630 (cromer = 1): fixed dt
+ * with the last-step clamp, compensated sums for x and v */
+int64_t orc_euler(double *x, double *v, int n, const double *m, double G, int method, double eps,
+                  double dt, double tf, int cromer)
+{
+    const size_t sz = (size_t)n * 3;
+    double *a = malloc(sz * sizeof(double));
+    double *xe = calloc(sz, sizeof(double));
+    double *ve = calloc(sz, sizeof(double));
+    const int64_t total = (int64_t)ceil(tf / dt);
+    double t = 0.0;
+    int64_t steps = 0;
+    while (steps < total)
+    {
+        if (t + dt > tf)
+            dt = tf - t;
+        accel(a, n, x, m, G, method, eps);
+        for (size_t q = 0; q < sz; q++)
+        {
+            const double x0 = x[q], v0 = v[q];
+            if (!cromer)
+            {
+                xe[q] += v[q] * dt;
+                ve[q] += a[q] * dt;
+                x[q] = x0 + xe[q];
+                v[q] = v0 + ve[q];
+                xe[q] += x0 - x[q];
+                ve[q] += v0 - v[q];
+            }
+            else
+            {
+                ve[q] += a[q] * dt;
+                v[q] = v0 + ve[q];
+                ve[q] += v0 - v[q];
+                xe[q] += v[q] * dt;
+                x[q] = x0 + xe[q];
+                xe[q] += x0 - x[q];
+            }
+        }
+        steps++;
+        t = steps * dt;
+    }
+    free(a);
+    free(xe);
+    free(ve);
+    return steps;
+}
+
+/* src/integrator.c:632-892: classic RK4 (no last-step clamp), compensated
+ * sums in the final update */
+int64_t orc_rk4(double *x, double *v, int n, const double *m, double G, int method, double eps,
+                double dt, double tf)
+{
+    const size_t sz = (size_t)n * 3;
+    double *x0 = malloc(sz * sizeof(double)), *v0 = malloc(sz * sizeof(double));
+    double *vk = malloc(4 * sz * sizeof(double)), *xk = malloc(4 * sz * sizeof(double));
+    double *xe = calloc(sz, sizeof(double)), *ve = calloc(sz, sizeof(double));
+    const int64_t total = (int64_t)ceil(tf / dt);
+    int64_t steps = 0;
+    while (steps < total)
+    {
+        memcpy(x0, x, sz * sizeof(double));
+        memcpy(v0, v, sz * sizeof(double));
+        for (int k = 0; k < 4; k++)
+        {
+            if (k > 0)
+            {
+                /* stages 2, 3: half steps from k-1; stage 4: a full step */
+                const double *xp = xk + (size_t)(k - 1) * sz, *vp = vk + (size_t)(k - 1) * sz;
+                for (size_t q = 0; q < sz; q++)
+                {
+                    if (k < 3)
+                    {
+                        x[q] = x0[q] + 0.5 * xp[q] * dt;
+                        v[q] = v0[q] + 0.5 * vp[q] * dt;
+                    }
+                    else
+                    {
+                        x[q] = x0[q] + xp[q] * dt;
+                        v[q] = v0[q] + vp[q] * dt;
+                    }
+                }
+            }
+            accel(vk + (size_t)k * sz, n, x, m, G, method, eps);
+            memcpy(xk + (size_t)k * sz, v, sz * sizeof(double));
+        }
+        for (size_t q = 0; q < sz; q++)
+        {
+            const double *V = vk, *X = xk;
+            ve[q] += (V[q] + 2 * V[sz + q] + 2 * V[2 * sz + q] + V[3 * sz + q]) * dt / 6.0;
+            xe[q] += (X[q] + 2 * X[sz + q] + 2 * X[2 * sz + q] + X[3 * sz + q]) * dt / 6.0;
+            v[q] = v0[q] + ve[q];
+            x[q] = x0[q] + xe[q];
+            ve[q] += v0[q] - v[q];
+            xe[q] += x0[q] - x[q];
+        }
+        steps++;
+    }
+    free(x0);
+    free(v0);
+    free(vk);
+    free(xk);
+    free(xe);
+    free(ve);
+    return steps;
+}
+
+/* ------------------------------------------------------------------ */
+/* Embedded Runge-Kutta (src/integrator_rk_embedded.c:346-987)        */
+/* ------------------------------------------------------------------ */
+
+/* src/integrator_rk_embedded.c:346-520 (the else-less branch at :495-502
+ * makes the power law the only formula) */
+static double rke_initial_dt(int n, const double *x, const double *v, const double *m, double G,
+                             int method, double eps, double tol, int power)
+{
+    const size_t sz = (size_t)n * 3;
+    double *a = malloc(sz * sizeof(double)), *a1 = malloc(sz * sizeof(double));
+    double *x1 = malloc(sz * sizeof(double)), *v1 = malloc(sz * sizeof(double));
+    double *tx = malloc(sz * sizeof(double)), *tv = malloc(sz * sizeof(double));
+    accel(a, n, x, m, G, method, eps);
+    for (size_t q = 0; q < sz; q++)
+    {
+        tx[q] = tol + tol * fabs(x[q]);
+        tv[q] = tol + tol * fabs(v[q]);
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (size_t q = 0; q < sz; q++)
+    {
+        s0 += (x[q] / tx[q]) * (x[q] / tx[q]);
+        s0 += (v[q] / tv[q]) * (v[q] / tv[q]);
+        s1 += (v[q] / tx[q]) * (v[q] / tx[q]);
+        s1 += (a[q] / tv[q]) * (a[q] / tv[q]);
+    }
+    const double d0 = sqrt(s0 / (n * 6)), d1 = sqrt(s1 / (n * 6));
+    const double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-4 : d0 / d1;
+    for (size_t q = 0; q < sz; q++)
+    {
+        x1[q] = x[q] + (dt0 / 100.0) * v[q];
+        v1[q] = v[q] + (dt0 / 100.0) * a[q];
+    }
+    accel(a1, n, x1, m, G, method, eps);
+    for (size_t q = 0; q < sz; q++)
+    {
+        s2 += ((v1[q] - v[q]) / tx[q]) * ((v1[q] - v[q]) / tx[q]);
+        s2 += ((a1[q] - a[q]) / tv[q]) * ((a1[q] - a[q]) / tv[q]);
+    }
+    const double d2 = sqrt(s2 / (n * 6)) / dt0;
+    const double dt1 = pow((double)(0.01L / fmax(d1, d2)), (double)(1.0L / (1 + power)));
+    const double dt = fmin((double)(100.0L * dt0), dt1);
+    free(a);
+    free(a1);
+    free(x1);
+    free(v1);
+    free(tx);
+    free(tv);
+    return dt * 1e-2;
+}
+
+int64_t orc_rk_embedded(double *x, double *v, int n, const double *m, double G, int method,
+                        double eps, int integrator, double tol, double initial_dt, double tf)
+{
+    RkTableau T;
+    if (rk_tableau(integrator, &T) != 0)
+        return -1;
+    const int S = T.stages;
+    const int min_power = T.power < T.power_test ? T.power : T.power_test;
+    double dw[RK_MAX_STAGES];
+    for (int s = 0; s < S; s++)
+        dw[s] = T.weights[s] - T.weights_test[s];
+    const double safety_max = 6.0, safety_min = 0.33;
+    const double safety = pow(0.38, (1.0 / (1.0 + (double)min_power)));
+    const size_t sz = (size_t)n * 3;
+    double *x1 = malloc(sz * sizeof(double)), *v1 = malloc(sz * sizeof(double));
+    double *vk = malloc((size_t)S * sz * sizeof(double)), *xk = malloc((size_t)S * sz * sizeof(double));
+    double *tv = malloc(sz * sizeof(double)), *tx = malloc(sz * sizeof(double));
+    double *dv = malloc(sz * sizeof(double)), *dx = malloc(sz * sizeof(double));
+    double *xe = calloc(sz, sizeof(double)), *ve = calloc(sz, sizeof(double));
+    double *txe = calloc(sz, sizeof(double)), *tve = calloc(sz, sizeof(double));
+
+    double dt;
+    if (initial_dt > 0.0)
+        dt = initial_dt;
+    else
+    {
+        dt = rke_initial_dt(n, x, v, m, G, method, eps, tol, T.power);
+        if (dt > tf)
+            dt = tf;
+    }
+    double t = 0.0;
+    int64_t steps = 0;
+    while (t < tf)
+    {
+        /* stages (src/integrator_rk_embedded.c:737-783) */
+        accel(vk, n, x, m, G, method, eps);
+        memcpy(xk, v, sz * sizeof(double));
+        for (int s = 1; s < S; s++)
+        {
+            for (size_t q = 0; q < sz; q++)
+            {
+                double sv = 0.0, sx = 0.0;
+                for (int i = 0; i < s; i++)
+                {
+                    sv += T.coeff[(s - 1) * (S - 1) + i] * vk[(size_t)i * sz + q];
+                    sx += T.coeff[(s - 1) * (S - 1) + i] * xk[(size_t)i * sz + q];
+                }
+                tv[q] = v[q] + dt * sv + ve[q];
+                tx[q] = x[q] + dt * sx + xe[q];
+            }
+            accel(vk + (size_t)s * sz, n, tx, m, G, method, eps);
+            memcpy(xk + (size_t)s * sz, tv, sz * sizeof(double));
+        }
+        /* weights, error estimate and compensated update (:800-829) */
+        for (size_t q = 0; q < sz; q++)
+        {
+            double sv = 0.0, sx = 0.0, ev = 0.0, ex = 0.0;
+            for (int s = 0; s < S; s++)
+            {
+                sv += T.weights[s] * vk[(size_t)s * sz + q];
+                sx += T.weights[s] * xk[(size_t)s * sz + q];
+                ev += dt * dw[s] * vk[(size_t)s * sz + q];
+                ex += dt * dw[s] * xk[(size_t)s * sz + q];
+            }
+            dv[q] = ev;
+            dx[q] = ex;
+            tve[q] = ve[q] + dt * sv;
+            txe[q] = xe[q] + dt * sx;
+            v1[q] = v[q] + tve[q];
+            x1[q] = x[q] + txe[q];
+            tve[q] += v[q] - v1[q];
+            txe[q] += x[q] - x1[q];
+        }
+        /* error norm (:831-853), summed in index order */
+        double sum = 0.0;
+        for (size_t q = 0; q < sz; q++)
+        {
+            const double sv = tol + fmax(fabs(v[q]), fabs(v1[q])) * tol;
+            const double sx = tol + fmax(fabs(x[q]), fabs(x1[q])) * tol;
+            double e = dv[q] / sv;
+            sum += e * e;
+            e = dx[q] / sx;
+            sum += e * e;
+        }
+        double error = sqrt(sum / (n * 3 * 2));
+        if (error <= 1 || dt <= tf * 1e-12)
+        {
+            steps++;
+            t += dt;
+            memcpy(x, x1, sz * sizeof(double));
+            memcpy(v, v1, sz * sizeof(double));
+            memcpy(xe, txe, sz * sizeof(double));
+            memcpy(ve, tve, sz * sizeof(double));
+        }
+        /* step size (:887-914) */
+        if (error < 1e-10)
+            error = 1e-10;
+        const double dt_new = dt * safety / pow(error, (1.0 / (1.0 + (double)min_power)));
+        if (dt_new > safety_max * dt)
+            dt *= safety_max;
+        else if (dt_new < safety_min * dt)
+            dt *= safety_min;
+        else
+            dt = dt_new;
+        if (dt_new / tf < 1e-12)
+            dt = tf * 1e-12;
+        if (t < tf && t + dt > tf)
+            dt = tf - t;
+    }
+    free(x1);
+    free(v1);
+    free(vk);
+    free(xk);
+    free(tv);
+    free(tx);
+    free(dv);
+    free(dx);
+    free(xe);
+    free(ve);
+    free(txe);
+    free(tve);
     return steps;
 }

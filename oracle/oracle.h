@@ -127,6 +127,15 @@ int orc_whfast(int *n, int32_t *ids, double *x, double *v, double *m, double G, 
 int64_t orc_ias15(double *x, double *v, int n, const double *m, double G, int method, double eps,
                   double tol, double initial_dt, double tf);
 
+/* Euler (cromer 0) / Euler-Cromer (cromer 1), RK4, embedded RK (integrator
+ * ids 5-8); each returns the number of accepted steps */
+int64_t orc_euler(double *x, double *v, int n, const double *m, double G, int method, double eps,
+                  double dt, double tf, int cromer);
+int64_t orc_rk4(double *x, double *v, int n, const double *m, double G, int method, double eps,
+                double dt, double tf);
+int64_t orc_rk_embedded(double *x, double *v, int n, const double *m, double G, int method,
+                        double eps, int integrator, double tol, double initial_dt, double tf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,12 @@ def lib():
         L.orc_whfast.restype = i
         L.orc_ias15.argtypes = [PD, PD, i, PD, d, i, d, d, d, d]
         L.orc_ias15.restype = i64
+        L.orc_euler.argtypes = [PD, PD, i, PD, d, i, d, d, d, i]
+        L.orc_euler.restype = i64
+        L.orc_rk4.argtypes = [PD, PD, i, PD, d, i, d, d, d]
+        L.orc_rk4.restype = i64
+        L.orc_rk_embedded.argtypes = [PD, PD, i, PD, d, i, d, i, d, d, d]
+        L.orc_rk_embedded.restype = i64
         _lib = L
     return _lib
 
@@ -260,6 +266,30 @@ def ias15(x, v, m, G, tol, tf, eps=0.0, method=1, initial_dt=-1.0):
     x, v, m = _c(x).copy(), _c(v).copy(), _c(m)
     steps = lib().orc_ias15(_abi.dptr(x), _abi.dptr(v), len(m), _abi.dptr(m), G, method, eps, tol,
                             initial_dt, tf)
+    return x, v, steps
+
+
+def euler(x, v, m, G, dt, tf, eps=0.0, method=1, cromer=False):
+    """Euler (src/integrator.c:276-512) or Euler-Cromer (:514-630); returns (x, v, steps)."""
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m)
+    steps = lib().orc_euler(_abi.dptr(x), _abi.dptr(v), len(m), _abi.dptr(m), G, method, eps, dt, tf,
+                            1 if cromer else 0)
+    return x, v, steps
+
+
+def rk4(x, v, m, G, dt, tf, eps=0.0, method=1):
+    """RK4 (src/integrator.c:632-892); returns (x, v, steps)."""
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m)
+    steps = lib().orc_rk4(_abi.dptr(x), _abi.dptr(v), len(m), _abi.dptr(m), G, method, eps, dt, tf)
+    return x, v, steps
+
+
+def rk_embedded(x, v, m, G, integrator, tol, tf, eps=0.0, method=1, initial_dt=-1.0):
+    """Embedded RK 5 RKF45 / 6 DOPRI / 7 DVERK / 8 RKF78 (src/integrator_rk_embedded.c:522-987);
+    returns (x, v, accepted steps)."""
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m)
+    steps = lib().orc_rk_embedded(_abi.dptr(x), _abi.dptr(v), len(m), _abi.dptr(m), G, method, eps,
+                                  integrator, tol, initial_dt, tf)
     return x, v, steps
 
 

@@ -29,6 +29,15 @@ exported functions on seeded inputs and stores inputs and outputs as data:
                     over the default G, src/system.c:888-930), tolerance 1e-9,
                     t = 70; and N=64 Plummer (seed 8), eps=1e-2, tolerance
                     1e-9, t = 2: final x, v and step counts
+  f9_rk.npz         Euler, Euler-Cromer, RK4 (src/integrator.c:276-892) and
+                    the embedded RKF45, DOPRI, DVERK, RKF78
+                    (src/integrator_rk_embedded.c:346-987) on N=64 Plummer
+                    (seed 9), eps=1e-2: fixed steps dt=1e-3 to t=0.2 (the
+                    last one clamped) and RK4 with the massless form (half
+                    the particles massless); embedded methods at tolerance
+                    1e-9 to t=1 from the automatic initial step; DOPRI on the
+                    Pythagorean three-body problem, tolerance 1e-12, t=20:
+                    final x, v and step counts
 """
 import hashlib
 import os
@@ -134,8 +143,32 @@ def f8():
     np.savez_compressed(os.path.join(HERE, "f8_ias15.npz"), **out)
 
 
+def f9():
+    out = {}
+    x, v, m = ic.plummer(64, 9)
+    out.update(x=x, v=v, m=m)
+    for integ in (1, 2, 3):
+        ids, xf, vf, mf, steps = O.ref_simulation(x, v, m, 1.0, integ, 1e-3, 0.2005, method=1, eps=1e-2)
+        out.update({f"i{integ}_x": xf, f"i{integ}_v": vf, f"i{integ}_steps": steps})
+        print("integrator", integ, "steps", steps, flush=True)
+    mm = m.copy()
+    mm[32:] = 0.0
+    ids, xf, vf, mf, steps = O.ref_simulation(x, v, mm, 1.0, 3, 1e-3, 0.2005, method=2, eps=1e-2)
+    out.update(rk4_massless_m=mm, rk4_massless_x=xf, rk4_massless_v=vf, rk4_massless_steps=steps)
+    for integ in (5, 6, 7, 8):
+        ids, xf, vf, mf, steps = O.ref_simulation(x, v, m, 1.0, integ, -1.0, 1.0, method=1, eps=1e-2,
+                                                  tolerance=1e-9)
+        out.update({f"i{integ}_x": xf, f"i{integ}_v": vf, f"i{integ}_steps": steps})
+        print("integrator", integ, "steps", steps, flush=True)
+    x, v, m = pyth3()
+    ids, xf, vf, mf, steps = O.ref_simulation(x, v, m, G_DEFAULT, 6, -1.0, 20.0, method=1, tolerance=1e-12)
+    out.update(p3_x=x, p3_v=v, p3_m=m, p3_G=G_DEFAULT, p3_x_final=xf, p3_v_final=vf, p3_steps=steps)
+    print("pyth-3-body DOPRI steps", steps, flush=True)
+    np.savez_compressed(os.path.join(HERE, "f9_rk.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f8"]
+    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f8", "f9"]
     for w in which:
-        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f8": f8}[w]()
+        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f8": f8, "f9": f9}[w]()
         print("wrote", w, flush=True)
