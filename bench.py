@@ -220,6 +220,35 @@ def run_small_leapfrog(L, ic):
     return out
 
 
+def run_integrators(L, ic):
+    """Device-resident integrators through launch_simulation, Plummer N=1024
+    (seed 1), eps=1e-3, FP64 pairwise forces (bitwise with the reference):
+    fixed-step Euler / RK4 at dt=1e-3 to t=0.2, the adaptive DOPRI, RKF78 and
+    IAS15 at tolerance 1e-9 to t=0.2; and RK4 with the FP32 split direct sum at
+    N=2^17 (seed 2), 10 steps."""
+    x, v, m = ic.plummer(1024, 1)
+    out = {"workload": "Plummer N=1024 (seed 1), eps=1e-3, FP64 pairwise forces, to t=0.2"}
+    cases = (("euler", 1, 1e-3, -1.0, 1), ("rk4", 3, 1e-3, -1.0, 4), ("dopri", 6, -1.0, 1e-9, 7),
+             ("rkf78", 8, -1.0, 1e-9, 13), ("ias15", 9, -1.0, 1e-9, None))
+    for name, integ, dt, tol, evals in cases:
+        L.launch_simulation(x[:64], v[:64], m[:64], 1.0, integ, dt, 0.01, method=1, eps=1e-3, tolerance=tol)
+        t0 = time.perf_counter()
+        r = L.launch_simulation(x, v, m, 1.0, integ, dt, 0.2, method=1, eps=1e-3, tolerance=tol)
+        wall = time.perf_counter() - t0
+        steps = int(r[4])
+        out[name] = {"steps": steps, "ms_per_step": 1e3 * wall / max(steps, 1)}
+        if evals:
+            out[name]["force_evals_per_s"] = steps * evals / wall
+    x, v, m = ic.plummer(1 << 17, 2)
+    L.launch_simulation(x[:4096], v[:4096], m[:4096], 1.0, 3, 1e-3, 2e-3, method=16, eps=1e-3)
+    t0 = time.perf_counter()
+    r = L.launch_simulation(x, v, m, 1.0, 3, 1e-3, 1e-2, method=16, eps=1e-3)
+    wall = time.perf_counter() - t0
+    out["rk4_fp32_n131072"] = {"steps": int(r[4]), "ms_per_step": 1e3 * wall / int(r[4]),
+                               "interactions_per_s": 4 * int(r[4]) * float(1 << 17) * ((1 << 17) - 1) / wall}
+    return out
+
+
 def run_whfast(L, ic, steps):
     off, X, V, M = ic.ensemble(4096, 3)
     P = int(off[-1])
@@ -257,6 +286,7 @@ def main():
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--no-whfast", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="Barnes-Hut without the learned correction")
+    ap.add_argument("--no-integrators", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="run the FP32 shape autotuner first (default: the library's default shape)")
     args = ap.parse_args()
@@ -323,6 +353,10 @@ def main():
     if not args.no_bh:
         out["bh"] = run_bh(L, ic, learned=not args.no_learned)
     out["leapfrog_n1024"] = run_small_leapfrog(L, ic)
+    if not args.no_integrators:
+        out["integrators"] = run_integrators(L, ic)
+        log("integrators: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms/step" for k, v in out["integrators"].items()
+                                        if isinstance(v, dict)))
     log(f"leapfrog N=1024: graph {out['leapfrog_n1024']['graph_ms_per_step']:.4f} ms/step, "
         f"eager {out['leapfrog_n1024']['eager_ms_per_step']:.4f} ms/step")
     if not args.no_whfast:

@@ -76,6 +76,18 @@ ErrorStatus ias15(System *system, IntegratorParam *ip, AccelerationParam *ap, Ou
     return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, status, settings, tf));
 }
 
+ErrorStatus rk_embedded(System *system, IntegratorParam *ip, AccelerationParam *ap,
+                        OutputParam *op, SimulationStatus *status, Settings *settings,
+                        const double tf)
+{
+    if (!system || !ip || !ap || !op || !status || !settings)
+        return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    if (op->method != OUTPUT_METHOD_DISABLED)
+        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
+                                "Snapshot output is not provided by the GPU library");
+    return WRAP_TRACEBACK(rk_embedded_gpu(system, ip, ap, status, settings, tf));
+}
+
 ErrorStatus whfast(System *system, IntegratorParam *ip, AccelerationParam *ap, OutputParam *op,
                    SimulationStatus *status, Settings *settings, const double tf)
 {

@@ -41,6 +41,7 @@ EXPORTS = [
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
     "gs_learned_reset", "gs_learned_forward", "gs_learned_gradients", "gs_learned_features",
     "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "ias15", "whfast",
+    "rk_embedded",
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
     "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
 ]
@@ -123,6 +124,8 @@ def load(path=None):
                        C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
         "whfast": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
                         C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
+        "rk_embedded": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
+                             C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

@@ -289,17 +289,29 @@ IntegratorParam get_new_integrator_param(void);
 /* replaces src/integrator.h:45 (src/integrator.c:131-189) */
 ErrorStatus finalize_integration_param(IntegratorParam *GS_RESTRICT integration_param);
 
-/* replaces src/integrator.h:62-70 (src/integrator.c:191-279).  LEAPFROG runs
- * device-resident on the GPU (src/integrator.c:894-1121 semantics, identical
- * FP64 op order); WHFAST runs the GPU WHFast (src/integrator_whfast.c:200-407
- * semantics); IAS15 runs the device-resident IAS15 below.  The other
- * integrators return GRAV_NOT_IMPLEMENTED_ERROR. */
+/* replaces src/integrator.h:62-70 (src/integrator.c:191-279).  Every
+ * integrator runs device-resident on the GPU with the reference's FP64
+ * operation order: EULER, EULER_CROMER, RK4 (src/integrator.c:276-892),
+ * LEAPFROG (:894-1121), the embedded RKF45 / DOPRI / DVERK / RKF78 below,
+ * IAS15 and WHFAST (src/integrator_whfast.c:200-407 semantics).  Output must
+ * be OUTPUT_METHOD_DISABLED. */
 ErrorStatus integrator_launch_simulation(System *GS_RESTRICT system,
                                          IntegratorParam *GS_RESTRICT integrator_param,
                                          AccelerationParam *GS_RESTRICT acceleration_param,
                                          OutputParam *GS_RESTRICT output_param,
                                          SimulationStatus *GS_RESTRICT simulation_status,
                                          Settings *GS_RESTRICT settings, const double tf);
+
+/* replaces src/integrator.h:85-93 (src/integrator_rk_embedded.c:522-987):
+ * the embedded Runge-Kutta methods (integrator_param->integrator RKF45,
+ * DOPRI, DVERK or RKF78) with the reference's tableaus, initial step, error
+ * norm (summed in index order) and step-size control, every per-component
+ * update an FP64 GPU kernel in the reference's operation order (bitwise equal
+ * to the reference with the FP64 direct sums).  The FP32 force methods are
+ * refused (GRAV_VALUE_ERROR).  Output must be OUTPUT_METHOD_DISABLED. */
+ErrorStatus rk_embedded(System *system, IntegratorParam *integrator_param,
+                        AccelerationParam *acceleration_param, OutputParam *output_param,
+                        SimulationStatus *simulation_status, Settings *settings, const double tf);
 
 /* replaces src/integrator.h:108-116 (src/integrator_ias15.c:229-615):
  * IAS15 with the reference's predictor-corrector, step-size control and
