@@ -415,6 +415,31 @@ ErrorStatus gs_sim_download(GsSim *s, double *x, double *v)
     return make_success_error_status();
 }
 
+}  // extern "C"
+
+namespace gs {
+
+// Snapshot of a Leapfrog run between steps: x, and v_{n+1/2} brought back to
+// v_n = v - 0.5 a dt in a scratch copy (src/integrator.c:1045-1072); the run's
+// own v is left as it is.
+ErrorStatus sim_leapfrog_snapshot(GsSim *s, SnapshotWriter *w, double t, double dt)
+{
+    if (!s || !w) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    double *vs;
+    GS_TRY(w->scratch(&vs));
+    GS_HIP(hipMemcpyAsync(vs, s->v, sizeof(double) * 3 * s->n, hipMemcpyDeviceToDevice,
+                          c->stream));
+    GS_HIP(launch_lf_sync(c, vs, s->a, s->n, dt));
+    return w->write_device(s->x, vs, t, dt);
+}
+
+}  // namespace gs
+
+extern "C" {
+
 ErrorStatus gs_sim_acceleration(GsSim *s, const AccelerationParam *p, double *a_host)
 {
     if (!s || !p) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");

@@ -167,4 +167,43 @@ ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out);
 ErrorStatus device_acceleration(Context *c, const double *x, const double *m, int n, double G,
                                 const AccelerationParam *p, double *a);
 
+// ---------------------------------------------------------------- snapshots
+// CSV snapshots of a device-resident run (file name, metadata, header and row
+// formats of src/output.c:351-469; output times of src/integrator.c:400-415).
+// A snapshot is opened by the caller's thread (so an unwritable directory
+// fails at the step that asks for it), copied device -> pinned host on the
+// run's stream behind an event, and formatted by a pool of writer threads
+// while the run continues.  Particle ids and masses are taken once at
+// begin(): the runs that use this keep them fixed.
+class SnapshotWriter {
+public:
+    SnapshotWriter() = default;
+    ~SnapshotWriter();
+    SnapshotWriter(const SnapshotWriter &) = delete;
+    SnapshotWriter &operator=(const SnapshotWriter &) = delete;
+    // op == NULL or OUTPUT_METHOD_DISABLED: off, every call below is a no-op.
+    ErrorStatus begin(Context *c, OutputParam *op, const System *sys);
+    bool on() const { return impl_ != nullptr; }
+    // the reference's test `t >= next_output_time`
+    bool due(double t) const;
+    // state in host memory (the initial snapshot)
+    ErrorStatus write_host(const double *x, const double *v, double t, double dt);
+    // state in device memory, copied on c->stream behind the work queued so far
+    ErrorStatus write_device(const double *x, const double *v, double t, double dt);
+    // device buffer of 3 n doubles for a derived velocity (allocated on first use)
+    ErrorStatus scratch(double **out);
+    // waits for every queued snapshot; the first writer error, if any
+    ErrorStatus finish();
+
+private:
+    struct Impl;
+    Impl *impl_ = nullptr;
+};
+
+// One CSV snapshot written synchronously from host arrays (the body of
+// output_snapshot_csv, src/output.c:351-469).
+ErrorStatus write_csv_snapshot(const char *path, int n, double G, double t, double dt,
+                               const int32 *ids, const double *m, const double *x,
+                               const double *v, const OutputParam *op);
+
 }  // namespace gs

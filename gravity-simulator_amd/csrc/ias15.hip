@@ -322,7 +322,8 @@ static ErrorStatus ias15_initial_dt(IasState &S, int n, double G, const Accelera
 
 // The main loop of src/integrator_ias15.c:229-615 on device state.
 ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                      SimulationStatus *status, Settings *settings, const double tf)
+                      OutputParam *op, SimulationStatus *status, Settings *settings,
+                      const double tf)
 {
     const int n = system->num_particles;
     if (n <= 0) return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "Number of particles must be positive");
@@ -345,6 +346,8 @@ ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
     GS_HIP(hipMemcpyAsync(S.v, system->v, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
     GS_HIP(hipMemcpyAsync(S.m, system->m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     const double G = system->G;
+    SnapshotWriter snaps;
+    GS_TRY(snaps.begin(c, op, system));
 
     Ias15Coeffs K;
     ias15_coeffs(&K);
@@ -367,6 +370,10 @@ ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
         if (dt > tf) dt = tf;
     }
 
+    // initial snapshot after the initial step size, before the status is
+    // reset (src/integrator_ias15.c:381-410)
+    if (snaps.on() && op->output_initial)
+        GS_TRY(snaps.write_host(system->x, system->v, status->t, status->dt));
     status->t = 0.0;
     status->dt = dt;
     status->num_steps = 0;
@@ -415,6 +422,7 @@ ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
             std::swap(S.x, S.x1);
             std::swap(S.v, S.v1);
             std::swap(S.a, S.a1);
+            if (snaps.due(status->t)) GS_TRY(snaps.write_device(S.x, S.v, status->t, dt));
         }
 
         if (dt_new > (dt / safety_fac))
@@ -434,6 +442,7 @@ ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
         if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
     }
 
+    GS_TRY(snaps.finish());
     GS_HIP(hipMemcpyAsync(system->x, S.x, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipMemcpyAsync(system->v, S.v, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));

@@ -13,22 +13,33 @@ namespace gs {
 ErrorStatus whfast_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
                        SimulationStatus *status, Settings *settings, double tf);
 ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                      SimulationStatus *status, Settings *settings, double tf);
+                      OutputParam *op, SimulationStatus *status, Settings *settings, double tf);
 ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                      SimulationStatus *status, Settings *settings, double tf, bool cromer);
+                      OutputParam *op, SimulationStatus *status, Settings *settings, double tf,
+                      bool cromer);
 ErrorStatus rk4_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                    SimulationStatus *status, Settings *settings, double tf);
+                    OutputParam *op, SimulationStatus *status, Settings *settings, double tf);
 ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                            SimulationStatus *status, Settings *settings, double tf);
+                            OutputParam *op, SimulationStatus *status, Settings *settings,
+                            double tf);
+ErrorStatus sim_leapfrog_snapshot(GsSim *s, SnapshotWriter *w, double t, double dt);
 }
 
 using namespace gs;
 
 static ErrorStatus leapfrog_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                                SimulationStatus *status, Settings *settings, const double tf)
+                                OutputParam *op, SimulationStatus *status, Settings *settings,
+                                const double tf)
 {
     const int n = system->num_particles;
     double dt = ip->dt;
+    Context *c;
+    GS_TRY(context(&c));
+    SnapshotWriter snaps;
+    GS_TRY(snaps.begin(c, op, system));
+    // initial snapshot before the first half kick (src/integrator.c:945-959)
+    if (snaps.on() && op->output_initial)
+        GS_TRY(snaps.write_host(system->x, system->v, status->t, status->dt));
     GsSim *sim;
     GS_TRY(gs_sim_create(&sim, n, system->G));
     ErrorStatus st = gs_sim_upload(sim, system->x, system->v, system->m);
@@ -40,13 +51,22 @@ static ErrorStatus leapfrog_gpu(System *system, IntegratorParam *ip, Acceleratio
     // steps 0 .. total-2 end at (i+1) dt <= (total-1) dt < tf, so only the last
     // one can be clamped (src/integrator.c:1001-1006): the others run in
     // chunks (a replayed HIP graph for the direct-sum methods), checking the
-    // exit flag between chunks
+    // exit flag between chunks.  A chunk ends early at a step that is due for
+    // a snapshot (t = steps dt >= next output time, src/integrator.c:1045).
     const int64 chunk = 1024;
     while (st.return_code == GRAV_SUCCESS && status->num_steps < total - 1) {
-        const int64 k = std::min(chunk, total - 1 - status->num_steps);
+        int64 k = std::min(chunk, total - 1 - status->num_steps);
+        if (snaps.on())
+            for (int64 j = 1; j < k; j++)
+                if (snaps.due(static_cast<double>(status->num_steps + j) * dt)) {
+                    k = j;
+                    break;
+                }
         st = gs_sim_leapfrog_steps(sim, ap, dt, k);
         status->num_steps += k;
         status->t = status->num_steps * dt;
+        if (st.return_code == GRAV_SUCCESS && snaps.due(status->t))
+            st = sim_leapfrog_snapshot(sim, &snaps, status->t, dt);
         if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
     }
     const bool exited = settings->is_exit_ptr && *settings->is_exit_ptr;
@@ -56,10 +76,14 @@ static ErrorStatus leapfrog_gpu(System *system, IntegratorParam *ip, Acceleratio
         st = gs_sim_leapfrog_steps(sim, ap, dt, 1);
         status->num_steps++;
         status->t = status->num_steps * dt;
+        if (st.return_code == GRAV_SUCCESS && snaps.due(status->t))
+            st = sim_leapfrog_snapshot(sim, &snaps, status->t, dt);
     }
     if (st.return_code == GRAV_SUCCESS) st = gs_sim_leapfrog_end(sim, dt);
     if (st.return_code == GRAV_SUCCESS) st = gs_sim_download(sim, system->x, system->v);
+    const ErrorStatus fin = snaps.finish();
     gs_sim_destroy(sim);
+    if (st.return_code == GRAV_SUCCESS) return fin;
     return st;
 }
 
@@ -70,10 +94,7 @@ ErrorStatus ias15(System *system, IntegratorParam *ip, AccelerationParam *ap, Ou
 {
     if (!system || !ip || !ap || !op || !status || !settings)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
-    if (op->method != OUTPUT_METHOD_DISABLED)
-        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
-                                "Snapshot output is not provided by the GPU library");
-    return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, status, settings, tf));
+    return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, op, status, settings, tf));
 }
 
 ErrorStatus rk_embedded(System *system, IntegratorParam *ip, AccelerationParam *ap,
@@ -82,10 +103,7 @@ ErrorStatus rk_embedded(System *system, IntegratorParam *ip, AccelerationParam *
 {
     if (!system || !ip || !ap || !op || !status || !settings)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
-    if (op->method != OUTPUT_METHOD_DISABLED)
-        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
-                                "Snapshot output is not provided by the GPU library");
-    return WRAP_TRACEBACK(rk_embedded_gpu(system, ip, ap, status, settings, tf));
+    return WRAP_TRACEBACK(rk_embedded_gpu(system, ip, ap, op, status, settings, tf));
 }
 
 ErrorStatus whfast(System *system, IntegratorParam *ip, AccelerationParam *ap, OutputParam *op,
@@ -95,7 +113,7 @@ ErrorStatus whfast(System *system, IntegratorParam *ip, AccelerationParam *ap, O
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
     if (op->method != OUTPUT_METHOD_DISABLED)
         return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
-                                "Snapshot output is not provided by the GPU library");
+                                "Snapshot output is not provided for the GPU WHFast");
     return WRAP_TRACEBACK(whfast_gpu(system, ip, ap, status, settings, tf));
 }
 
@@ -106,27 +124,24 @@ ErrorStatus integrator_launch_simulation(System *system, IntegratorParam *ip,
 {
     if (!system || !ip || !ap || !op || !status || !settings)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
-    if (op->method != OUTPUT_METHOD_DISABLED)
-        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
-                                "Snapshot output is not provided by the GPU library");
     switch (ip->integrator) {
         case INTEGRATOR_LEAPFROG:
-            return WRAP_TRACEBACK(leapfrog_gpu(system, ip, ap, status, settings, tf));
+            return WRAP_TRACEBACK(leapfrog_gpu(system, ip, ap, op, status, settings, tf));
         case INTEGRATOR_WHFAST:
-            return WRAP_TRACEBACK(whfast_gpu(system, ip, ap, status, settings, tf));
+            return WRAP_TRACEBACK(whfast(system, ip, ap, op, status, settings, tf));
         case INTEGRATOR_IAS15:
-            return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, status, settings, tf));
+            return WRAP_TRACEBACK(ias15_gpu(system, ip, ap, op, status, settings, tf));
         case INTEGRATOR_EULER:
-            return WRAP_TRACEBACK(euler_gpu(system, ip, ap, status, settings, tf, false));
+            return WRAP_TRACEBACK(euler_gpu(system, ip, ap, op, status, settings, tf, false));
         case INTEGRATOR_EULER_CROMER:
-            return WRAP_TRACEBACK(euler_gpu(system, ip, ap, status, settings, tf, true));
+            return WRAP_TRACEBACK(euler_gpu(system, ip, ap, op, status, settings, tf, true));
         case INTEGRATOR_RK4:
-            return WRAP_TRACEBACK(rk4_gpu(system, ip, ap, status, settings, tf));
+            return WRAP_TRACEBACK(rk4_gpu(system, ip, ap, op, status, settings, tf));
         case INTEGRATOR_RKF45:
         case INTEGRATOR_DOPRI:
         case INTEGRATOR_DVERK:
         case INTEGRATOR_RKF78:
-            return WRAP_TRACEBACK(rk_embedded_gpu(system, ip, ap, status, settings, tf));
+            return WRAP_TRACEBACK(rk_embedded_gpu(system, ip, ap, op, status, settings, tf));
         default:
             return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "Invalid integrator");
     }
@@ -190,7 +205,8 @@ int launch_simulation_python(int32 *num_particles, int32 *particle_ids, double *
     op.velocity_output_dtype = velocity_output_dtype;
     op.mass_output_dtype = mass_output_dtype;
 
-    SimulationStatus status;
+    // (the reference leaves this uninitialised; an initial snapshot prints it)
+    SimulationStatus status = {0.0, 0.0, 0};
     Settings settings = get_new_settings();
     settings.verbose = verbose;
     settings.enable_progress_bar = enable_progress_bar;

@@ -1,7 +1,11 @@
 // params.cpp -- parameter structs, defaults and validation with the
 // reference's defaults, messages and error codes.
+#include <sys/stat.h>
+
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <ctime>
 
 #include "gs_internal.h"
 
@@ -116,27 +120,57 @@ OutputParam get_new_output_param(void)
     return p;
 }
 
-// src/output.c:100-124 (method check).  Snapshot writing is outside this
-// library, so the file-based methods report GRAV_NOT_IMPLEMENTED_ERROR.
+// src/output.c:100-245: method check (HDF5 is not built in, as in a
+// reference build without HDF5), positive interval, a directory path that
+// ends in '/' (snapshots_YYYYMMDD_HHMMSS/ when none is given), created here;
+// an existing directory gives a warning.
 ErrorStatus finalize_output_param(OutputParam *p, const Settings *settings)
 {
-    (void)settings;
     if (!p)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "Output parameter pointer is NULL");
     switch (p->method) {
         case OUTPUT_METHOD_DISABLED:
             return make_success_error_status();
         case OUTPUT_METHOD_CSV:
+            break;
         case OUTPUT_METHOD_HDF5:
-            return WRAP_RAISE_ERROR_FMT(
-                GRAV_NOT_IMPLEMENTED_ERROR,
-                "Snapshot output (method %d) is not provided by the GPU library; use "
-                "OUTPUT_METHOD_DISABLED and read the state back",
-                p->method);
+            return WRAP_RAISE_ERROR(GRAV_MEMORY_ERROR, "HDF5 output method is not available");
         default:
             return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unknown output method. Got: %d",
                                         p->method);
     }
+    if (p->output_interval <= 0.0)
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Output interval must be positive. Got: %.17g",
+                                    p->output_interval);
+    if (!p->output_dir) {
+        const size_t len = strlen("snapshots_YYYYMMDD_HHMMSS/") + 1;
+        char *dir = static_cast<char *>(malloc(len));
+        if (!dir)
+            return WRAP_RAISE_ERROR(GRAV_MEMORY_ERROR,
+                                    "Failed to allocate memory for directory path.");
+        const time_t now = time(nullptr);
+        strftime(dir, len, "snapshots_%Y%m%d_%H%M%S/", localtime(&now));
+        p->output_dir = dir;
+    } else {
+        const size_t len = strlen(p->output_dir);
+        if (len == 0 || p->output_dir[len - 1] != '/')
+            return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                        "Directory path for storing snapshots must end with a "
+                                        "trailing slash (\"/\"). Got: \"%s\".",
+                                        p->output_dir);
+    }
+    if (mkdir(p->output_dir, 0777) == -1) {
+        struct stat st;
+        if (stat(p->output_dir, &st) != 0)
+            return WRAP_RAISE_ERROR_FMT(GRAV_OS_ERROR,
+                                        "Failed to access path for storing snapshots: \"%s\".",
+                                        p->output_dir);
+        if ((st.st_mode & S_IFDIR) && settings && settings->verbose >= GRAV_VERBOSITY_IGNORE_INFO)
+            GS_TRY(WRAP_RAISE_WARNING_FMT("Directory for storing snapshots already exists. The "
+                                          "files will be overwritten. Directory: \"%s\".",
+                                          p->output_dir));
+    }
+    return make_success_error_status();
 }
 
 // src/settings.c:5-14: every Settings shares one static exit flag

@@ -1,6 +1,7 @@
 // python_iface.cpp -- the remaining ctypes entry points the grav_sim Python
 // package binds at start-up (grav_sim/utils.py:38-58,
-// src/python_interface.c:8-92,245-304).  Host-side and off the hot path.
+// src/python_interface.c:8-92,245-304).  Host-side and off the hot path;
+// load_built_in_system_python is in systems.cpp.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,26 +13,6 @@ extern "C" {
 void free_memory_int32(int32 *ptr) { free(ptr); }
 
 void free_memory_double(double *ptr) { free(ptr); }
-
-// Built-in initial conditions (src/system.c:579-1095) are data tables of the
-// reference's examples, not part of the force path: report that clearly.
-int32 load_built_in_system_python(const char *system_name, int *num_particles_ptr,
-                                  int32 **particle_ids_ptr, double **x_ptr, double **v_ptr,
-                                  double **m_ptr, double *G_ptr)
-{
-    (void)num_particles_ptr;
-    (void)particle_ids_ptr;
-    (void)x_ptr;
-    (void)v_ptr;
-    (void)m_ptr;
-    (void)G_ptr;
-    ErrorStatus st = WRAP_RAISE_ERROR_FMT(
-        GRAV_NOT_IMPLEMENTED_ERROR,
-        "Built-in system \"%s\" is not provided by the GPU library; pass the particles explicitly",
-        system_name ? system_name : "(null)");
-    print_and_free_traceback(&st);
-    return GRAV_NOT_IMPLEMENTED_ERROR;
-}
 
 // Orbital elements -> Cartesian state (the formulae of src/utils.c:82-141).
 void keplerian_to_cartesian(double *x, double *v, const double a, const double e,

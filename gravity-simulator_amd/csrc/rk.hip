@@ -289,7 +289,8 @@ ErrorStatus seq_sum(Context *c, const double *terms, size_t count, double *dev_o
 }  // namespace
 
 ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                      SimulationStatus *status, Settings *settings, double tf, bool cromer)
+                      OutputParam *op, SimulationStatus *status, Settings *settings, double tf,
+                      bool cromer)
 {
     GS_TRY(check_args(system, ap));
     const int n = system->num_particles;
@@ -302,6 +303,10 @@ ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
     GS_TRY(S.alloc(c, n, 3 * n3, &rest));
     double *a = rest, *xe = a + n3, *ve = xe + n3;
     GS_TRY(S.upload(system));
+    SnapshotWriter snaps;
+    GS_TRY(snaps.begin(c, op, system));
+    if (snaps.on() && op->output_initial)
+        GS_TRY(snaps.write_host(system->x, system->v, status->t, status->dt));
     const double G = system->G;
     double dt = ip->dt;
     const long long total = static_cast<long long>(ceil(tf / dt));
@@ -315,8 +320,10 @@ ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
         GS_HIP(hipGetLastError());
         steps++;
         t = steps * dt;
+        if (snaps.due(t)) GS_TRY(snaps.write_device(S.x, S.v, t, dt));
         if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
     }
+    GS_TRY(snaps.finish());
     GS_TRY(S.download(system, S.x, S.v));
     status->t = t;
     status->dt = dt;
@@ -325,7 +332,7 @@ ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap
 }
 
 ErrorStatus rk4_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                    SimulationStatus *status, Settings *settings, double tf)
+                    OutputParam *op, SimulationStatus *status, Settings *settings, double tf)
 {
     GS_TRY(check_args(system, ap));
     const int n = system->num_particles;
@@ -339,6 +346,10 @@ ErrorStatus rk4_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
     double *x0 = rest, *v0 = x0 + n3, *xk = v0 + n3, *vk = xk + 4 * n3, *xe = vk + 4 * n3,
            *ve = xe + n3;
     GS_TRY(S.upload(system));
+    SnapshotWriter snaps;
+    GS_TRY(snaps.begin(c, op, system));
+    if (snaps.on() && op->output_initial)
+        GS_TRY(snaps.write_host(system->x, system->v, status->t, status->dt));
     const double G = system->G;
     const double dt = ip->dt;
     const long long total = static_cast<long long>(ceil(tf / dt));
@@ -361,8 +372,10 @@ ErrorStatus rk4_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
                            v0, xk, vk, xe, ve, dt, n3);
         GS_HIP(hipGetLastError());
         steps++;
+        if (snaps.due(steps * dt)) GS_TRY(snaps.write_device(S.x, S.v, steps * dt, dt));
         if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
     }
+    GS_TRY(snaps.finish());
     GS_TRY(S.download(system, S.x, S.v));
     status->t = steps * dt;
     status->dt = dt;
@@ -371,7 +384,8 @@ ErrorStatus rk4_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
 }
 
 ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                            SimulationStatus *status, Settings *settings, double tf)
+                            OutputParam *op, SimulationStatus *status, Settings *settings,
+                            double tf)
 {
     GS_TRY(check_args(system, ap));
     // The step control follows the embedded error estimate down to 1e-12 tf:
@@ -413,6 +427,8 @@ ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationPar
     GS_TRY(pinned(c, 64, &hp));
     double *hsum = static_cast<double *>(hp);
     GS_TRY(S.upload(system));
+    SnapshotWriter snaps;
+    GS_TRY(snaps.begin(c, op, system));
     const double G = system->G;
     double *x = S.x, *v = S.v;
     const dim3 grid = rk_grid(n3), block(RK_BLOCK);
@@ -453,6 +469,9 @@ ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationPar
         if (dt > tf) dt = tf;
     }
 
+    // initial snapshot after the initial step size (src/integrator_rk_embedded.c:689-718)
+    if (snaps.on() && op->output_initial)
+        GS_TRY(snaps.write_host(system->x, system->v, status->t, status->dt));
     double t = 0.0;
     long long steps = 0;
     RkRow row;
@@ -479,6 +498,7 @@ ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationPar
             std::swap(v, v1);
             std::swap(xe, txe);
             std::swap(ve, tve);
+            if (snaps.due(t)) GS_TRY(snaps.write_device(x, v, t, dt));
         }
         if (error < 1e-10) error = 1e-10;
         const double dt_new =
@@ -493,6 +513,7 @@ ErrorStatus rk_embedded_gpu(System *system, IntegratorParam *ip, AccelerationPar
         if (t < tf && t + dt > tf) dt = tf - t;
         if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
     }
+    GS_TRY(snaps.finish());
     GS_TRY(S.download(system, x, v));
     status->t = t;
     status->dt = dt;

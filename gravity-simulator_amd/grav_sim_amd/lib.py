@@ -44,6 +44,7 @@ EXPORTS = [
     "rk_embedded",
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
     "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
+    "initialize_built_in_system", "output_snapshot",
 ]
 
 
@@ -126,6 +127,10 @@ def load(path=None):
                         C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
         "rk_embedded": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
                              C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
+        "initialize_built_in_system": (ES, [pS, C.c_char_p, C.c_bool]),
+        "output_snapshot": (ES, [C.POINTER(_abi.OutputParam), pS, C.POINTER(_abi.IntegratorParam), pP,
+                                 C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings)]),
+        "finalize_output_param": (ES, [C.POINTER(_abi.OutputParam), C.POINTER(_abi.Settings)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

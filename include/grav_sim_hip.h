@@ -137,6 +137,12 @@ ErrorStatus get_initialized_system(System *GS_RESTRICT system, const int num_par
 ErrorStatus finalize_system(System *GS_RESTRICT system);
 /* replaces src/system.h (free_system; definition src/system.c:113-119) */
 void free_system(System *GS_RESTRICT system);
+/* replaces src/system.h:237-241 (src/system.c:579-1095): the named initial conditions circular_binary_orbit,
+ * eccentric_binary_orbit, 3d_helix, sun_earth_moon, figure-8, pyth-3-body,
+ * solar_system, solar_system_plus, bitwise equal to the reference's. */
+ErrorStatus initialize_built_in_system(System *GS_RESTRICT system,
+                                       const char *GS_RESTRICT system_name,
+                                       const bool is_memory_initialized);
 
 /* ------------------------------------------------------------------ */
 /* Integrator / output parameters   (src/common.h:29-56)               */
@@ -180,9 +186,10 @@ typedef struct SimulationStatus
 
 /* replaces src/output.h:34 (defaults src/output.c:85-98) */
 OutputParam get_new_output_param(void);
-/* replaces src/output.h:44-47 (src/output.c:126-240).  Snapshot writing is
- * not part of this library: only OUTPUT_METHOD_DISABLED is accepted by the
- * integrators; CSV/HDF5 return GRAV_NOT_IMPLEMENTED_ERROR there. */
+/* replaces src/output.h:44-47 (src/output.c:126-240): method check,
+ * positive interval, directory path ending in '/' (snapshots_YYYYMMDD_HHMMSS/
+ * when NULL), created here.  HDF5 is not built in: GRAV_MEMORY_ERROR, as in a
+ * reference build without HDF5. */
 ErrorStatus finalize_output_param(OutputParam *GS_RESTRICT output_param,
                                   const Settings *GS_RESTRICT settings);
 
@@ -293,8 +300,11 @@ ErrorStatus finalize_integration_param(IntegratorParam *GS_RESTRICT integration_
  * integrator runs device-resident on the GPU with the reference's FP64
  * operation order: EULER, EULER_CROMER, RK4 (src/integrator.c:276-892),
  * LEAPFROG (:894-1121), the embedded RKF45 / DOPRI / DVERK / RKF78 below,
- * IAS15 and WHFAST (src/integrator_whfast.c:200-407 semantics).  Output must
- * be OUTPUT_METHOD_DISABLED. */
+ * IAS15 and WHFAST (src/integrator_whfast.c:200-407 semantics).
+ * OUTPUT_METHOD_CSV writes the reference's snapshots (times, files, format)
+ * for every integrator but WHFAST, which refuses output
+ * (GRAV_NOT_IMPLEMENTED_ERROR); device -> host copies and CSV formatting run
+ * behind the step loop on writer threads. */
 ErrorStatus integrator_launch_simulation(System *GS_RESTRICT system,
                                          IntegratorParam *GS_RESTRICT integrator_param,
                                          AccelerationParam *GS_RESTRICT acceleration_param,
@@ -308,7 +318,7 @@ ErrorStatus integrator_launch_simulation(System *GS_RESTRICT system,
  * norm (summed in index order) and step-size control, every per-component
  * update an FP64 GPU kernel in the reference's operation order (bitwise equal
  * to the reference with the FP64 direct sums).  The FP32 force methods are
- * refused (GRAV_VALUE_ERROR).  Output must be OUTPUT_METHOD_DISABLED. */
+ * refused (GRAV_VALUE_ERROR).  CSV snapshots as integrator_launch_simulation. */
 ErrorStatus rk_embedded(System *system, IntegratorParam *integrator_param,
                         AccelerationParam *acceleration_param, OutputParam *output_param,
                         SimulationStatus *simulation_status, Settings *settings, const double tf);
@@ -317,17 +327,26 @@ ErrorStatus rk_embedded(System *system, IntegratorParam *integrator_param,
  * IAS15 with the reference's predictor-corrector, step-size control and
  * compensated sums, every per-component update an FP64 GPU kernel in the
  * reference's operation order and the forces from acceleration_param on the
- * device (bitwise equal to the reference with the FP64 direct sums).  Output
- * must be OUTPUT_METHOD_DISABLED. */
+ * device (bitwise equal to the reference with the FP64 direct sums).  CSV
+ * snapshots as integrator_launch_simulation. */
 ErrorStatus ias15(System *system, IntegratorParam *integrator_param,
                   AccelerationParam *acceleration_param, OutputParam *output_param,
                   SimulationStatus *simulation_status, Settings *settings, const double tf);
 
 /* replaces src/integrator.h:131-139 (src/integrator_whfast.c:200-407): the GPU
- * WHFast on one system (gs_whfast_ensemble batches many). */
+ * WHFast on one system (gs_whfast_ensemble batches many).  Output must be
+ * OUTPUT_METHOD_DISABLED. */
 ErrorStatus whfast(System *system, IntegratorParam *integrator_param,
                    AccelerationParam *acceleration_param, OutputParam *output_param,
                    SimulationStatus *simulation_status, Settings *settings, const double tf);
+
+/* replaces src/output.h:61-68 (src/output.c:241-293):
+ * one CSV snapshot of the host-side system, snapshot_%05d.csv numbered by
+ * output_param->output_count_, which it increments. */
+ErrorStatus output_snapshot(OutputParam *output_param, const System *system,
+                            const IntegratorParam *integrator_param,
+                            const AccelerationParam *acceleration_param,
+                            const SimulationStatus *simulation_status, const Settings *settings);
 
 /* replaces src/grav_sim.h:47-55 (src/grav_sim.c:61-138) */
 ErrorStatus launch_simulation(System *GS_RESTRICT system,
@@ -369,8 +388,8 @@ void compute_energy_python(double *GS_RESTRICT energy, const double G,
 void free_memory_int32(int32 *GS_RESTRICT ptr);
 void free_memory_double(double *GS_RESTRICT ptr);
 
-/* replaces src/python_interface.h:27-35.  Built-in initial conditions are not
- * provided: prints a traceback and returns GRAV_NOT_IMPLEMENTED_ERROR. */
+/* replaces src/python_interface.h:27-35 (src/python_interface.c:18-53): the
+ * built-in system in malloc'ed arrays the caller frees with free_memory_*. */
 int32 load_built_in_system_python(const char *system_name, int *num_particles_ptr,
                                   int32 **particle_ids_ptr, double **x_ptr, double **v_ptr,
                                   double **m_ptr, double *G_ptr);

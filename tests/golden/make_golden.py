@@ -38,6 +38,15 @@ exported functions on seeded inputs and stores inputs and outputs as data:
                     1e-9 to t=1 from the automatic initial step; DOPRI on the
                     Pythagorean three-body problem, tolerance 1e-12, t=20:
                     final x, v and step counts
+  f10_python_api.npz  the built-in systems (src/system.c:579-1095) as
+                    load_built_in_system_python returns them
+                    (src/python_interface.c:18-53), and the CSV snapshot
+                    files (src/output.c:351-469) that reference
+                    launch_simulation writes for N=8 Plummer (seed 10),
+                    eps=1e-2: Euler, Euler-Cromer, RK4 (float columns),
+                    Leapfrog (with and without the initial snapshot) at
+                    dt=1e-3 to t=0.0505 every 0.01; DOPRI, RKF78 and IAS15 at
+                    tolerance 1e-9 to t=0.5 every 0.1
 """
 import hashlib
 import os
@@ -167,8 +176,91 @@ def f9():
     np.savez_compressed(os.path.join(HERE, "f9_rk.npz"), **out)
 
 
+BUILT_IN = ["circular_binary_orbit", "eccentric_binary_orbit", "3d_helix", "sun_earth_moon", "figure-8",
+            "pyth-3-body", "solar_system", "solar_system_plus"]
+
+# (name, integrator, dt, tolerance, tf, interval, output_initial, float columns)
+CSV_CASES = [
+    ("euler", 1, 1e-3, -1.0, 0.0505, 0.01, True, False),
+    ("euler_cromer", 2, 1e-3, -1.0, 0.0505, 0.01, True, False),
+    ("rk4_float", 3, 1e-3, -1.0, 0.0505, 0.01, True, True),
+    ("leapfrog", 4, 1e-3, -1.0, 0.0505, 0.01, True, False),
+    ("leapfrog_noinit", 4, 1e-3, -1.0, 0.0505, 0.01, False, False),
+    ("dopri", 6, -1.0, 1e-9, 0.5, 0.1, True, False),
+    ("rkf78", 8, -1.0, 1e-9, 0.5, 0.1, True, False),
+    ("ias15", 9, -1.0, 1e-9, 0.5, 0.1, True, False),
+]
+
+
+def ref_builtin(name):
+    import ctypes as C
+    R = O.ref()
+    n = C.c_int()
+    ids = C.POINTER(C.c_int32)()
+    px, pv, pm = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    G = C.c_double()
+    rc = R.load_built_in_system_python(name.encode(), C.byref(n), C.byref(ids), C.byref(px), C.byref(pv),
+                                       C.byref(pm), C.byref(G))
+    assert rc == 0, name
+    k = n.value
+    out = (np.ctypeslib.as_array(ids, (k,)).copy(), np.ctypeslib.as_array(px, (k, 3)).copy(),
+           np.ctypeslib.as_array(pv, (k, 3)).copy(), np.ctypeslib.as_array(pm, (k,)).copy(), G.value)
+    R.free_memory_int32(ids)
+    for p in (px, pv, pm):
+        R.free_memory_double(p)
+    return out
+
+
+def ref_csv_run(x, v, m, G, integ, dt, tol, tf, interval, initial, floats, eps, out_dir):
+    """Reference launch_simulation with CSV output into out_dir; returns the file texts in name order."""
+    import ctypes as C
+    from grav_sim_amd import _abi
+    R = O.ref()
+    x, v, m = x.copy(), v.copy(), m.copy()
+    ids = np.arange(len(m), dtype=np.int32)
+    s = _abi.make_system(x, v, m, G, ids)
+    ip = R.get_new_integrator_param()
+    ip.integrator, ip.dt, ip.tolerance = integ, dt, tol
+    ap = R.get_new_acceleration_param()
+    ap.method, ap.softening_length = 1, eps
+    op = R.get_new_output_param()
+    op.method = 2
+    d = (out_dir.rstrip("/") + "/").encode()
+    op.output_dir = d
+    op.output_initial = initial
+    op.output_interval = interval
+    if floats:
+        op.coordinate_output_dtype = op.velocity_output_dtype = op.mass_output_dtype = 1
+    st_ = _abi.SimulationStatus()
+    se = R.get_new_settings()
+    se.verbose = 0
+    se.enable_progress_bar = False
+    st = R.launch_simulation(C.byref(s), C.byref(ip), C.byref(ap), C.byref(op), C.byref(st_), C.byref(se), tf)
+    assert st.return_code == 0
+    names = sorted(os.listdir(out_dir))
+    return names, [open(os.path.join(out_dir, f)).read() for f in names]
+
+
+def f10():
+    import tempfile
+    out = {}
+    for name in BUILT_IN:
+        ids, x, v, m, G = ref_builtin(name)
+        out.update({f"{name}_ids": ids, f"{name}_x": x, f"{name}_v": v, f"{name}_m": m, f"{name}_G": G})
+    x, v, m = ic.plummer(8, 10)
+    out.update(csv_x=x, csv_v=v, csv_m=m)
+    for case in CSV_CASES:
+        name, integ, dt, tol, tf, interval, initial, floats = case
+        with tempfile.TemporaryDirectory() as d:
+            names, texts = ref_csv_run(x, v, m, 1.0, integ, dt, tol, tf, interval, initial, floats, 1e-2, d)
+        out[f"csv_{name}_names"] = np.array(names)
+        out[f"csv_{name}_texts"] = np.array(texts)
+        print("csv", name, len(names), "files", flush=True)
+    np.savez_compressed(os.path.join(HERE, "f10_python_api.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f8", "f9"]
+    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f8", "f9", "f10"]
     for w in which:
-        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f8": f8, "f9": f9}[w]()
+        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f8": f8, "f9": f9, "f10": f10}[w]()
         print("wrote", w, flush=True)
