@@ -246,7 +246,48 @@ def run_integrators(L, ic):
     wall = time.perf_counter() - t0
     out["rk4_fp32_n131072"] = {"steps": int(r[4]), "ms_per_step": 1e3 * wall / int(r[4]),
                                "interactions_per_s": 4 * int(r[4]) * float(1 << 17) * ((1 << 17) - 1) / wall}
+    out["snapshots"] = run_snapshots(L, x, v, m)
     return out
+
+
+def run_snapshots(L, x, v, m):
+    """CSV snapshot output behind the step loop: Leapfrog with the FP32 direct
+    sum at N=2^17, 64 steps of dt=2^-10, with and without a snapshot every 4
+    steps (17 files of 131072 rows); and the time one synchronous snapshot of
+    the same state takes on the host (what a step loop that formats in line,
+    as the reference's does, pays per snapshot)."""
+    import ctypes as C
+    import shutil
+    import tempfile
+    from grav_sim_amd import _abi
+    dt, tf = 2.0 ** -10, 2.0 ** -4
+    t0 = time.perf_counter()
+    L.launch_simulation(x, v, m, 1.0, 4, dt, tf, method=16, eps=1e-3)
+    plain = time.perf_counter() - t0
+    d = tempfile.mkdtemp(prefix="gs_snap_")
+    try:
+        t0 = time.perf_counter()
+        L.launch_simulation(x, v, m, 1.0, 4, dt, tf, method=16, eps=1e-3, output_dir=d, output_interval=4 * dt)
+        with_out = time.perf_counter() - t0
+        files = len(os.listdir(d))
+        lib = L.load()
+        xs, vs, ms = (np.ascontiguousarray(a) for a in (x, v, m))
+        s = _abi.make_system(xs, vs, ms, 1.0)
+        op = lib.get_new_output_param()
+        op.method, op.output_dir, op.output_interval = 2, (d + "/").encode(), 1.0
+        op.output_count_ = 1000
+        t0 = time.perf_counter()
+        st = lib.output_snapshot(C.byref(op), C.byref(s), C.byref(lib.get_new_integrator_param()),
+                                 C.byref(L.param(1)), C.byref(_abi.SimulationStatus()),
+                                 C.byref(lib.get_new_settings()))
+        sync_one = time.perf_counter() - t0
+        L.check(st, "output_snapshot")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"workload": "Leapfrog, FP32 direct sum, Plummer N=2^17 (seed 2), 64 steps, a snapshot every 4 steps",
+            "files": files, "run_s_without_output": plain, "run_s_with_output": with_out,
+            "sync_snapshot_s": sync_one, "inline_formatting_s_est": files * sync_one,
+            "rows_per_s_sync": len(m) / sync_one}
 
 
 def run_whfast(L, ic, steps):
@@ -356,7 +397,10 @@ def main():
     if not args.no_integrators:
         out["integrators"] = run_integrators(L, ic)
         log("integrators: " + ", ".join(f"{k} {v['ms_per_step']:.3f} ms/step" for k, v in out["integrators"].items()
-                                        if isinstance(v, dict)))
+                                        if isinstance(v, dict) and "ms_per_step" in v))
+        sn = out["integrators"]["snapshots"]
+        log(f"snapshots: {sn['files']} files, run {sn['run_s_with_output']:.3f} s with output vs "
+            f"{sn['run_s_without_output']:.3f} s without; one synchronous snapshot {sn['sync_snapshot_s']:.3f} s")
     log(f"leapfrog N=1024: graph {out['leapfrog_n1024']['graph_ms_per_step']:.4f} ms/step, "
         f"eager {out['leapfrog_n1024']['eager_ms_per_step']:.4f} ms/step")
     if not args.no_whfast:

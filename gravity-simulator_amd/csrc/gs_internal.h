@@ -174,7 +174,8 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
 // fails at the step that asks for it), copied device -> pinned host on the
 // run's stream behind an event, and formatted by a pool of writer threads
 // while the run continues.  Particle ids and masses are taken once at
-// begin(): the runs that use this keep them fixed.
+// begin() for the runs that keep them fixed (write_device), or copied with
+// every snapshot (write_device_particles).
 class SnapshotWriter {
 public:
     SnapshotWriter() = default;
@@ -190,6 +191,10 @@ public:
     ErrorStatus write_host(const double *x, const double *v, double t, double dt);
     // state in device memory, copied on c->stream behind the work queued so far
     ErrorStatus write_device(const double *x, const double *v, double t, double dt);
+    // the same for a run that reorders and removes particles: masses, ids and
+    // the particle count (*count <= the count at begin()) come from the device
+    ErrorStatus write_device_particles(const double *x, const double *v, const double *m,
+                                       const int32 *ids, const int *count, double t, double dt);
     // device buffer of 3 n doubles for a derived velocity (allocated on first use)
     ErrorStatus scratch(double **out);
     // waits for every queued snapshot; the first writer error, if any

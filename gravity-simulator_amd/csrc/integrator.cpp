@@ -11,7 +11,7 @@
 
 namespace gs {
 ErrorStatus whfast_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                       SimulationStatus *status, Settings *settings, double tf);
+                       OutputParam *op, SimulationStatus *status, Settings *settings, double tf);
 ErrorStatus ias15_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
                       OutputParam *op, SimulationStatus *status, Settings *settings, double tf);
 ErrorStatus euler_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
@@ -111,10 +111,7 @@ ErrorStatus whfast(System *system, IntegratorParam *ip, AccelerationParam *ap, O
 {
     if (!system || !ip || !ap || !op || !status || !settings)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
-    if (op->method != OUTPUT_METHOD_DISABLED)
-        return WRAP_RAISE_ERROR(GRAV_NOT_IMPLEMENTED_ERROR,
-                                "Snapshot output is not provided for the GPU WHFast");
-    return WRAP_TRACEBACK(whfast_gpu(system, ip, ap, status, settings, tf));
+    return WRAP_TRACEBACK(whfast_gpu(system, ip, ap, op, status, settings, tf));
 }
 
 ErrorStatus integrator_launch_simulation(System *system, IntegratorParam *ip,

@@ -112,7 +112,9 @@ struct SnapshotWriter::Impl {
     std::string fmt;
 
     struct Buf {
-        double *h = nullptr;  // x then v, 3 n doubles each (pinned)
+        // pinned: x and v (3 n doubles each), then m (n doubles), ids (n
+        // ints) and the count, the last three used by write_device_particles
+        double *h = nullptr;
         hipEvent_t ev = nullptr;
     };
     struct Job {
@@ -121,7 +123,11 @@ struct SnapshotWriter::Impl {
         std::string path;
         double t, dt;
         bool device;
+        bool particles;  // masses, ids and count are in the buffer
     };
+    double *buf_m(int b) const { return bufs[b].h + 2 * n3; }
+    int32 *buf_ids(int b) const { return reinterpret_cast<int32 *>(bufs[b].h + 2 * n3 + n); }
+    int *buf_count(int b) const { return reinterpret_cast<int *>(buf_ids(b) + n); }
     std::vector<Buf> bufs;
     std::vector<int> free_bufs;
     std::deque<Job> jobs;
@@ -152,8 +158,16 @@ struct SnapshotWriter::Impl {
             }
             if (err.empty()) {
                 const double *x = bufs[j.buf].h, *v = x + n3;
-                const bool ok =
-                    format_snapshot(j.f, fmt, n, G, j.t, j.dt, ids.data(), m.data(), x, v);
+                int k = n;
+                const int32 *pid = ids.data();
+                const double *pm = m.data();
+                if (j.particles) {
+                    k = *buf_count(j.buf);
+                    pid = buf_ids(j.buf);
+                    pm = buf_m(j.buf);
+                }
+                bool ok = k >= 0 && k <= n;
+                ok = ok && format_snapshot(j.f, fmt, k, G, j.t, j.dt, pid, pm, x, v);
                 if (!ok) err = "failed to write snapshot file \"" + j.path + "\"";
             }
             if (fclose(j.f) != 0 && err.empty())
@@ -191,11 +205,12 @@ struct SnapshotWriter::Impl {
         return make_success_error_status();
     }
 
-    void submit(int buf, FILE *f, std::string &&path, double t, double dt, bool device)
+    void submit(int buf, FILE *f, std::string &&path, double t, double dt, bool device,
+                bool particles = false)
     {
         {
             std::lock_guard<std::mutex> lk(mu);
-            jobs.push_back(Job{buf, f, std::move(path), t, dt, device});
+            jobs.push_back(Job{buf, f, std::move(path), t, dt, device, particles});
         }
         cv_job.notify_one();
         // src/output.c:331; the step loops then move the next output time
@@ -259,7 +274,7 @@ ErrorStatus SnapshotWriter::begin(Context *c, OutputParam *op, const System *sys
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const int nthreads = static_cast<int>(std::min(4u, std::max(1u, hw / 2)));
     // Pinned buffers: one per thread plus two being filled, within 1 GiB.
-    const size_t bytes = sizeof(double) * 2 * p->n3;
+    const size_t bytes = sizeof(double) * (2 * p->n3 + p->n) + sizeof(int32) * p->n + sizeof(int);
     const size_t by_mem = std::max<size_t>(2, (size_t{1} << 30) / std::max<size_t>(bytes, 1));
     const int nbufs = static_cast<int>(std::min<size_t>(nthreads + 2, by_mem));
     p->bufs.resize(nbufs);
@@ -311,6 +326,39 @@ ErrorStatus SnapshotWriter::write_device(const double *x, const double *v, doubl
         GS_HIP(e);
     }
     p->submit(b, f, std::move(path), t, dt, true);
+    return make_success_error_status();
+}
+
+ErrorStatus SnapshotWriter::write_device_particles(const double *x, const double *v,
+                                                   const double *m, const int32 *ids,
+                                                   const int *count, double t, double dt)
+{
+    if (!impl_) return make_success_error_status();
+    Impl *p = impl_;
+    FILE *f;
+    std::string path;
+    int b;
+    GS_TRY(p->open_next(&f, &path, &b));
+    const size_t b3 = sizeof(double) * p->n3;
+    hipStream_t st = p->c->stream;
+    hipError_t e = hipMemcpyAsync(p->bufs[b].h, x, b3, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->bufs[b].h + p->n3, v, b3, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->buf_m(b), m, sizeof(double) * p->n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->buf_ids(b), ids, sizeof(int32) * p->n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->buf_count(b), count, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(p->bufs[b].ev, st);
+    if (e != hipSuccess) {
+        fclose(f);
+        {
+            std::lock_guard<std::mutex> lk(p->mu);
+            p->free_bufs.push_back(b);
+        }
+        GS_HIP(e);
+    }
+    p->submit(b, f, std::move(path), t, dt, true, true);
     return make_success_error_status();
 }
 

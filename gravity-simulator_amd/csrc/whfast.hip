@@ -47,6 +47,7 @@ struct WfArgs {
     int *status;      // per system: bit 0 = Stumpff argument not finite (no removal)
     double G, eps, dt, dt_last;
     long long step0, nsteps, total_steps;
+    int snapshot_last;  // the launch's last step is a snapshot step
     int method;       // 1 pairwise form, 2 massless form
     int remove_invalid;
 };
@@ -606,7 +607,8 @@ __device__ void c2j_component(int n, int d, const double *q, double *jq, const d
 __device__ __forceinline__ void accel_kick_v(const WfShared &S, int i, bool massive, int p,
                                              const double *xi, const double *x0,
                                              const double *jxi, int ginside, double m0, double G,
-                                             double eps3, double *jv, double dt)
+                                             double eps3, double *jv, double dt,
+                                             double *jv_sync = nullptr)
 {
     double a[3];
     const double tv0 = xi[0] - x0[0], tv1 = xi[1] - x0[1], tv2 = xi[2] - x0[2];
@@ -672,17 +674,25 @@ __device__ __forceinline__ void accel_kick_v(const WfShared &S, int i, bool mass
     jv[3 * i] += a[0] * dt;
     jv[3 * i + 1] += a[1] * dt;
     jv[3 * i + 2] += a[2] * dt;
+    if (jv_sync) {
+        // the kicked velocity taken back half a step for a snapshot
+        // (src/integrator_whfast.c:346-351)
+        const double hdt = -0.5 * dt;
+        jv_sync[3 * i] = jv[3 * i] + a[0] * hdt;
+        jv_sync[3 * i + 1] = jv[3 * i + 1] + a[1] * hdt;
+        jv_sync[3 * i + 2] = jv[3 * i + 2] + a[2] * hdt;
+    }
 }
 
 // The same with the positions read from global memory.
 __device__ void accel_kick(const WfShared &S, int i, bool massive, int p, const double *x,
                            const double *jx, const double *m, double G, double eps3,
-                           double *jv, double dt)
+                           double *jv, double dt, double *jv_sync)
 {
     int g = 0;  // massive particles inside i
     if (!massive)
         while (g < S.nm && S.mlist[g] < i) g++;
-    accel_kick_v(S, i, massive, p, &x[3 * i], x, &jx[3 * i], g, m[0], G, eps3, jv, dt);
+    accel_kick_v(S, i, massive, p, &x[3 * i], x, &jx[3 * i], g, m[0], G, eps3, jv, dt, jv_sync);
 }
 
 __device__ void load_massive_positions(WfShared &S, const double *x)
@@ -765,18 +775,19 @@ __device__ void straddle_sums(WfShared &S, double G, double eps3)
     __syncthreads();
 }
 
+// jv_sync != NULL: also the half-step-back velocities of a snapshot.
 __device__ void accel_kick_all(WfShared &S, int n, const double *x, const double *jx,
                                const double *m, double G, double eps3, double *jv, double dt,
-                               int method)
+                               int method, double *jv_sync = nullptr)
 {
     load_massive_positions(S, x);
     straddle_sums(S, G, eps3);
     // massive particles (ordinal >= 1), one per thread
     for (int p = 1 + static_cast<int>(threadIdx.x); p < S.nm; p += WF_BLOCK)
-        accel_kick(S, S.mlist[p], true, p, x, jx, m, G, eps3, jv, dt);
+        accel_kick(S, S.mlist[p], true, p, x, jx, m, G, eps3, jv, dt, jv_sync);
     if (method == 2) {
         for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK)
-            if (m[i] == 0.0) accel_kick(S, i, false, -1, x, jx, m, G, eps3, jv, dt);
+            if (m[i] == 0.0) accel_kick(S, i, false, -1, x, jx, m, G, eps3, jv, dt, jv_sync);
     }
     __syncthreads();
 }
@@ -788,7 +799,7 @@ __device__ void accel_kick_all(WfShared &S, int n, const double *x, const double
 // for the serial recurrence.
 __device__ void kick_from_quotients(WfShared &S, const WfScratch &X, int n, double *x,
                                     const double *jx, double G, double eps3, double *jv, double dt,
-                                    int method, bool write_x)
+                                    int method, bool write_x, double *jv_sync)
 {
     const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
     for (int q = threadIdx.x; q < nm; q += WF_BLOCK) {
@@ -804,7 +815,8 @@ __device__ void kick_from_quotients(WfShared &S, const WfScratch &X, int n, doub
     const double m0 = S.mm[0];
     for (int p = 1 + static_cast<int>(threadIdx.x); p < nm; p += WF_BLOCK) {
         const int i = S.mlist[p];
-        accel_kick_v(S, i, true, p, S.mx[p], S.mx[0], &jx[3 * i], 0, m0, G, eps3, jv, dt);
+        accel_kick_v(S, i, true, p, S.mx[p], S.mx[0], &jx[3 * i], 0, m0, G, eps3, jv, dt,
+                     jv_sync);
     }
     if (method == 2) {
         for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
@@ -818,7 +830,8 @@ __device__ void kick_from_quotients(WfShared &S, const WfScratch &X, int n, doub
                 x[3 * i + 1] = xi[1];
                 x[3 * i + 2] = xi[2];
             }
-            accel_kick_v(S, i, false, -1, xi, S.mx[0], jxi, g + 1, m0, G, eps3, jv, dt);
+            accel_kick_v(S, i, false, -1, xi, S.mx[0], jxi, g + 1, m0, G, eps3, jv, dt,
+                         jv_sync);
         }
     }
     __syncthreads();
@@ -946,15 +959,28 @@ __global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         // caller: rebuild them (from the drifted, not yet kicked, Jacobi
         // velocities, as the reference's last jacobi_to_cartesian does) on
         // the last step of each launch
-        if (last) j2c_block(n, jv, A.v + 3 * o, m, S, X);
+        // A launch that ends on a snapshot step instead returns the
+        // velocities the snapshot holds: the kicked Jacobi velocities taken
+        // back half a step (in jv2, free until the next sort), converted
+        // after the kick.  Either way A.v is what the reference leaves in
+        // system->v at that point.
+        const bool snap = last && A.snapshot_last;
+        double *jv_sync = snap ? jv2 : nullptr;
+        if (last && !snap) j2c_block(n, jv, A.v + 3 * o, m, S, X);
         // (pos_bad covers every drifted particle: a superset of the runs,
         // and compaction only removes particles)
         j2c_quotients(n, jx, S, X, pos_bad);
         if (X.j.fallback[0] | X.j.fallback[1] | X.j.fallback[2]) {
             j2c_fill(n, jx, x, m, S, X);
-            accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, dt, A.method);
+            accel_kick_all(S, n, x, jx, m, A.G, eps3, jv, dt, A.method, jv_sync);
         } else {
-            kick_from_quotients(S, X, n, x, jx, A.G, eps3, jv, dt, A.method, last);
+            kick_from_quotients(S, X, n, x, jx, A.G, eps3, jv, dt, A.method, last, jv_sync);
+        }
+        if (snap) {
+            // particle 0 gets no acceleration: jv + 0 * (-dt / 2) == jv
+            if (threadIdx.x < 3) jv2[threadIdx.x] = jv[threadIdx.x];
+            __syncthreads();
+            j2c_block(n, jv2, A.v + 3 * o, m, S, X);
         }
     }
     if (threadIdx.x == 0) A.count[blockIdx.x] = n;
@@ -964,8 +990,11 @@ __global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
 ErrorStatus whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, double *x, double *v,
                             double *m, double G, int method, double eps, double dt, double tf,
                             bool remove_invalid, int *out_counts, int64 *steps_done,
-                            const bool *is_exit)
+                            const bool *is_exit, SnapshotWriter *snaps,
+                            const SimulationStatus *status0, const OutputParam *op)
 {
+    if (snaps && snaps->on() && num_systems != 1)
+        return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "snapshots are written for one system only");
     if (num_systems <= 0 || !offsets || !ids || !x || !v || !m || !out_counts)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument or no systems");
     if (method != ACCELERATION_METHOD_PAIRWISE && method != ACCELERATION_METHOD_MASSLESS)
@@ -1066,24 +1095,53 @@ ErrorStatus whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, d
     }
     A.step0 = 0;
     A.nsteps = 0;
+    A.snapshot_last = 0;
     if (!e) {
         const int rec = timing_begin(c, "whfast_init");
         hipLaunchKernelGGL(k_whfast_init, dim3(num_systems), dim3(WF_BLOCK), 0, c->stream, A);
         timing_end(c, rec);
         e = hipGetLastError();
     }
+    const bool out = snaps && snaps->on();
+    // initial snapshot of the distance-sorted state (src/integrator_whfast.c:240-262)
+    if (!e && out && op->output_initial)
+        st = snaps->write_device_particles(A.x, A.v, A.m, A.ids, A.count, status0->t,
+                                           status0->dt);
     const long long chunk = 256;
     long long done = 0;
-    while (!e && done < total) {
+    while (!e && st.return_code == GRAV_SUCCESS && done < total) {
         A.step0 = done;
         A.nsteps = std::min(chunk, total - done);
+        A.snapshot_last = 0;
+        // a launch ends at the first step whose time t = steps dt reaches the
+        // next output time (src/integrator_whfast.c:343-366)
+        double t_end = 0.0, dt_end = dt;
+        if (out)
+            for (long long j = 1; j <= A.nsteps; j++) {
+                const long long k = done + j;
+                const double dtk = k == total ? A.dt_last : dt;
+                if (snaps->due(static_cast<double>(k) * dtk)) {
+                    A.nsteps = j;
+                    A.snapshot_last = 1;
+                    t_end = static_cast<double>(k) * dtk;
+                    dt_end = dtk;
+                    break;
+                }
+            }
         const int rec = timing_begin(c, "whfast_steps");
         hipLaunchKernelGGL(k_whfast_steps, dim3(num_systems), dim3(WF_BLOCK), 0, c->stream, A);
         timing_end(c, rec);
         e = hipGetLastError();
         done += A.nsteps;
+        if (!e && A.snapshot_last)
+            st = snaps->write_device_particles(A.x, A.v, A.m, A.ids, A.count, t_end, dt_end);
         if (!e) e = hipStreamSynchronize(c->stream);
         if (is_exit && *is_exit) break;
+    }
+    if (st.return_code != GRAV_SUCCESS) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(buf);
+        return st;
     }
     std::vector<int> stat(num_systems);
     if (!e) e = hipMemcpyAsync(cnt.data(), A.count, sizeof(int) * num_systems, hipMemcpyDeviceToHost, c->stream);
@@ -1101,15 +1159,20 @@ ErrorStatus whfast_ensemble(int num_systems, const int64 *offsets, int32 *ids, d
 }
 
 ErrorStatus whfast_gpu(System *system, IntegratorParam *ip, AccelerationParam *ap,
-                       SimulationStatus *status, Settings *settings, double tf)
+                       OutputParam *op, SimulationStatus *status, Settings *settings, double tf)
 {
     const int64 off[2] = {0, system->num_particles};
     int count = 0;
     int64 steps = 0;
+    Context *c;
+    GS_TRY(context(&c));
+    SnapshotWriter snaps;
+    GS_TRY(snaps.begin(c, op, system));
     GS_TRY(whfast_ensemble(1, off, system->particle_ids, system->x, system->v, system->m,
                            system->G, ap->method, ap->softening_length, ip->dt, tf,
                            ip->whfast_remove_invalid_particles, &count, &steps,
-                           settings->is_exit_ptr));
+                           settings->is_exit_ptr, &snaps, status, op));
+    GS_TRY(snaps.finish());
     system->num_particles = count;
     status->num_steps = steps;
     const long long total = static_cast<long long>(ceil(tf / ip->dt));
@@ -1131,5 +1194,6 @@ extern "C" ErrorStatus gs_whfast_ensemble(int num_systems, const int64 *offsets,
                                           int *out_counts)
 {
     return gs::whfast_ensemble(num_systems, offsets, ids, x, v, m, G, method, eps, dt, tf,
-                               remove_invalid, out_counts, nullptr, nullptr);
+                               remove_invalid, out_counts, nullptr, nullptr, nullptr, nullptr,
+                               nullptr);
 }

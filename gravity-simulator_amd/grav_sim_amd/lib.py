@@ -325,8 +325,10 @@ class Sim:
 
 
 def launch_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, theta=1.0, leaf=-1,
-                      remove_invalid=True, tolerance=-1.0, initial_dt=-1.0):
-    """launch_simulation with output disabled; returns (ids, x, v, m, steps)."""
+                      remove_invalid=True, tolerance=-1.0, initial_dt=-1.0, output_dir=None,
+                      output_interval=1.0, output_initial=True):
+    """launch_simulation; returns (ids, x, v, m, steps).  With `output_dir`,
+    CSV snapshots every `output_interval` are written there."""
     L = load()
     x, v, m = _c(x).copy(), _c(v).copy(), _c(m).copy()
     ids = np.arange(len(m), dtype=np.int32)
@@ -339,6 +341,12 @@ def launch_simulation(x, v, m, G, integrator, dt, tf, method=1, eps=0.0, theta=1
     ip.initial_dt = initial_dt
     ap = param(method, theta, eps, leaf)
     op = L.get_new_output_param()
+    if output_dir is not None:
+        d = (str(output_dir).rstrip("/") + "/").encode()
+        op.method = 2
+        op.output_dir = d
+        op.output_interval = output_interval
+        op.output_initial = output_initial
     st = _abi.SimulationStatus()
     se = L.get_new_settings()
     se.verbose = 0

@@ -301,10 +301,10 @@ ErrorStatus finalize_integration_param(IntegratorParam *GS_RESTRICT integration_
  * operation order: EULER, EULER_CROMER, RK4 (src/integrator.c:276-892),
  * LEAPFROG (:894-1121), the embedded RKF45 / DOPRI / DVERK / RKF78 below,
  * IAS15 and WHFAST (src/integrator_whfast.c:200-407 semantics).
- * OUTPUT_METHOD_CSV writes the reference's snapshots (times, files, format)
- * for every integrator but WHFAST, which refuses output
- * (GRAV_NOT_IMPLEMENTED_ERROR); device -> host copies and CSV formatting run
- * behind the step loop on writer threads. */
+ * OUTPUT_METHOD_CSV writes the reference's snapshots (times, files, format,
+ * and the state they leave in the system) for every integrator; device ->
+ * host copies and CSV formatting run behind the step loop on writer
+ * threads. */
 ErrorStatus integrator_launch_simulation(System *GS_RESTRICT system,
                                          IntegratorParam *GS_RESTRICT integrator_param,
                                          AccelerationParam *GS_RESTRICT acceleration_param,
@@ -334,8 +334,8 @@ ErrorStatus ias15(System *system, IntegratorParam *integrator_param,
                   SimulationStatus *simulation_status, Settings *settings, const double tf);
 
 /* replaces src/integrator.h:131-139 (src/integrator_whfast.c:200-407): the GPU
- * WHFast on one system (gs_whfast_ensemble batches many).  Output must be
- * OUTPUT_METHOD_DISABLED. */
+ * WHFast on one system (gs_whfast_ensemble batches many).  CSV snapshots as
+ * integrator_launch_simulation, in the reference's distance order. */
 ErrorStatus whfast(System *system, IntegratorParam *integrator_param,
                    AccelerationParam *acceleration_param, OutputParam *output_param,
                    SimulationStatus *simulation_status, Settings *settings, const double tf);

@@ -46,7 +46,10 @@ exported functions on seeded inputs and stores inputs and outputs as data:
                     eps=1e-2: Euler, Euler-Cromer, RK4 (float columns),
                     Leapfrog (with and without the initial snapshot) at
                     dt=1e-3 to t=0.0505 every 0.01; DOPRI, RKF78 and IAS15 at
-                    tolerance 1e-9 to t=0.5 every 0.1
+                    tolerance 1e-9 to t=0.5 every 0.1; WHFast (massless form)
+                    on the largest of 8 planetary systems (ensemble seed 23), dt=0.01 to
+                    t=1 every 0.1 and to t=0.5 every 0.25, with the final
+                    state
 """
 import hashlib
 import os
@@ -211,8 +214,10 @@ def ref_builtin(name):
     return out
 
 
-def ref_csv_run(x, v, m, G, integ, dt, tol, tf, interval, initial, floats, eps, out_dir):
-    """Reference launch_simulation with CSV output into out_dir; returns the file texts in name order."""
+def ref_csv_run(x, v, m, G, integ, dt, tol, tf, interval, initial, floats, eps, out_dir, method=1,
+                final=False):
+    """Reference launch_simulation with CSV output into out_dir; returns the file
+    names and texts in name order (and the final ids, x, v when `final`)."""
     import ctypes as C
     from grav_sim_amd import _abi
     R = O.ref()
@@ -222,7 +227,7 @@ def ref_csv_run(x, v, m, G, integ, dt, tol, tf, interval, initial, floats, eps, 
     ip = R.get_new_integrator_param()
     ip.integrator, ip.dt, ip.tolerance = integ, dt, tol
     ap = R.get_new_acceleration_param()
-    ap.method, ap.softening_length = 1, eps
+    ap.method, ap.softening_length = method, eps
     op = R.get_new_output_param()
     op.method = 2
     d = (out_dir.rstrip("/") + "/").encode()
@@ -238,7 +243,11 @@ def ref_csv_run(x, v, m, G, integ, dt, tol, tf, interval, initial, floats, eps, 
     st = R.launch_simulation(C.byref(s), C.byref(ip), C.byref(ap), C.byref(op), C.byref(st_), C.byref(se), tf)
     assert st.return_code == 0
     names = sorted(os.listdir(out_dir))
-    return names, [open(os.path.join(out_dir, f)).read() for f in names]
+    texts = [open(os.path.join(out_dir, f)).read() for f in names]
+    if final:
+        k = s.num_particles
+        return names, texts, ids[:k].copy(), x[:k].copy(), v[:k].copy()
+    return names, texts
 
 
 def f10():
@@ -255,6 +264,21 @@ def f10():
             names, texts = ref_csv_run(x, v, m, 1.0, integ, dt, tol, tf, interval, initial, floats, 1e-2, d)
         out[f"csv_{name}_names"] = np.array(names)
         out[f"csv_{name}_texts"] = np.array(texts)
+        print("csv", name, len(names), "files", flush=True)
+    # WHFast, massless form, one planetary system: snapshots in distance order
+    # and the final state (whose velocities are the snapshot's when the last
+    # step writes one)
+    off, X, V, M = ic.ensemble(8, 23, max_test=40)
+    s0 = int(np.argmax(np.diff(off)))
+    sl = slice(off[s0], off[s0 + 1])
+    out.update(wh_x=X[sl], wh_v=V[sl], wh_m=M[sl])
+    for name, tf, interval in (("whfast", 1.0, 0.1), ("whfast_last", 0.5, 0.25)):
+        with tempfile.TemporaryDirectory() as d:
+            names, texts, fid, fx, fv = ref_csv_run(X[sl], V[sl], M[sl], 1.0, 10, 0.01, -1.0, tf, interval, True,
+                                                    False, 0.0, d, method=2, final=True)
+        out[f"csv_{name}_names"] = np.array(names)
+        out[f"csv_{name}_texts"] = np.array(texts)
+        out.update({f"{name}_ids_final": fid, f"{name}_x_final": fx, f"{name}_v_final": fv})
         print("csv", name, len(names), "files", flush=True)
     np.savez_compressed(os.path.join(HERE, "f10_python_api.npz"), **out)
 
