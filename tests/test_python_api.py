@@ -179,3 +179,33 @@ def test_read_csv_data_and_momenta(f10, tmp_path):
     assert np.allclose(am, (m * np.cross(x, v).sum(-1)).sum(-1), atol=1e-15)
     gs.delete_snapshots(tmp_path)
     assert not list(tmp_path.glob("snapshot_*.csv"))
+
+
+@pytest.mark.parametrize("floats", [False, True])
+def test_snapshot_rows_print_like_printf_g(tmp_path, floats):
+    """Every column of a snapshot row is the %.17g (%.8g) text of its value:
+    random bit patterns (subnormals, signed zeros, extreme exponents) checked
+    against Python's own correctly rounded %-formatting."""
+    rng = np.random.default_rng(5)
+    n = 4000
+    bits = rng.integers(0, 2 ** 63 - 1, size=7 * n, dtype=np.int64) * rng.choice([-1, 1], 7 * n)
+    vals = bits.view(np.float64)
+    vals = np.where(np.isfinite(vals), vals, 0.5)
+    vals[:6] = [0.0, -0.0, 5e-324, -2.2250738585072014e-308, 1.7976931348623157e308, 0.1]
+    m, x, v = np.ascontiguousarray(vals[:n]), vals[n:4 * n].reshape(n, 3).copy(), vals[4 * n:].reshape(n, 3).copy()
+    ids = rng.integers(-2 ** 31, 2 ** 31 - 1, size=n, dtype=np.int32)
+    lib = L.load()
+    s = _abi.make_system(x, v, m, 1.0, ids)
+    op = _output_param(lib, (str(tmp_path) + "/").encode(), floats)
+    status = _abi.SimulationStatus(0.25, 1e-3, 0)
+    st = lib.output_snapshot(C.byref(op), C.byref(s), C.byref(lib.get_new_integrator_param()),
+                             C.byref(lib.get_new_acceleration_param()), C.byref(status),
+                             C.byref(lib.get_new_settings()))
+    assert st.return_code == 0
+    rows = (tmp_path / "snapshot_00000.csv").read_text().splitlines()
+    assert rows[:5] == [f"# num_particles: {n}", "# G: 1", "# time: 0.25", "# dt: 0.001",
+                        "particle_id,m,x,y,z,vx,vy,vz"]
+    fmt = "%.8g" if floats else "%.17g"
+    for i, r in enumerate(rows[5:]):
+        want = [str(ids[i])] + [fmt % val for val in (m[i], *x[i], *v[i])]
+        assert r.split(",") == want, i
