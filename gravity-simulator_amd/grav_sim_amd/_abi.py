@@ -102,6 +102,26 @@ class Settings(C.Structure):
     _fields_ = [("verbose", C.c_int), ("enable_progress_bar", C.c_bool), ("is_exit_ptr", C.POINTER(C.c_bool))]
 
 
+class CosmologicalSystem(C.Structure):
+    # src/system.h:22-38, 128 bytes
+    _fields_ = [
+        ("num_particles", C.c_int),
+        ("particle_ids", PI),
+        ("x", PD),
+        ("v", PD),
+        ("m", PD),
+        ("h", C.c_double),
+        ("omega_m", C.c_double),
+        ("omega_lambda", C.c_double),
+        ("omega_k", C.c_double),
+        ("box_center", C.c_double * 3),
+        ("box_width", C.c_double),
+        ("unit_mass", C.c_double),
+        ("unit_length", C.c_double),
+        ("unit_time", C.c_double),
+    ]
+
+
 class LinearOctree(C.Structure):
     _fields_ = [
         ("box_width", C.c_double),

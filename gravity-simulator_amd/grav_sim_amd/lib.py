@@ -45,6 +45,9 @@ EXPORTS = [
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
     "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
     "initialize_built_in_system", "output_snapshot",
+    "compute_omega_k", "compute_da", "get_new_cosmological_system", "get_initialized_cosmological_system",
+    "finalize_cosmological_system", "free_cosmological_system", "set_periodic_boundary_conditions",
+    "acceleration_PM", "leapfrog_cosmology", "output_snapshot_cosmology", "launch_cosmological_simulation",
 ]
 
 
@@ -131,6 +134,19 @@ def load(path=None):
         "output_snapshot": (ES, [C.POINTER(_abi.OutputParam), pS, C.POINTER(_abi.IntegratorParam), pP,
                                  C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings)]),
         "finalize_output_param": (ES, [C.POINTER(_abi.OutputParam), C.POINTER(_abi.Settings)]),
+        "compute_omega_k": (C.c_double, [C.c_double, C.c_double]),
+        "compute_da": (C.c_double, [C.c_double] * 4),
+        "get_new_cosmological_system": (_abi.CosmologicalSystem, []),
+        "finalize_cosmological_system": (ES, [C.POINTER(_abi.CosmologicalSystem)]),
+        "set_periodic_boundary_conditions": (None, [C.POINTER(_abi.CosmologicalSystem)]),
+        "acceleration_PM": (ES, [PD, C.c_int, PD, PD, PD, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_int, C.c_double]),
+        "leapfrog_cosmology": (ES, [C.POINTER(_abi.CosmologicalSystem), C.POINTER(_abi.OutputParam),
+                                    C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double,
+                                    C.c_double, C.c_double, C.c_int]),
+        "launch_cosmological_simulation": (ES, [C.POINTER(_abi.CosmologicalSystem), C.POINTER(_abi.OutputParam),
+                                                C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings),
+                                                C.c_double, C.c_double, C.c_double, C.c_int]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -463,3 +479,51 @@ def learned_training_report():
     v = [C.c_double(0.0) for _ in range(5)]
     check(load().gs_learned_training_report(*[C.byref(t) for t in v]), "gs_learned_training_report")
     return dict(zip(("alpha", "p99_net", "p99_without", "median_net", "median_without"), [t.value for t in v]))
+
+
+# ---------------------------------------------------------------- particle mesh / cosmology
+def acceleration_pm(x, m, box_center, box_width, H0, omega_m, mean_bkg_density, grid, scale_factor):
+    """acceleration_PM on host arrays (+grad phi, as the reference returns it)."""
+    L = load()
+    x, m = _c(x), _c(m)
+    a = np.zeros_like(x)
+    c = _c(box_center)
+    check(L.acceleration_PM(_abi.dptr(a), len(m), _abi.dptr(x), _abi.dptr(m), _abi.dptr(c), float(box_width),
+                            float(H0), float(omega_m), float(mean_bkg_density), int(grid), float(scale_factor)),
+          "acceleration_PM")
+    return a
+
+
+def cosmological_system(x, v, m, box_center, box_width, h, omega_m, omega_lambda, ids=None):
+    """A CosmologicalSystem over numpy arrays (kept alive by the caller)."""
+    s = _abi.CosmologicalSystem()
+    if ids is None:
+        ids = np.arange(len(m), dtype=np.int32)
+    s.num_particles = len(m)
+    s.particle_ids = _abi.iptr(ids)
+    s.x, s.v, s.m = _abi.dptr(x), _abi.dptr(v), _abi.dptr(m)
+    s.h, s.omega_m, s.omega_lambda = h, omega_m, omega_lambda
+    s.omega_k = 1.0 - omega_m - omega_lambda
+    for d in range(3):
+        s.box_center[d] = box_center[d]
+    s.box_width = box_width
+    s.unit_mass = s.unit_length = s.unit_time = 1.0
+    return s, ids
+
+
+def leapfrog_cosmology(x, v, m, box_center, box_width, h, omega_m, omega_lambda, dt, a_begin, a_final, grid,
+                       launch=False):
+    """The cosmological Leapfrog (output disabled); returns (x, num_steps, t).
+    launch=True goes through launch_cosmological_simulation's checks."""
+    L = load()
+    x, v, m = _c(x).copy(), _c(v).copy(), _c(m).copy()
+    s, ids = cosmological_system(x, v, m, box_center, box_width, h, omega_m, omega_lambda)
+    op = L.get_new_output_param()
+    st = _abi.SimulationStatus()
+    se = L.get_new_settings()
+    se.verbose = 0
+    se.enable_progress_bar = False
+    f = L.launch_cosmological_simulation if launch else L.leapfrog_cosmology
+    check(f(C.byref(s), C.byref(op), C.byref(st), C.byref(se), float(dt), float(a_begin), float(a_final),
+            int(grid)), "leapfrog_cosmology")
+    return x, st.num_steps, st.t

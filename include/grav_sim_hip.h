@@ -137,6 +137,39 @@ ErrorStatus get_initialized_system(System *GS_RESTRICT system, const int num_par
 ErrorStatus finalize_system(System *GS_RESTRICT system);
 /* replaces src/system.h (free_system; definition src/system.c:113-119) */
 void free_system(System *GS_RESTRICT system);
+/* 128 bytes: num_particles@0, particle_ids@8, x@16, v@24, m@32, h@40,
+ * omega_m@48, omega_lambda@56, omega_k@64, box_center@72, box_width@96,
+ * unit_mass@104, unit_length@112, unit_time@120 (src/system.h:22-38) */
+typedef struct CosmologicalSystem
+{
+    int num_particles;
+    int *particle_ids;
+    double *x;
+    double *v;
+    double *m;
+    double h;
+    double omega_m;
+    double omega_lambda;
+    double omega_k;
+    double box_center[3];
+    double box_width;
+    double unit_mass;
+    double unit_length;
+    double unit_time;
+} CosmologicalSystem;
+
+/* replaces src/system.h:78 (src/system.c:121-141) */
+CosmologicalSystem get_new_cosmological_system(void);
+/* replaces src/system.h:88-91 (src/system.c:143-172) */
+ErrorStatus get_initialized_cosmological_system(CosmologicalSystem *GS_RESTRICT system,
+                                                const int num_particles);
+/* replaces src/system.h (finalize_cosmological_system; src/system.c:174-242) */
+ErrorStatus finalize_cosmological_system(CosmologicalSystem *GS_RESTRICT system);
+/* replaces src/system.h (free_cosmological_system; src/system.c:244-250) */
+void free_cosmological_system(CosmologicalSystem *GS_RESTRICT system);
+/* replaces src/system.h (set_periodic_boundary_conditions; src/system.c:1337-1380) */
+void set_periodic_boundary_conditions(CosmologicalSystem *GS_RESTRICT system);
+
 /* replaces src/system.h:237-241 (src/system.c:579-1095): the named initial conditions circular_binary_orbit,
  * eccentric_binary_orbit, 3d_helix, sun_earth_moon, figure-8, pyth-3-body,
  * solar_system, solar_system_plus, bitwise equal to the reference's. */
@@ -347,6 +380,46 @@ ErrorStatus output_snapshot(OutputParam *output_param, const System *system,
                             const IntegratorParam *integrator_param,
                             const AccelerationParam *acceleration_param,
                             const SimulationStatus *simulation_status, const Settings *settings);
+
+/* ------------------------------------------------------------------ */
+/* Particle mesh and cosmology                                         */
+/* ------------------------------------------------------------------ */
+/* replaces src/cosmology.h (compute_omega_k, compute_da; src/cosmology.c:5-26) */
+double compute_omega_k(const double omega_m, const double omega_lambda);
+double compute_da(const double a, const double H0, const double omega_m, const double omega_lambda);
+
+/* replaces src/acceleration.h:73-85 (src/acceleration_PM.c:274-391): the PM
+ * acceleration (+grad phi of the CIC density contrast's potential, which the
+ * cosmological integrator subtracts) on the GPU: CIC deposit with FP64
+ * atomics, hipFFT r2c / c2r, 4th-order gradient, CIC interpolation.  Host
+ * arrays in and out.  pm_grid_size in 1..1024 (larger: GRAV_NOT_IMPLEMENTED_ERROR). */
+ErrorStatus acceleration_PM(double *GS_RESTRICT a, const int num_particles,
+                            const double *GS_RESTRICT x, const double *GS_RESTRICT m,
+                            const double *GS_RESTRICT box_center, const double box_width,
+                            const double H0, const double omega_m, const double mean_bkg_density,
+                            const int pm_grid_size, const double scale_factor);
+
+/* replaces src/integrator.h:155-164 (src/integrator_cosmology_leapfrog.c:27-255):
+ * device-resident, the step arithmetic of the reference; snapshots are HDF5
+ * only in the reference and are not built in, so output must be disabled. */
+ErrorStatus leapfrog_cosmology(CosmologicalSystem *GS_RESTRICT system,
+                               OutputParam *GS_RESTRICT output_param,
+                               SimulationStatus *GS_RESTRICT simulation_status,
+                               Settings *GS_RESTRICT settings, double dt, const double a_begin,
+                               const double a_final, const int pm_grid_size);
+
+/* replaces src/output.h:80-85 (src/output.c:295-349) */
+ErrorStatus output_snapshot_cosmology(OutputParam *output_param, const CosmologicalSystem *system,
+                                      const SimulationStatus *simulation_status,
+                                      const Settings *settings);
+
+/* replaces src/grav_sim.h:71-80 (src/grav_sim.c:140-260) */
+ErrorStatus launch_cosmological_simulation(CosmologicalSystem *GS_RESTRICT system,
+                                           OutputParam *GS_RESTRICT output_param,
+                                           SimulationStatus *GS_RESTRICT simulation_status,
+                                           Settings *GS_RESTRICT settings, double dt,
+                                           const double a_begin, const double a_final,
+                                           const int pm_grid_size);
 
 /* replaces src/grav_sim.h:47-55 (src/grav_sim.c:61-138) */
 ErrorStatus launch_simulation(System *GS_RESTRICT system,

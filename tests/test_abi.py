@@ -200,3 +200,33 @@ def test_ias15_refuses_fp32_forces():
     assert st.return_code == _abi.GRAV_VALUE_ERROR
     assert "FP64" in L.tb_text(st)
     lib.free_traceback(C.byref(st))
+
+
+def test_cosmological_system_layout_and_host_helpers():
+    """CosmologicalSystem is the reference's 128-byte struct; the host-side
+    helpers (defaults, validation, periodic wrap, expansion rate) follow
+    src/system.c:121-250,1337-1380 and src/cosmology.c:5-26 and need no GPU."""
+    import numpy as np
+    from oracle import pm as P
+    assert C.sizeof(_abi.CosmologicalSystem) == 128
+    assert _abi.CosmologicalSystem.box_center.offset == 72 and _abi.CosmologicalSystem.unit_time.offset == 120
+    lib = L.load()
+    s0 = lib.get_new_cosmological_system()
+    assert (s0.h, s0.omega_m, s0.omega_lambda, s0.box_width, s0.unit_mass) == (-1.0, -1.0, -1.0, -1.0, 1.0)
+    rng = np.random.default_rng(7)
+    c, w = [0.5, -0.25, 3.0], 0.75
+    x = np.asarray(c) + rng.uniform(-5.0, 5.0, size=(3000, 3))
+    v, m = np.zeros_like(x), np.ones(3000)
+    x0 = x.copy()
+    s, ids = L.cosmological_system(x, v, m, c, w, 0.7, 0.3, 0.7)
+    lib.set_periodic_boundary_conditions(C.byref(s))  # wraps x in place
+    assert not np.array_equal(x, x0) and np.array_equal(x, P.wrap(x0, c, w))
+    for a in (0.02, 0.5, 1.0):
+        assert lib.compute_da(a, 67.0, 0.3, 0.7) == P.compute_da(a, 67.0, 0.3, 0.7)
+    assert lib.finalize_cosmological_system(C.byref(s)).return_code == 0
+    for field, bad in (("h", 0.0), ("omega_m", -1.0), ("omega_lambda", 0.0), ("box_width", 0.0)):
+        t, _ = L.cosmological_system(x, v, m, c, w, 0.7, 0.3, 0.7)
+        setattr(t, field, bad)
+        st = lib.finalize_cosmological_system(C.byref(t))
+        assert st.return_code == _abi.GRAV_VALUE_ERROR, field
+        lib.free_traceback(C.byref(st))
