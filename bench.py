@@ -290,6 +290,69 @@ def run_snapshots(L, x, v, m):
             "rows_per_s_sync": len(m) / sync_one}
 
 
+def pm_lattice(nside, box, seed):
+    """A perturbed lattice (nside^3 particles, Gaussian displacements of 0.3
+    lattice spacings) in [0, box)^3, unit total mass."""
+    rng = np.random.default_rng(seed)
+    q = (np.arange(nside) + 0.5) * (box / nside)
+    x = np.stack(np.meshgrid(q, q, q, indexing="ij"), axis=-1).reshape(-1, 3)
+    x = np.mod(x + 0.3 * (box / nside) * rng.standard_normal(x.shape), box)
+    return x, np.full(len(x), 1.0 / len(x))
+
+
+def run_pm(L, evals=5, cpu=True):
+    """Particle mesh: one force evaluation for 128^3 particles on a 256^3 grid
+    (per-kernel times over `evals` evaluations), 50 steps of the
+    cosmological Leapfrog (LambdaCDM, a = 0.1 -> 0.2), and the numpy
+    restatement on a 64^3-particle, 64^3-grid sample as the CPU baseline."""
+    nside, grid, box = 128, 256, 100.0
+    x, m = pm_lattice(nside, box, 13)
+    c, bw = np.full(3, box / 2), box / 2
+    rho = m.sum() / box ** 3
+    H0, om, a = 70.0, 0.3, 0.5
+    L.acceleration_pm(x, m, c, bw, H0, om, rho, grid, a)  # plans, buffers
+    L.timing_enable(True)
+    L.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        L.acceleration_pm(x, m, c, bw, H0, om, rho, grid, a)
+    wall = (time.perf_counter() - t0) / evals
+    ks = {k: L.timing_query(k)[0] / evals for k in ("pm_deposit", "pm_fft", "pm_interp")}
+    L.timing_enable(False)
+    n = len(m)
+    dev_ms = sum(ks.values())
+    cells = float(grid) ** 3
+    # algorithmic bytes: the deposit clears and fills the grid (16 B/cell) from
+    # x, m (32 B/particle); the gradient + gather reads the potential once
+    # (8 B/cell), x and writes a (48 B/particle)
+    dep_gbs = (16.0 * cells + 32.0 * n) / (ks["pm_deposit"] * 1e-3) / 1e9
+    interp_gbs = (8.0 * cells + 48.0 * n) / (ks["pm_interp"] * 1e-3) / 1e9
+    L.leapfrog_cosmology(x, np.zeros_like(x), m, c, bw, 0.7, 0.3, 0.7, 0.01, 0.1, 0.11, grid)
+    t0 = time.perf_counter()
+    xs, steps, tt = L.leapfrog_cosmology(x, np.zeros_like(x), m, c, bw, 0.7, 0.3, 0.7, 0.002, 0.1, 0.2, grid)
+    cos_ms = (time.perf_counter() - t0) / steps * 1e3
+    out = {"workload": f"perturbed lattice {nside}^3 = {n} particles, {grid}^3 grid, FP64",
+           "particles": n, "grid": grid, "kernel_ms": ks, "device_ms_per_eval": dev_ms,
+           "particles_per_s": n / (dev_ms * 1e-3), "wall_ms_per_eval_incl_transfers": wall * 1e3,
+           "deposit_gbs": dep_gbs, "grad_interp_gbs": interp_gbs,
+           "cosmology_leapfrog": {"steps": int(steps), "ms_per_step": cos_ms}}
+    if cpu:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import pm as P
+        xs, ms = pm_lattice(64, box, 14)
+        t0 = time.perf_counter()
+        P.acceleration_pm(xs, ms, c, bw, H0, om, ms.sum() / box ** 3, 64, a)
+        cpu_s = time.perf_counter() - t0
+        L.acceleration_pm(xs, ms, c, bw, H0, om, ms.sum() / box ** 3, 64, a)
+        t0 = time.perf_counter()
+        L.acceleration_pm(xs, ms, c, bw, H0, om, ms.sum() / box ** 3, 64, a)
+        gpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "sample": "64^3 particles, 64^3 grid (numpy restatement)",
+                               "cpu_s": cpu_s, "gpu_wall_s_same_sample": gpu_s,
+                               "cpu_particles_per_s": len(ms) / cpu_s}
+    return out
+
+
 def run_whfast(L, ic, steps):
     off, X, V, M = ic.ensemble(4096, 3)
     P = int(off[-1])
@@ -326,6 +389,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--no-whfast", action="store_true")
+    ap.add_argument("--no-pm", action="store_true", help="skip the particle-mesh leg")
     ap.add_argument("--no-learned", action="store_true", help="Barnes-Hut without the learned correction")
     ap.add_argument("--no-integrators", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -406,6 +470,10 @@ def main():
     if not args.no_whfast:
         out["whfast"] = run_whfast(L, ic, args.whfast_steps)
         log(f"whfast: {out['whfast']['particle_steps_per_s']:.3e} particle-steps/s")
+    if not args.no_pm:
+        out["pm"] = run_pm(L, cpu=not args.no_cpu)
+        log(f"pm: {out['pm']['particles_per_s']:.3e} particles/s ({out['pm']['device_ms_per_eval']:.3f} ms per "
+            f"evaluation), cosmological Leapfrog {out['pm']['cosmology_leapfrog']['ms_per_step']:.3f} ms/step")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, m, args.cpu_sample)
     print(json.dumps(out), flush=True)

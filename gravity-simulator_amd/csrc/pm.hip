@@ -4,19 +4,23 @@
 // src/cosmology.c:5-26, src/grav_sim.c:140-260) on the GPU.
 //
 // One force evaluation is a chain of launches on the context stream:
-//   k_pm_clear       grid := 0
-//   k_cic_deposit    cloud in cell: 8 FP64 atomic adds per particle
-//   k_pm_contrast    delta = rho / (cell volume * mean density) - 1
-//   hipfftExecD2Z    real -> half spectrum, n x n x (n/2 + 1)
-//   k_pm_green       * (-1/k^2) * 3/2 Omega_m H0^2 / (a n^3)
-//   hipfftExecZ2D    half spectrum -> potential (unnormalised, like FFTW's)
-//   k_pm_gradient    +grad phi by 4th-order central differences, AoS 3 n^3
-//   k_cic_interp     8-corner CIC gather per particle
-// Every step but the deposit and the transforms has the reference's
-// operation order.  The deposit adds in no fixed order (the reference's
-// OpenMP loop adds without synchronisation, so it has none either), and the
-// scale factor is applied in the spectrum instead of after the inverse
-// transform: results agree with the reference's to rounding.
+//   k_pm_clear         grid := 0
+//   k_cic_deposit      cloud in cell: 8 FP64 atomic adds per particle
+//   hipfftExecD2Z      real -> half spectrum, n x n x (n/2 + 1)
+//   k_pm_green         * (-1/k^2) * 3/2 Omega_m H0^2 / (a n^3) / (cell volume * mean density)
+//   hipfftExecZ2D      half spectrum -> potential (unnormalised, like FFTW's)
+//   k_cic_grad_interp  per particle: +grad phi at its 8 CIC nodes by 4th-order
+//                      central differences, then the CIC gather
+// The reference's density-contrast pass, delta = rho / (cell volume * mean
+// density) - 1, is folded into the Green's function: the transform is
+// linear, and the -1 only touches k = 0, which the Green's function zeroes.
+// The reference's gradient grid (3 n^3 doubles, written once and gathered
+// from) is not formed: each particle evaluates the same stencil expression
+// at its own 8 nodes, which gives the same doubles.  The deposit adds in no
+// fixed order (the reference's OpenMP loop adds without synchronisation, so
+// it has none either), and the scale factors are applied in the spectrum
+// instead of before and after the transforms: results agree with the
+// reference's to rounding.
 #include <hipfft/hipfft.h>
 
 #include <cmath>
@@ -122,14 +126,9 @@ __global__ void k_cic_deposit(const double *__restrict__ x, const double *__rest
     }
 }
 
-__global__ void k_pm_contrast(double *__restrict__ g, size_t count, double denom)
-{
-    const size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (q < count) g[q] = (g[q] / denom) - 1.0;
-}
-
 // Green's function of the Poisson equation on the half spectrum
-// (src/acceleration_PM.c:313-335), with the potential's scale folded in.
+// (src/acceleration_PM.c:313-335), with the potential's scale and the
+// density normalisation folded into `factor`.
 __global__ void k_pm_green(double2 *__restrict__ spec, int N, double two_pi_l2, double factor)
 {
     const int H = N / 2 + 1;
@@ -148,43 +147,55 @@ __global__ void k_pm_green(double2 *__restrict__ spec, int N, double two_pi_l2, 
     spec[q] = s;
 }
 
-// +grad phi (src/acceleration_PM.c:212-272), one cell per lane.
-__global__ void k_pm_gradient(const double *__restrict__ phi, int N, double twelve_h,
-                              double *__restrict__ acc)
-{
-    const size_t count = static_cast<size_t>(N) * N * N;
-    const size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (q >= count) return;
-    const int k = static_cast<int>(q % N);
-    const int j = static_cast<int>((q / N) % N);
-    const int i = static_cast<int>(q / (static_cast<size_t>(N) * N));
-    const size_t N2 = static_cast<size_t>(N) * N;
-    auto at = [&](int a, int b, int c) { return phi[a * N2 + static_cast<size_t>(b) * N + c]; };
-    const int im2 = (i - 2 + N) % N, im = (i - 1 + N) % N, ip = (i + 1) % N, ip2 = (i + 2) % N;
-    const int jm2 = (j - 2 + N) % N, jm = (j - 1 + N) % N, jp = (j + 1) % N, jp2 = (j + 2) % N;
-    const int km2 = (k - 2 + N) % N, km = (k - 1 + N) % N, kp = (k + 1) % N, kp2 = (k + 2) % N;
-    acc[3 * q + 0] = (at(im2, j, k) - 8.0 * at(im, j, k) + 8.0 * at(ip, j, k) - at(ip2, j, k)) / twelve_h;
-    acc[3 * q + 1] = (at(i, jm2, k) - 8.0 * at(i, jm, k) + 8.0 * at(i, jp, k) - at(i, jp2, k)) / twelve_h;
-    acc[3 * q + 2] = (at(i, j, km2) - 8.0 * at(i, j, km) + 8.0 * at(i, j, kp) - at(i, j, kp2)) / twelve_h;
-}
-
-// CIC interpolation of the grid acceleration (src/acceleration_PM.c:107-210).
-__global__ void k_cic_interp(const double *__restrict__ x, int n, int N, double c0, double c1,
-                             double c2, double bw, double inv, const double *__restrict__ acc,
-                             double *__restrict__ a)
+// +grad phi at a particle's 8 CIC nodes (src/acceleration_PM.c:212-272: the
+// same 4th-order expression per node and axis) and the CIC gather of those
+// values (:107-210), one particle per lane.  Along each axis the stencils of
+// the two nodes share one line of 6 potential values.
+__global__ void k_cic_grad_interp(const double *__restrict__ x, int n, int N, double c0,
+                                  double c1, double c2, double bw, double inv,
+                                  const double *__restrict__ phi, double twelve_h,
+                                  double *__restrict__ a)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const Cell c = cell_of(&x[3 * static_cast<size_t>(p)], c0, c1, c2, bw, inv);
+    const size_t N2 = static_cast<size_t>(N) * N;
+    size_t li[6], lj[6], lk[6];  // node indices i-2 .. i+3 along each axis
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        li[t] = wrap_index(c.i[0] + t - 2, N) * N2;
+        lj[t] = wrap_index(c.i[1] + t - 2, N) * static_cast<size_t>(N);
+        lk[t] = wrap_index(c.i[2] + t - 2, N);
+    }
+    double g[8][3];  // node (dx, dy, dz) at g[dx + 2 dy + 4 dz]
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            double lx[6], ly[6], lz[6];
+#pragma unroll
+            for (int t = 0; t < 6; t++) {
+                lx[t] = phi[li[t] + lj[2 + u] + lk[2 + v]];
+                ly[t] = phi[li[2 + u] + lj[t] + lk[2 + v]];
+                lz[t] = phi[li[2 + u] + lj[2 + v] + lk[t]];
+            }
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                g[e + 2 * u + 4 * v][0] = (lx[e] - 8.0 * lx[e + 1] + 8.0 * lx[e + 3] - lx[e + 4]) / twelve_h;
+                g[u + 2 * e + 4 * v][1] = (ly[e] - 8.0 * ly[e + 1] + 8.0 * ly[e + 3] - ly[e + 4]) / twelve_h;
+                g[u + 2 * v + 4 * e][2] = (lz[e] - 8.0 * lz[e + 1] + 8.0 * lz[e + 3] - lz[e + 4]) / twelve_h;
+            }
+        }
     double s[3];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         int dx, dy, dz;
         corner(k, dx, dy, dz);
-        const double *g = &acc[3 * node(c, dx, dy, dz, N)];
+        const double *gk = g[dx + 2 * dy + 4 * dz];
         const double wx = dx ? c.w[0] : c.wm[0], wy = dy ? c.w[1] : c.wm[1], wz = dz ? c.w[2] : c.wm[2];
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double t = g[d] * wx * wy * wz;
+            const double t = gk[d] * wx * wy * wz;
             s[d] = k == 0 ? t : s[d] + t;
         }
     }
@@ -233,9 +244,8 @@ ErrorStatus fft_error(hipfftResult r, const char *what)
 struct PmWork {
     Context *c = nullptr;
     int N = 0;
-    double *grid = nullptr;   // density contrast, then the potential (N^3)
+    double *grid = nullptr;   // deposited density, then the potential (N^3)
     double2 *spec = nullptr;  // half spectrum N x N x (N/2 + 1)
-    double *acc = nullptr;    // +grad phi, AoS (3 N^3)
     hipfftHandle r2c = 0, c2r = 0;
     bool have_r2c = false, have_c2r = false;
 
@@ -246,7 +256,6 @@ struct PmWork {
         if (have_c2r) (void)hipfftDestroy(c2r);
         if (grid) (void)hipFree(grid);
         if (spec) (void)hipFree(spec);
-        if (acc) (void)hipFree(acc);
     }
 
     ErrorStatus init(Context *ctx, int n)
@@ -257,7 +266,6 @@ struct PmWork {
         const size_t nh = static_cast<size_t>(n) * n * (n / 2 + 1);
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&grid), sizeof(double) * n3));
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&spec), sizeof(double2) * nh));
-        GS_HIP(hipMalloc(reinterpret_cast<void **>(&acc), sizeof(double) * 3 * n3));
         GS_FFT(hipfftPlan3d(&r2c, n, n, n, HIPFFT_D2Z));
         have_r2c = true;
         GS_FFT(hipfftPlan3d(&c2r, n, n, n, HIPFFT_Z2D));
@@ -267,6 +275,31 @@ struct PmWork {
         return make_success_error_status();
     }
 };
+
+namespace {
+
+// The buffers and plans of the last grid size used, kept for the next call
+// (plans take milliseconds to build; a 256^3 grid holds 270 MB).  Used under
+// the context mutex.
+PmWork *g_pm = nullptr;
+
+ErrorStatus pm_work(Context *c, int N, PmWork **out)
+{
+    if (!g_pm || g_pm->N != N) {
+        delete g_pm;
+        g_pm = new PmWork;
+        ErrorStatus st = g_pm->init(c, N);
+        if (st.return_code != GRAV_SUCCESS) {
+            delete g_pm;
+            g_pm = nullptr;
+            return st;
+        }
+    }
+    *out = g_pm;
+    return make_success_error_status();
+}
+
+}  // namespace
 
 // The PM acceleration of n particles (device arrays) into a (device).
 ErrorStatus pm_acceleration(PmWork &W, const double *x, const double *m, int n,
@@ -285,28 +318,21 @@ ErrorStatus pm_acceleration(PmWork &W, const double *x, const double *m, int n,
     hipLaunchKernelGGL(k_pm_clear, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.grid, n3g);
     hipLaunchKernelGGL(k_cic_deposit, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, m, n, N,
                        center[0], center[1], center[2], bw, inv, W.grid);
-    const double cell_volume = cell * cell * cell;
-    hipLaunchKernelGGL(k_pm_contrast, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.grid, n3g,
-                       cell_volume * mean_bkg_density);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     rec = timing_begin(c, "pm_fft");
     GS_FFT(hipfftExecD2Z(W.r2c, W.grid, reinterpret_cast<hipfftDoubleComplex *>(W.spec)));
     const double two_pi_l2 = 4.0 * M_PI * M_PI / (box_length * box_length);
     const double factor = ((3.0 / 2.0) * omega_m * H0 * H0) / (scale_factor * static_cast<double>(n3g));
+    const double cell_volume = cell * cell * cell;
     hipLaunchKernelGGL(k_pm_green, grid_for(nh), dim3(PM_BLOCK), 0, c->stream, W.spec, N,
-                       two_pi_l2, factor);
+                       two_pi_l2, factor / (cell_volume * mean_bkg_density));
     GS_HIP(hipGetLastError());
     GS_FFT(hipfftExecZ2D(W.c2r, reinterpret_cast<hipfftDoubleComplex *>(W.spec), W.grid));
     timing_end(c, rec);
-    rec = timing_begin(c, "pm_gradient");
-    hipLaunchKernelGGL(k_pm_gradient, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.grid, N,
-                       12.0 * cell, W.acc);
-    timing_end(c, rec);
-    GS_HIP(hipGetLastError());
     rec = timing_begin(c, "pm_interp");
-    hipLaunchKernelGGL(k_cic_interp, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, n, N,
-                       center[0], center[1], center[2], bw, inv, W.acc, a);
+    hipLaunchKernelGGL(k_cic_grad_interp, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, n, N,
+                       center[0], center[1], center[2], bw, inv, W.grid, 12.0 * cell, a);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     return make_success_error_status();
@@ -475,9 +501,9 @@ ErrorStatus acceleration_PM(double *a, const int num_particles, const double *x,
     GS_TRY(slot(c, SLOT_A, sizeof(double) * n3, &da));
     GS_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
     GS_HIP(hipMemcpyAsync(dm, m, sizeof(double) * num_particles, hipMemcpyHostToDevice, c->stream));
-    PmWork W;
-    GS_TRY(W.init(c, pm_grid_size));
-    GS_TRY(pm_acceleration(W, static_cast<double *>(dx), static_cast<double *>(dm), num_particles,
+    PmWork *W;
+    GS_TRY(pm_work(c, pm_grid_size, &W));
+    GS_TRY(pm_acceleration(*W, static_cast<double *>(dx), static_cast<double *>(dm), num_particles,
                            box_center, box_width, H0, omega_m, mean_bkg_density, scale_factor,
                            static_cast<double *>(da)));
     GS_HIP(hipMemcpyAsync(a, da, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
@@ -556,8 +582,8 @@ ErrorStatus leapfrog_cosmology(CosmologicalSystem *system, OutputParam *op,
     GS_HIP(hipMemcpyAsync(x, system->x, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
     GS_HIP(hipMemcpyAsync(v, system->v, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
     GS_HIP(hipMemcpyAsync(m, system->m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    PmWork W;
-    GS_TRY(W.init(c, pm_grid_size));
+    PmWork *W;
+    GS_TRY(pm_work(c, pm_grid_size, &W));
     const dim3 g3 = grid_for(n3), blk(PM_BLOCK);
     const double *ctr = system->box_center;
 
@@ -566,7 +592,7 @@ ErrorStatus leapfrog_cosmology(CosmologicalSystem *system, OutputParam *op,
     GS_TRY(pm_wrap(c, x, n, ctr, system->box_width));
     hipLaunchKernelGGL(k_cos_scale, g3, blk, 0, c->stream, p, v, t * t, n3);
     GS_HIP(hipGetLastError());
-    GS_TRY(pm_acceleration(W, x, m, n, ctr, system->box_width, H0, om, mean_bkg_density, t, a));
+    GS_TRY(pm_acceleration(*W, x, m, n, ctr, system->box_width, H0, om, mean_bkg_density, t, a));
     // (a step size of 0 would make the reference's step count infinite)
     const long long total =
         dt > 0.0 ? static_cast<long long>(ceil((a_final - a_begin) / dt)) : 0;
@@ -582,7 +608,7 @@ ErrorStatus leapfrog_cosmology(CosmologicalSystem *system, OutputParam *op,
         hipLaunchKernelGGL(k_cos_drift, g3, blk, 0, c->stream, x, p, dt, (t * t) * da, n3);
         GS_HIP(hipGetLastError());
         GS_TRY(pm_wrap(c, x, n, ctr, system->box_width));
-        GS_TRY(pm_acceleration(W, x, m, n, ctr, system->box_width, H0, om, mean_bkg_density, t, a));
+        GS_TRY(pm_acceleration(*W, x, m, n, ctr, system->box_width, H0, om, mean_bkg_density, t, a));
         hipLaunchKernelGGL(k_cos_kick, g3, blk, 0, c->stream, p, a, 0.5 * dt, da, n3);
         GS_HIP(hipGetLastError());
         status->num_steps++;
