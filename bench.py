@@ -108,7 +108,8 @@ def bh_case(L, sim, n, method, theta, reps=5, label=None):
     wall, parts = timed_evals(L, sim, p, reps, BH_KERNELS)
     nint = L.last_walk_interactions()
     med, p99, mx = sim.accel_error(p, samples=4096)
-    r = {"method": label or {3: "monopole", 19: "quadrupole", 18: "learned"}[method], "theta": theta,
+    r = {"method": label or {3: "monopole", 19: "quadrupole", 18: "learned", 20: "group",
+                                 21: "group+quadrupole"}[method], "theta": theta,
          "particles_per_s": n / wall, "ms_per_eval": wall * 1e3, "interactions": nint,
          "median_rel": med, "p99_rel": p99, "max_rel": mx}
     r.update({k: round(v, 4) for k, v in parts.items()})
@@ -124,7 +125,7 @@ def run_bh(L, ic, learned=True):
     x, v, m = ic.two_spheres(n4, 4)
     sim = L.Sim(x, v, m, 1.0)
     out["n2e20"] = {"workload": "two Plummer spheres N=2^20 (seed 4), eps=1e-3, leaf=1",
-                    "plain": bh_case(L, sim, n4, 3, 0.5)}
+                    "plain": bh_case(L, sim, n4, 3, 0.5), "group": bh_case(L, sim, n4, 20, 0.5)}
     sim.close()
     log(f"BH 2^20 plain: {out['n2e20']['plain']['particles_per_s']:.3e} particles/s")
 
@@ -136,7 +137,9 @@ def run_bh(L, ic, learned=True):
     res = {"workload": "two Plummer spheres N=2^21 (seed 5), eps=1e-3, leaf=1; learned correction "
                        "trained on 10^6 samples of seed 7", "p99_target": P99_TARGET}
     res["plain"] = bh_case(L, sim, n5, 3, 0.5)
-    log(f"BH 2^21 plain theta 0.5: p99 {res['plain']['p99_rel']:.2e}")
+    res["group"] = bh_case(L, sim, n5, 20, 0.5)
+    log(f"BH 2^21 plain theta 0.5: p99 {res['plain']['p99_rel']:.2e}; group walk "
+        f"{res['group']['particles_per_s'] / res['plain']['particles_per_s']:.2f}x, p99 {res['group']['p99_rel']:.2e}")
     cands = []
     for th in (0.45, 0.4, 0.35, 0.3, 0.25, 0.2):
         r = bh_case(L, sim, n5, 3, th, reps=3)
@@ -150,7 +153,19 @@ def run_bh(L, ic, learned=True):
             res["quadrupole_at_target"] = r
             cands.append(r)
             break
-    log("BH 2^21 monopole / quadrupole at target done")
+    for th in (0.45, 0.4, 0.35, 0.3, 0.25):
+        r = bh_case(L, sim, n5, 20, th, reps=3)
+        if r["p99_rel"] <= P99_TARGET:
+            res["group_at_target"] = r
+            cands.append(r)
+            break
+    for th in (0.7, 0.65, 0.6, 0.55, 0.5, 0.45):
+        r = bh_case(L, sim, n5, 21, th, reps=3)
+        if r["p99_rel"] <= P99_TARGET:
+            res["group_quadrupole_at_target"] = r
+            cands.append(r)
+            break
+    log("BH 2^21 monopole / quadrupole / group at target done")
 
     if not learned:
         sim.close()

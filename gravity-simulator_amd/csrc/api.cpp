@@ -48,6 +48,15 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
                              p->method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE ? 2 : 1));
             return make_success_error_status();
         }
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP:
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE: {
+            const bool q = p->method == ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE;
+            DeviceTree *t;
+            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, q, &t));
+            GS_TRY(tree_walk_group(c, t, G, eps, p->opening_angle, a, &g_last_walk_interactions,
+                                   q ? 2 : 1));
+            return make_success_error_status();
+        }
         case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
             return learned_acceleration(c, x, m, n, G, p, a, &g_last_walk_interactions);
         default:
@@ -60,7 +69,7 @@ static bool known_method(int method)
 {
     return (method >= ACCELERATION_METHOD_PAIRWISE && method <= ACCELERATION_METHOD_BARNES_HUT) ||
            (method >= ACCELERATION_METHOD_PAIRWISE_FP32 &&
-            method <= ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE);
+            method <= ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE);
 }
 
 // Host-buffer form used by acceleration(): H2D x and m, compute, D2H a.

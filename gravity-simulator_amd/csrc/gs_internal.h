@@ -160,6 +160,10 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
 ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, double eps,
                       double theta, double *a, long long *interactions, int order = 1,
                       float *feat = nullptr);
+// Group walk (one wavefront per 64 sorted particles, group opening test);
+// order 1: monopoles, 2: with quadrupole terms (tree built with_quad).
+ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, double theta,
+                            double *a, long long *interactions, int order);
 // Host copy of the tree in the reference struct layout.
 ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out);
 

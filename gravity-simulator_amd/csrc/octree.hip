@@ -21,6 +21,7 @@
 //                  lanes that accept a node add its monopole, the others
 //                  descend.  Geometry and sums in FP64, 1/r^3 through FP32
 //                  rsqrt.
+#include <cmath>
 #include <cstring>
 
 #include "gs_internal.h"
@@ -857,6 +858,350 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
     if (lane == 0 && n_int) atomicAdd(n_int, tot + wcnt);
 }
 
+// ------------------------------------------------------------------ group walk
+// One wavefront walks the tree for 64 consecutive particles in sorted order
+// (a compact group): a node is accepted for the whole group when it holds none
+// of the group's particles and l^2 < theta^2 d^2, d the distance from its
+// centre of mass to the group's bounding box -- a test every particle of the
+// group would pass on its own (d_i >= d), so the interaction set refines the
+// per-particle walk's.  The traversal is breadth-parallel: each pass takes up
+// to 64 pending nodes from the wave's LDS stack, one per lane; accepted nodes
+// and single-particle leaves go to LDS interaction lists, opened nodes push
+// their children.  Each full list (64 entries) is then summed by every lane
+// for its own particle like a direct-sum tile: coordinates relative to the
+// group centre in FP32, 64-term FP32 partial sums added to FP64 totals.
+// The pending frontier of a group reaches 550 (median) to ~900 nodes at
+// N = 2^20 - 2^21: a 2048-entry stack (8 KB) keeps the passes full; two
+// wavefronts per workgroup, 13 KB of LDS each.
+constexpr int GW_WAVES = 2;
+constexpr int GW_BLOCK = 64 * GW_WAVES;
+constexpr int GW_STACK = 2048;
+constexpr int GW_LIST = 128;
+// batch passes keep the stack below this; single-node passes then grow it by
+// at most 7 per level of the remaining depth
+constexpr int GW_BATCH_LIMIT = GW_STACK - 8 * (MAXL + 2);
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Quadrupole term in FP32, R = x_i - com (the same expression as quad_term).
+__device__ __forceinline__ void quad_termf(const float *__restrict__ q, float r0, float r1,
+                                           float r2, float ri, float &o0, float &o1, float &o2)
+{
+    const float qxx = q[0], qyy = q[1], qzz = q[2], qxy = q[3], qxz = q[4], qyz = q[5];
+    const float QR0 = __builtin_fmaf(qxz, r2, __builtin_fmaf(qxy, r1, qxx * r0));
+    const float QR1 = __builtin_fmaf(qyz, r2, __builtin_fmaf(qyy, r1, qxy * r0));
+    const float QR2 = __builtin_fmaf(qzz, r2, __builtin_fmaf(qyz, r1, qxz * r0));
+    const float RQR = __builtin_fmaf(r2, QR2, __builtin_fmaf(r1, QR1, r0 * QR0));
+    const float ir2 = ri * ri;
+    const float ir5 = ir2 * ir2 * ri;
+    const float sc = ir5 * __builtin_fmaf(-7.5f * RQR, ir2, q[6]);
+    const float k3 = 3.0f * ir5;
+    o0 = __builtin_fmaf(k3, QR0, sc * r0);
+    o1 = __builtin_fmaf(k3, QR1, sc * r1);
+    o2 = __builtin_fmaf(k3, QR2, sc * r2);
+}
+
+// Walk groups from the tree: the children with at most 64 particles of each
+// node with more are packed, in order, into runs of at most 64 contiguous
+// particles (a root of n <= 64 particles is one group; a leaf of more than
+// 64 particles gives runs of 64).  A group's box can cover cells outside the
+// group, whose nodes the whole group must then open down to the leaves: a
+// run whose box is wider than `cap` in some axis (a few times the width of a
+// 64-particle group at the mean density) gives one group per particle --
+// sparse outliers, for which that is the per-particle walk.
+__device__ void emit_group(const double4 *__restrict__ P, int f, int c, double cap,
+                           int2 *__restrict__ groups, unsigned *__restrict__ ngroups)
+{
+    double lo0 = P[f].x, lo1 = P[f].y, lo2 = P[f].z, hi0 = lo0, hi1 = lo1, hi2 = lo2;
+    for (int q = f + 1; q < f + c; q++) {
+        const double4 p = P[q];
+        lo0 = fmin(lo0, p.x), lo1 = fmin(lo1, p.y), lo2 = fmin(lo2, p.z);
+        hi0 = fmax(hi0, p.x), hi1 = fmax(hi1, p.y), hi2 = fmax(hi2, p.z);
+    }
+    if (hi0 - lo0 > cap || hi1 - lo1 > cap || hi2 - lo2 > cap) {
+        for (int q = f; q < f + c; q++) groups[atomicAdd(ngroups, 1u)] = make_int2(q, 1);
+    } else {
+        groups[atomicAdd(ngroups, 1u)] = make_int2(f, c);
+    }
+}
+
+__global__ void k_group_list(const NodeRec *__restrict__ nodes, int num_nodes, int n,
+                             const double4 *__restrict__ P, double cap,
+                             int2 *__restrict__ groups, unsigned *__restrict__ ngroups)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= num_nodes) return;
+    const NodeRec nd = nodes[p];
+    if (p == 0 && nd.count <= 64) {
+        groups[atomicAdd(ngroups, 1u)] = make_int2(0, n);
+        return;
+    }
+    if (nd.count <= 64) return;
+    if (nd.nchild == 0) {
+        // a leaf of more than 64 particles: consecutive runs of 64
+        for (int f = nd.first; f < nd.first + nd.count; f += 64)
+            emit_group(P, f, min(64, nd.first + nd.count - f), cap, groups, ngroups);
+        return;
+    }
+    int start = -1, cnt = 0;
+    for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
+        const int cf = nodes[c].first, cc = nodes[c].count;
+        if (cc > 64 || cnt + cc > 64) {
+            if (cnt > 0) emit_group(P, start, cnt, cap, groups, ngroups);
+            start = -1;
+            cnt = 0;
+        }
+        if (cc <= 64) {
+            if (cnt == 0) start = cf;
+            cnt += cc;
+        }
+    }
+    if (cnt > 0) emit_group(P, start, cnt, cap, groups, ngroups);
+}
+
+template <bool QUAD>
+__global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restrict__ nodes,
+                                                         const double4 *__restrict__ P,
+                                                         const int *__restrict__ sidx, int n,
+                                                         const int2 *__restrict__ groups,
+                                                         int num_groups, WalkArgs args,
+                                                         double *__restrict__ a,
+                                                         unsigned long long *__restrict__ n_int,
+                                                         const float *__restrict__ quad)
+{
+    __shared__ int stk_all[GW_WAVES][GW_STACK];
+    __shared__ float4 nl_all[GW_WAVES][GW_LIST];  // accepted nodes: com - centre, mass
+    __shared__ int ni_all[GW_WAVES][GW_LIST];     // their node indices
+    __shared__ float4 pl_all[GW_WAVES][GW_LIST];  // particles: x - centre, mass
+    __shared__ int pi_all[GW_WAVES][GW_LIST];     // their sorted positions
+    __shared__ int rinc_all[GW_WAVES][64];        // opened multi-particle leaves:
+    __shared__ int rbeg_all[GW_WAVES][64];        // inclusive count prefix, first - offset
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int gi = blockIdx.x * GW_WAVES + wave;
+    if (gi >= num_groups) return;
+    const int2 grp = groups[gi];  // (first sorted position, count <= 64)
+    const int g0 = grp.x;
+    const int gend = g0 + grp.y;
+    const int s = g0 + lane;
+    const bool valid = s < gend;
+    int *stk = stk_all[wave];
+    float4 *nl = nl_all[wave];
+    int *ni = ni_all[wave];
+    float4 *pl = pl_all[wave];
+    int *pi = pi_all[wave];
+    int *rinc = rinc_all[wave];
+    int *rbeg = rbeg_all[wave];
+
+    const double4 ps = P[valid ? s : g0];
+    const double lo0 = wave_min(ps.x), lo1 = wave_min(ps.y), lo2 = wave_min(ps.z);
+    const double hi0 = wave_max(ps.x), hi1 = wave_max(ps.y), hi2 = wave_max(ps.z);
+    const double gc0 = 0.5 * (lo0 + hi0), gc1 = 0.5 * (lo1 + hi1), gc2 = 0.5 * (lo2 + hi2);
+    const float xr = static_cast<float>(ps.x - gc0), yr = static_cast<float>(ps.y - gc1),
+                zr = static_cast<float>(ps.z - gc2);
+    const float eps2 = static_cast<float>(args.eps2);
+    double A0 = 0.0, A1 = 0.0, A2 = 0.0;
+    unsigned cnt = 0;
+    bool overflow = false;
+    int top = 1, nn = 0, np = 0;
+
+    // Sum of list entries [0, m) for this lane's particle.
+    auto sum_nodes = [&](int m) {
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        for (int e = 0; e < m; e++) {
+            const float4 q = nl[e];
+            const float dx = q.x - xr, dy = q.y - yr, dz = q.z - zr;
+            float r2 = __builtin_fmaf(dx, dx, eps2);
+            r2 = __builtin_fmaf(dy, dy, r2);
+            r2 = __builtin_fmaf(dz, dz, r2);
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const float w = (q.w * ri) * (ri * ri);
+            f0 = __builtin_fmaf(dx, w, f0);
+            f1 = __builtin_fmaf(dy, w, f1);
+            f2 = __builtin_fmaf(dz, w, f2);
+            if (QUAD) {
+                const int node = __builtin_amdgcn_readfirstlane(ni[e]);
+                float o0, o1, o2;
+                quad_termf(quad + 8 * static_cast<size_t>(node), -dx, -dy, -dz, ri, o0, o1, o2);
+                f0 += o0;
+                f1 += o1;
+                f2 += o2;
+            }
+        }
+        A0 += static_cast<double>(f0);
+        A1 += static_cast<double>(f1);
+        A2 += static_cast<double>(f2);
+        cnt += m;
+    };
+    auto sum_parts = [&](int m) {
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        unsigned self = 0;
+        for (int e = 0; e < m; e++) {
+            const float4 q = pl[e];
+            const float dx = q.x - xr, dy = q.y - yr, dz = q.z - zr;
+            float r2 = __builtin_fmaf(dx, dx, eps2);
+            r2 = __builtin_fmaf(dy, dy, r2);
+            r2 = __builtin_fmaf(dz, dz, r2);
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const bool me = pi[e] == s;
+            const float w = me ? 0.0f : (q.w * ri) * (ri * ri);
+            self += me ? 1u : 0u;
+            f0 = __builtin_fmaf(dx, w, f0);
+            f1 = __builtin_fmaf(dy, w, f1);
+            f2 = __builtin_fmaf(dz, w, f2);
+        }
+        A0 += static_cast<double>(f0);
+        A1 += static_cast<double>(f1);
+        A2 += static_cast<double>(f2);
+        cnt += m - self;
+    };
+
+    // sum the first 64 particle-list entries, move the rest (< 64) to the front
+    auto flush_parts = [&]() {
+        sum_parts(64);
+        __builtin_amdgcn_wave_barrier();
+        if (lane < np - 64) {
+            pl[lane] = pl[64 + lane];
+            pi[lane] = pi[64 + lane];
+        }
+        np -= 64;
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    // from the root itself: it holds the group, so it is opened (a root that
+    // is a leaf -- n within the leaf capacity -- gives its particles directly)
+    if (lane == 0) stk[0] = 0;
+    __builtin_amdgcn_wave_barrier();
+    while (top > 0) {
+        int k = min(top, 64);
+        if (top + 7 * k > GW_BATCH_LIMIT) k = max(1, (GW_BATCH_LIMIT - top) / 7);
+        k = min(k, top);
+        const bool act = lane < k;
+        const int node = act ? stk[top - k + lane] : 0;
+        top -= k;
+        __builtin_amdgcn_wave_barrier();
+        const NodeRec nd = nodes[node];
+        const bool leaf = nd.nchild == 0;
+        const bool single = act && leaf && nd.count == 1;
+        const bool incl = nd.first < gend && nd.first + nd.count > g0;
+        const double e0 = fmax(fmax(lo0 - nd.cx, nd.cx - hi0), 0.0);
+        const double e1 = fmax(fmax(lo1 - nd.cy, nd.cy - hi1), 0.0);
+        const double e2 = fmax(fmax(lo2 - nd.cz, nd.cz - hi2), 0.0);
+        const double d2 = e0 * e0 + e1 * e1 + e2 * e2;
+        const bool accept = !incl && nd.l2 < args.theta2 * d2;
+        const bool take_node = act && !single && accept;
+        const bool multi = act && leaf && !single && !accept;
+        const bool open = act && !leaf && !accept;
+        // accepted nodes -> node list
+        {
+            const unsigned long long B = __ballot(take_node);
+            if (take_node) {
+                const int at = nn + __popcll(B & ((1ull << lane) - 1));
+                nl[at] = make_float4(static_cast<float>(nd.cx - gc0), static_cast<float>(nd.cy - gc1),
+                                     static_cast<float>(nd.cz - gc2), nd.fmass);
+                ni[at] = node;
+            }
+            nn += __popcll(B);
+        }
+        // single-particle leaves -> particle list (the particle itself)
+        {
+            const unsigned long long B = __ballot(single);
+            if (single) {
+                const int at = np + __popcll(B & ((1ull << lane) - 1));
+                const double4 p = P[nd.first];
+                pl[at] = make_float4(static_cast<float>(p.x - gc0), static_cast<float>(p.y - gc1),
+                                     static_cast<float>(p.z - gc2), static_cast<float>(p.w));
+                pi[at] = nd.first;
+            }
+            np += __popcll(B);
+        }
+        // opened internal nodes -> children on the stack
+        {
+            const int c = open ? nd.nchild : 0;
+            int inc = c;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            const int total = __shfl(inc, 63);
+            if (top + total <= GW_STACK) {
+                for (int q = 0; q < c; q++) stk[top + inc - c + q] = nd.child0 + q;
+                top += total;
+            } else {
+                overflow = true;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (np >= 64) flush_parts();
+        // opened leaves of several particles: their particles into the
+        // particle list, 64 slots at a time, every lane loading one particle
+        // (slot t belongs to the first multi leaf whose inclusive count
+        // prefix exceeds t)
+        {
+            const int c = multi ? nd.count : 0;
+            int inc = c;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            const int total = __shfl(inc, 63);
+            if (total > 0) {
+                rinc[lane] = inc;
+                rbeg[lane] = nd.first - (inc - c);  // particle of slot t: rbeg[L] + t
+                __builtin_amdgcn_wave_barrier();
+                for (int j0 = 0; j0 < total; j0 += 64) {
+                    const int m = min(64, total - j0);
+                    if (lane < m) {
+                        const int t = j0 + lane;
+                        int L = 0;
+                        for (int step = 32; step > 0; step >>= 1)
+                            if (rinc[L + step - 1] <= t) L += step;
+                        const int q = rbeg[L] + t;
+                        const double4 p = P[q];
+                        pl[np + lane] = make_float4(static_cast<float>(p.x - gc0), static_cast<float>(p.y - gc1),
+                                                    static_cast<float>(p.z - gc2), static_cast<float>(p.w));
+                        pi[np + lane] = q;
+                    }
+                    np += m;
+                    __builtin_amdgcn_wave_barrier();
+                    if (np >= 64) flush_parts();
+                }
+            }
+        }
+        // full lists: sum 64 entries, move the rest (< 64) to the front
+        if (nn >= 64) {
+            sum_nodes(64);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nn - 64) {
+                nl[lane] = nl[64 + lane];
+                ni[lane] = ni[64 + lane];
+            }
+            nn -= 64;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (nn > 0) sum_nodes(nn);
+    if (np > 0) sum_parts(np);
+    if (valid) {
+        const int i = sidx[s];
+        a[3 * i + 0] = args.G * A0;
+        a[3 * i + 1] = args.G * A1;
+        a[3 * i + 2] = args.G * A2;
+    }
+    unsigned long long tot = valid ? cnt : 0u;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane == 0 && n_int) atomicAdd(n_int, tot + (overflow ? (1ull << 62) : 0ull));
+}
+
 // ------------------------------------------------------------------ host side
 static DeviceTree g_tree;
 
@@ -1166,6 +1511,62 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
             return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut walk stack overflow");
         *interactions = static_cast<long long>(h);
     }
+    return make_success_error_status();
+}
+
+// Group walk (methods 20 and 21): order 1 monopoles, 2 with quadrupoles.
+// Widest box of a walk group before it is split into single particles: four
+// times the width of a 64-particle group at the mean density of the tree's box.
+static double group_width_cap(double width, int n)
+{
+    return 4.0 * width * std::cbrt(64.0 / static_cast<double>(n));
+}
+
+ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, double theta,
+                            double *a, long long *interactions, int order)
+{
+    if (order == 2 && !t->has_quad)
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Quadrupole walk of a tree built without second moments");
+    WalkArgs args;
+    args.theta2 = theta * theta;
+    args.eps2 = eps * eps;
+    args.G = G;
+    void *p;
+    GS_TRY(slot(c, SLOT_MISC, 256, &p));
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
+    GS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+    // groups: at most one per particle
+    void *gp;
+    GS_TRY(slot(c, SLOT_NODE_TMP, sizeof(int2) * static_cast<size_t>(t->n) + 256, &gp));
+    int2 *groups = static_cast<int2 *>(gp);
+    unsigned *ng = reinterpret_cast<unsigned *>(static_cast<char *>(p) + 128);
+    GS_HIP(hipMemsetAsync(ng, 0, sizeof(unsigned), c->stream));
+    const int rec = timing_begin(c, "bh_walk");
+    const double cap = group_width_cap(t->width, t->n);
+    hipLaunchKernelGGL(k_group_list, dim3((t->num_nodes + 255) / 256), dim3(256), 0, c->stream,
+                       t->nodes, t->num_nodes, t->n, t->P, cap, groups, ng);
+    GS_HIP(hipGetLastError());
+    unsigned num_groups = 0;
+    GS_HIP(hipMemcpyAsync(&num_groups, ng, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    if (num_groups > static_cast<unsigned>(t->n))
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group list larger than the particles");
+    const int grid = (static_cast<int>(num_groups) + GW_WAVES - 1) / GW_WAVES;
+    if (order >= 2)
+        hipLaunchKernelGGL((k_walk_group<true>), dim3(grid), dim3(GW_BLOCK), 0, c->stream, t->nodes,
+                           t->P, t->sidx, t->n, groups, static_cast<int>(num_groups), args, a, cnt,
+                           t->quadf);
+    else
+        hipLaunchKernelGGL((k_walk_group<false>), dim3(grid), dim3(GW_BLOCK), 0, c->stream,
+                           t->nodes, t->P, t->sidx, t->n, groups, static_cast<int>(num_groups), args,
+                           a, cnt, t->quadf);
+    timing_end(c, rec);
+    GS_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    GS_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    if (h >> 62) return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group walk stack overflow");
+    if (interactions) *interactions = static_cast<long long>(h);
     return make_success_error_status();
 }
 

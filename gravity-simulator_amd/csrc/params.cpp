@@ -26,7 +26,9 @@ static bool is_bh(int method)
 {
     return method == ACCELERATION_METHOD_BARNES_HUT ||
            method == ACCELERATION_METHOD_BARNES_HUT_LEARNED ||
-           method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE;
+           method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE ||
+           method == ACCELERATION_METHOD_BARNES_HUT_GROUP ||
+           method == ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE;
 }
 
 static bool known_method(int method)
@@ -39,6 +41,8 @@ static bool known_method(int method)
         case ACCELERATION_METHOD_MASSLESS_FP32:
         case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
         case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE:
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP:
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE:
             return true;
         default:
             return false;

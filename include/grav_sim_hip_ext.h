@@ -34,6 +34,19 @@ extern "C" {
 /* Barnes-Hut with quadrupole moments (same tree and opening criterion as
  * ACCELERATION_METHOD_BARNES_HUT, multipole order 2). */
 #define ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE 19
+/* Barnes-Hut group walk: each wavefront walks the tree once for a group of
+ * at most 64 particles (children of at most 64 particles of a tree node with
+ * more, packed in order; groups wider than 4 w (64/n)^(1/3) in some axis,
+ * w the tree's box width, are split into single particles) and accepts a
+ * node for all of them
+ * when l^2 < theta^2 d^2 with d the distance from the node's centre of mass
+ * to the group's bounding box (none of the group inside the node).  Every
+ * particle would accept such a node on its own, so the interactions refine
+ * those of ACCELERATION_METHOD_BARNES_HUT at the same theta (more
+ * interactions, smaller errors); the force sums run as dense FP32 tiles
+ * relative to the group centre.  Monopoles (20) or with quadrupoles (21). */
+#define ACCELERATION_METHOD_BARNES_HUT_GROUP 20
+#define ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE 21
 
 /* ---- device information --------------------------------------------------- */
 typedef struct GsDeviceInfo

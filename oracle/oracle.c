@@ -402,6 +402,215 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
     }
 }
 
+/* Barnes-Hut walk of groups of at most `group` consecutive particles in
+ * sorted order, formed from the tree as the GPU forms them (below): a node
+ * is accepted for the whole group when
+ * none of the group's particles lies in it and (l)^2 < theta^2 d^2 with d the
+ * distance from the node's centre of mass to the group's bounding box, else
+ * opened (leaves: particle-particle terms, skipping self).  Every particle
+ * of the group then has d_i >= d, so each accepted node would also be
+ * accepted by the per-particle test of src/acceleration_barnes_hut.c:152-175;
+ * with group = 1 the box is the particle and the walk is orc_bh_walk's. */
+/* Appends the run [f, f + c) of sorted positions to the group list, or its
+ * particles one by one when its bounding box is wider than cap in some axis. */
+static int emit_group(int (*gl)[2], int ng, const double *x, const int32_t *sorted_idx, int f, int c,
+                      double cap)
+{
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++)
+        lo[k] = hi[k] = x[sorted_idx[f] * 3 + k];
+    for (int q = f + 1; q < f + c; q++)
+        for (int k = 0; k < 3; k++)
+        {
+            const double v = x[sorted_idx[q] * 3 + k];
+            lo[k] = fmin(lo[k], v);
+            hi[k] = fmax(hi[k], v);
+        }
+    if (hi[0] - lo[0] > cap || hi[1] - lo[1] > cap || hi[2] - lo[2] > cap)
+        for (int q = f; q < f + c; q++)
+            gl[ng][0] = q, gl[ng][1] = 1, ng++;
+    else
+        gl[ng][0] = f, gl[ng][1] = c, ng++;
+    return ng;
+}
+
+typedef struct
+{
+    const Walk *w;
+    int s0, s1; /* sorted positions [s0, s1) */
+    double lo[3], hi[3];
+    double (*xs)[3];
+    double (*acc)[3];
+    int64_t *cnt;
+    const int *idx;
+} GroupWalk;
+
+static void group_node(const GroupWalk *g, int node)
+{
+    const Walk *w = g->w;
+    const int d = w->depth[node];
+    const int f = w->first[node], c = w->count[node];
+    const int included = f < g->s1 && f + c > g->s0;
+    const int gn = g->s1 - g->s0;
+    if (!included)
+    {
+        double dd2 = 0.0;
+        const double com[3] = {w->comx[node], w->comy[node], w->comz[node]};
+        for (int k = 0; k < 3; k++)
+        {
+            const double e = com[k] < g->lo[k] ? g->lo[k] - com[k] : (com[k] > g->hi[k] ? com[k] - g->hi[k] : 0.0);
+            dd2 += e * e;
+        }
+        const double l = (w->width * 2.0) / (double)(2LL << d);
+        if ((l * l) < w->theta2 * dd2)
+        {
+            for (int q = 0; q < gn; q++)
+            {
+                const double R0 = g->xs[q][0] - com[0], R1 = g->xs[q][1] - com[1], R2 = g->xs[q][2] - com[2];
+                const double d2 = R0 * R0 + R1 * R1 + R2 * R2;
+                const double r = sqrt(d2 + w->eps2);
+                const double t = w->G * w->mass[node] / (r * r * r);
+                g->acc[q][0] -= t * R0;
+                g->acc[q][1] -= t * R1;
+                g->acc[q][2] -= t * R2;
+                if (w->quad)
+                    quad_accel(g->acc[q], &w->quad[6 * node], R0, R1, R2, d2 + w->eps2, w->G);
+                g->cnt[q]++;
+            }
+            return;
+        }
+    }
+    if (w->nchild[node] == 0)
+    {
+        for (int t = f; t < f + c; t++)
+        {
+            const int j = w->sidx[t];
+            for (int q = 0; q < gn; q++)
+            {
+                if (g->s0 + q == t)
+                    continue;
+                const double R0 = g->xs[q][0] - w->x[j * 3 + 0];
+                const double R1 = g->xs[q][1] - w->x[j * 3 + 1];
+                const double R2 = g->xs[q][2] - w->x[j * 3 + 2];
+                const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2 + w->eps2);
+                const double tv = w->G * w->m[j] / (r * r * r);
+                g->acc[q][0] -= tv * R0;
+                g->acc[q][1] -= tv * R1;
+                g->acc[q][2] -= tv * R2;
+                g->cnt[q]++;
+            }
+        }
+        return;
+    }
+    for (int ch = w->child0[node]; ch < w->child0[node] + w->nchild[node]; ch++)
+        group_node(g, ch);
+}
+
+void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const double *m,
+                       double G, double eps, double theta, double width,
+                       const int64_t *sorted_keys, const int32_t *sorted_idx, const int *depth,
+                       const int64_t *prefix, const int *count, const int *first,
+                       const int *nchild, const int *child0, const double *mass,
+                       const double *comx, const double *comy, const double *comz,
+                       const double *quad, int group)
+{
+    Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
+              depth, count, first,  nchild,    child0,        prefix, mass,        comx,
+              comy,  comz,  quad,   0};
+    /* groups from the tree: children of at most `group` particles of each node
+     * with more are packed in order into runs of at most `group` particles; a
+     * root within the size is one group; larger leaves give runs of `group`.
+     * A run wider than `cap` in some axis gives one group per particle. */
+    const double cap = 4.0 * width * cbrt(64.0 / (double)n);
+    int (*gl)[2] = malloc(sizeof(int[2]) * (size_t)(n > 0 ? n : 1));
+    int ng = 0;
+    if (count[0] <= group)
+        gl[ng][0] = 0, gl[ng][1] = n, ng++;
+    else
+    {
+        /* nodes are visited depth-first from the root (the group order does not
+         * change any particle's result); at most 8 pending nodes per level */
+        int *stack = malloc(sizeof(int) * 8 * (MAX_LEVEL + 2));
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0)
+        {
+            const int p = stack[--sp];
+            if (count[p] <= group)
+                continue;
+            if (nchild[p] == 0)
+            {
+                for (int f = first[p]; f < first[p] + count[p]; f += group)
+                    ng = emit_group(gl, ng, x, sorted_idx, f,
+                                    (first[p] + count[p] - f) < group ? (first[p] + count[p] - f) : group, cap);
+                continue;
+            }
+            int start = -1, cnt = 0;
+            for (int c = child0[p]; c < child0[p] + nchild[p]; c++)
+            {
+                if (count[c] > group || cnt + count[c] > group)
+                {
+                    if (cnt > 0)
+                        ng = emit_group(gl, ng, x, sorted_idx, start, cnt, cap);
+                    start = -1, cnt = 0;
+                }
+                if (count[c] <= group)
+                {
+                    if (cnt == 0)
+                        start = first[c];
+                    cnt += count[c];
+                }
+                else
+                    stack[sp++] = c;
+            }
+            if (cnt > 0)
+                ng = emit_group(gl, ng, x, sorted_idx, start, cnt, cap);
+        }
+        free(stack);
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int gi = 0; gi < ng; gi++)
+    {
+        GroupWalk g;
+        g.w = &w;
+        g.s0 = gl[gi][0];
+        g.s1 = gl[gi][0] + gl[gi][1];
+        const int gn = g.s1 - g.s0;
+        double (*xs)[3] = malloc(sizeof(double[3]) * gn);
+        double (*acc)[3] = calloc(gn, sizeof(double[3]));
+        int64_t *cnt = calloc(gn, sizeof(int64_t));
+        for (int k = 0; k < 3; k++)
+            g.lo[k] = INFINITY, g.hi[k] = -INFINITY;
+        for (int q = 0; q < gn; q++)
+        {
+            const int i = sorted_idx[g.s0 + q];
+            for (int k = 0; k < 3; k++)
+            {
+                xs[q][k] = x[i * 3 + k];
+                if (xs[q][k] < g.lo[k]) g.lo[k] = xs[q][k];
+                if (xs[q][k] > g.hi[k]) g.hi[k] = xs[q][k];
+            }
+        }
+        g.xs = xs;
+        g.acc = acc;
+        g.cnt = cnt;
+        for (int c = child0[0]; c < child0[0] + nchild[0]; c++)
+            group_node(&g, c);
+        for (int q = 0; q < gn; q++)
+        {
+            const int i = sorted_idx[g.s0 + q];
+            for (int k = 0; k < 3; k++)
+                a[i * 3 + k] = acc[q][k];
+            if (n_int)
+                n_int[i] = cnt[q];
+        }
+        free(xs);
+        free(acc);
+        free(cnt);
+    }
+    free(gl);
+}
+
 void orc_octree_quad(int num, const double *x, const double *m, const int32_t *sorted_idx,
                      const int *count, const int *first, const int *nchild, const int *child0,
                      const double *mass, const double *comx, const double *comy,

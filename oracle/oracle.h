@@ -83,6 +83,20 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
                  const double *mass, const double *comx, const double *comy, const double *comz,
                  const double *quad, double *feat);
 
+/* The walk of groups of at most `group` consecutive sorted particles (the
+ * children of at most `group` particles of each node with more, packed in
+ * order): a node is accepted for the whole group when it holds none of the
+ * group's particles and (l)^2 < theta^2 d^2 with d the distance from its
+ * centre of mass to the group's bounding box; group = 1 is orc_bh_walk
+ * (without features). */
+void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const double *m,
+                       double G, double eps, double theta, double width,
+                       const int64_t *sorted_keys, const int32_t *sorted_idx, const int *depth,
+                       const int64_t *prefix, const int *count, const int *first,
+                       const int *nchild, const int *child0, const double *mass,
+                       const double *comx, const double *comy, const double *comz,
+                       const double *quad, int group);
+
 /* Second central moments per node, Q_ab = sum m (x - com)_a (x - com)_b, in
  * the order xx yy zz xy xz yz: leaves over their particles, internal nodes
  * from their children by the parallel-axis shift.  An extension (the

@@ -65,6 +65,8 @@ def lib():
         L.orc_octree.restype = i
         L.orc_bh_walk.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI, PI,
                                   PD, PD, PD, PD, PD, PD]
+        L.orc_bh_walk_group.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI, PI,
+                                        PD, PD, PD, PD, PD, i]
         L.orc_octree_quad.argtypes = [i, PD, PD, PI, PI, PI, PI, PI, PD, PD, PD, PD, PD]
         L.orc_leapfrog.argtypes = [PD, PD, i, PD, d, i, d, d, d]
         L.orc_leapfrog.restype = i64
@@ -231,6 +233,24 @@ def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, fea
     if features:
         out = out + (feat,)
     return out if len(out) > 1 else out[0]
+
+
+def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None):
+    """Group walk (groups of `group` consecutive sorted particles); returns
+    (a, per-particle interaction counts).  group=1 is bh_walk."""
+    x, m = _c(x), _c(m)
+    n = len(m)
+    a = np.zeros_like(x)
+    cnt = np.zeros(n, dtype=np.int64)
+    t = tree
+    qp = _abi.dptr(_c(quad)) if quad is not None else None
+    lib().orc_bh_walk_group(_abi.dptr(a), cnt.ctypes.data_as(PI64), n, _abi.dptr(x), _abi.dptr(m), G, eps, theta,
+                            t["width"], _c(t["sorted_keys"], np.int64).ctypes.data_as(PI64),
+                            _abi.iptr(_c(t["sorted_idx"], np.int32)), _abi.iptr(t["depth"]),
+                            t["prefix"].ctypes.data_as(PI64), _abi.iptr(t["count"]), _abi.iptr(t["first"]),
+                            _abi.iptr(t["nchild"]), _abi.iptr(t["child0"]), _abi.dptr(t["mass"]),
+                            _abi.dptr(t["comx"]), _abi.dptr(t["comy"]), _abi.dptr(t["comz"]), qp, int(group))
+    return a, cnt
 
 
 def leapfrog(x, v, m, G, dt, tf, eps=0.0, method=1):

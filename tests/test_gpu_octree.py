@@ -172,3 +172,56 @@ def test_quadrupole_beats_monopole_config4():
     print(f"N=2^20 theta=0.5 p99: monopole {mono[1]:.3e}, quadrupole {quad[1]:.3e}")
     assert quad[1] < 0.5 * mono[1] and quad[0] < 0.5 * mono[0]
     sim.close()
+
+
+# ---------------------------------------------------------------- group walk (methods 20, 21)
+def _group_case(n, seed, theta, eps, leaf, quad):
+    x, v, m = ic.two_spheres(n, seed)
+    t = O.build_tree(x, m, leaf)
+    q = O.octree_quad(x, m, t) if quad else None
+    a_o, cnt = O.bh_walk_group(x, m, t, theta, 64, eps=eps, quad=q)
+    a_g = L.acceleration(x, m, 21 if quad else 20, eps=eps, theta=theta, leaf=leaf)
+    return x, m, a_o, cnt, a_g
+
+
+@pytest.mark.parametrize("theta,eps,leaf,quad", [(0.5, 0.0, 1, False), (0.8, 1e-3, 1, False), (0.5, 0.0, 8, False),
+                                                 (0.5, 0.0, 1, True), (0.8, 1e-3, 3, True)])
+def test_group_walk_matches_oracle_group_walk(theta, eps, leaf, quad):
+    """Same interaction set as the oracle's walk of 64-particle groups (equal
+    counts), forces within FP32 rounding of its FP64 sums."""
+    print("group walk case", theta, eps, leaf, quad, flush=True)
+    x, m, a_o, cnt, a_g = _group_case(8192, 7, theta, eps, leaf, quad)
+    assert L.last_walk_interactions() == int(cnt.sum())
+    e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
+@pytest.mark.parametrize("n,leaf", [(1, 1), (2, 1), (5, 8), (63, 1), (777, 2), (5000, 40)])
+def test_group_walk_ragged_and_theta_zero(n, leaf):
+    """theta = 0 opens every node: the direct sum with n (n - 1) interactions,
+    for partial groups, a root that is a leaf and large leaves."""
+    print("group walk ragged", n, leaf, flush=True)
+    x, v, m = ic.plummer(max(n, 2), 30 + n)
+    x, m = x[:n], m[:n]
+    a_g = L.acceleration(x, m, 20, eps=1e-3, theta=0.0, leaf=leaf)
+    assert L.last_walk_interactions() == n * (n - 1)
+    if n > 1:
+        a_d = O.pairwise(x, m, eps=1e-3)
+        e = np.linalg.norm(a_g - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
+        assert np.median(e) <= 1e-6 and e.max() <= 1e-4
+    else:
+        assert np.all(a_g == 0.0)
+
+
+def test_group_walk_error_below_per_particle_walk_config4():
+    """N=2^20 two-sphere at theta=0.5: the group walk's errors against the FP64
+    direct sum are below the per-particle walk's (it opens more nodes)."""
+    x, v, m = ic.two_spheres(1 << 20, 4)
+    sim = L.Sim(x, v, m, 1.0)
+    plain = sim.accel_error(L.param(3, theta=0.5, leaf=1), samples=4096)
+    group = sim.accel_error(L.param(20, theta=0.5, leaf=1), samples=4096)
+    gq = sim.accel_error(L.param(21, theta=0.5, leaf=1), samples=4096)
+    print(f"N=2^20 theta=0.5 p99: plain {plain[1]:.3e}, group {group[1]:.3e}, group+quad {gq[1]:.3e}")
+    assert group[1] < plain[1] and group[0] < plain[0]
+    assert gq[1] < group[1]
+    sim.close()
