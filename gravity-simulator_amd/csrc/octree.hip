@@ -871,11 +871,15 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
 // for its own particle like a direct-sum tile: coordinates relative to the
 // group centre in FP32, 64-term FP32 partial sums added to FP64 totals.
 // The pending frontier of a group reaches 550 (median) to ~900 nodes at
-// N = 2^20 - 2^21: a 2048-entry stack (8 KB) keeps the passes full; two
-// wavefronts per workgroup, 13 KB of LDS each.
-constexpr int GW_WAVES = 2;
+// N = 2^20 - 2^21.  A 1024-entry stack (4 KB; 9.5 KB of LDS per wavefront)
+// keeps most passes full and ~16 wavefronts resident per CU: at N = 2^21,
+// theta 0.5, the launch takes 5.2 ms against 6.0 ms with 2048 entries (fewer
+// resident wavefronts) and 5.8 / 12.2 ms with 768 / 512 (more single-node
+// passes once the frontier reaches the batch limit below).  One wavefront per
+// workgroup: 6.0 ms against 6.7 with two and 8.2 with four (2048 entries).
+constexpr int GW_WAVES = 1;
 constexpr int GW_BLOCK = 64 * GW_WAVES;
-constexpr int GW_STACK = 2048;
+constexpr int GW_STACK = 1024;
 constexpr int GW_LIST = 128;
 // batch passes keep the stack below this; single-node passes then grow it by
 // at most 7 per level of the remaining depth
