@@ -196,6 +196,26 @@ def test_group_walk_matches_oracle_group_walk(theta, eps, leaf, quad):
     assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
 
 
+@pytest.mark.parametrize("leaf,quad", [(1, False), (4, False), (1, True)])
+def test_group_walk_coincident_clump_and_outlier_halo(leaf, quad):
+    """150 coincident particles (one leaf at the deepest level: groups are its
+    runs of 64, and other groups open it as a leaf of more than 64 particles)
+    and a wide halo (runs wider than the cap become one group per particle):
+    the interaction set of the oracle's group walk, forces to FP32 rounding."""
+    print("group walk clump/halo", leaf, quad, flush=True)
+    x, v, m = ic.plummer(6000, 41)
+    rng = np.random.default_rng(41)
+    x[100:250] = x[100]
+    x[:40] = rng.uniform(-60.0, 60.0, size=(40, 3))
+    t = O.build_tree(x, m, leaf)
+    q = O.octree_quad(x, m, t) if quad else None
+    a_o, cnt = O.bh_walk_group(x, m, t, 0.6, 64, eps=1e-3, quad=q)
+    a_g = L.acceleration(x, m, 21 if quad else 20, eps=1e-3, theta=0.6, leaf=leaf)
+    assert L.last_walk_interactions() == int(cnt.sum())
+    e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
 @pytest.mark.parametrize("n,leaf", [(1, 1), (2, 1), (5, 8), (63, 1), (777, 2), (5000, 40)])
 def test_group_walk_ragged_and_theta_zero(n, leaf):
     """theta = 0 opens every node: the direct sum with n (n - 1) interactions,
