@@ -44,6 +44,13 @@ struct DeviceTree {
     unsigned *scan_tmp = nullptr;   // device: scan scratch
 };
 
+// Stable LSD radix sort of n (key, index) pairs on the low 8 * passes bits of
+// the (non-negative) keys, alternating between (k0, i0) and (k1, i1); the
+// sorted pairs end in (*ko, *io).  hist holds radix_hist_elements(n) counters
+// and stmp scan_temp_bytes(radix_hist_elements(n)) bytes.
+size_t radix_hist_elements(int n);
+ErrorStatus radix_sort_pairs(Context *c, long long *k0, int *i0, long long *k1, int *i1, int n,
+                             int passes, unsigned *hist, unsigned *stmp, long long **ko, int **io);
 ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_center,
                       double box_width, DeviceTree *t);
 

@@ -1246,6 +1246,38 @@ static ErrorStatus ensure_nodes(Context *c, DeviceTree *t, int need)
     return make_success_error_status();
 }
 
+size_t radix_hist_elements(int n)
+{
+    return static_cast<size_t>((n + RS_TILE - 1) / RS_TILE) * RS_BINS;
+}
+
+ErrorStatus radix_sort_pairs(Context *c, long long *k0, int *i0, long long *k1, int *i1, int n,
+                             int passes, unsigned *hist, unsigned *stmp, long long **ko, int **io)
+{
+    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    const size_t hn = radix_hist_elements(n);
+    if (static_cast<long long>(hn) > scan_max_elements())
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Too many elements for the radix sort: %d", n);
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = pass * RS_BITS;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, n, shift,
+                           hist, ntiles);
+        GS_HIP(scan_exclusive(c->stream, hist, hist, static_cast<long long>(hn), stmp, nullptr));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, i0, n,
+                           shift, hist, ntiles, k1, i1);
+        GS_HIP(hipGetLastError());
+        long long *tk = k0;
+        k0 = k1;
+        k1 = tk;
+        int *ti = i0;
+        i0 = i1;
+        i1 = ti;
+    }
+    *ko = k0;
+    *io = i0;
+    return make_success_error_status();
+}
+
 ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_center,
                       double box_width, DeviceTree *t)
 {
@@ -1284,21 +1316,11 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, n, box, k0,
                        i0);
     GS_HIP(hipGetLastError());
-    for (int pass = 0; pass < RS_PASSES; pass++) {
-        const int shift = pass * RS_BITS;
-        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, n, shift,
-                           hist, ntiles);
-        GS_HIP(scan_exclusive(c->stream, hist, hist, static_cast<long long>(hn), stmp, nullptr));
-        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, i0, n,
-                           shift, hist, ntiles, k1, i1);
-        GS_HIP(hipGetLastError());
-        long long *tk = k0;
-        k0 = k1;
-        k1 = tk;
-        int *ti = i0;
-        i0 = i1;
-        i1 = ti;
-    }
+    long long *ks;
+    int *is;
+    GS_TRY(radix_sort_pairs(c, k0, i0, k1, i1, n, RS_PASSES, hist, stmp, &ks, &is));
+    k0 = ks;
+    i0 = is;
     timing_end(c, rec);
     // 8 passes: the sorted data is back in the primary slots
     t->n = n;

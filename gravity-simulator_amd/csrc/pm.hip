@@ -4,8 +4,12 @@
 // src/cosmology.c:5-26, src/grav_sim.c:140-260) on the GPU.
 //
 // One force evaluation is a chain of launches on the context stream:
-//   k_pm_clear         grid := 0
-//   k_cic_deposit      cloud in cell: 8 FP64 atomic adds per particle
+//   k_pm_cell_keys     base cell of each particle (wrapped into the grid)
+//   radix sort         (cell, particle) pairs, stable, 8 bits per pass
+//   k_pm_cell_ranges   first and end sorted position of each cell's particles
+//   k_cic_gather       cloud in cell as a gather: each cell sums, corner by
+//                      corner, the particles whose base cell is that corner
+//                      away -- no atomics, and a fixed order of the sums
 //   hipfftExecD2Z      real -> half spectrum, n x n x (n/2 + 1)
 //   k_pm_green         * (-1/k^2) * 3/2 Omega_m H0^2 / (a n^3) / (cell volume * mean density)
 //   hipfftExecZ2D      half spectrum -> potential (unnormalised, like FFTW's)
@@ -16,11 +20,12 @@
 // linear, and the -1 only touches k = 0, which the Green's function zeroes.
 // The reference's gradient grid (3 n^3 doubles, written once and gathered
 // from) is not formed: each particle evaluates the same stencil expression
-// at its own 8 nodes, which gives the same doubles.  The deposit adds in no
-// fixed order (the reference's OpenMP loop adds without synchronisation, so
-// it has none either), and the scale factors are applied in the spectrum
-// instead of before and after the transforms: results agree with the
-// reference's to rounding.
+// at its own 8 nodes, which gives the same doubles.  Each cell's density is
+// summed in one fixed order (corners in the reference's order, particles by
+// index within a cell), not the reference's particle-by-particle order (its
+// OpenMP loop adds without synchronisation, so it has no fixed order), and
+// the scale factors are applied in the spectrum instead of before and after
+// the transforms: results agree with the reference's to rounding.
 #include <hipfft/hipfft.h>
 
 #include <cmath>
@@ -28,6 +33,8 @@
 #include <vector>
 
 #include "gs_internal.h"
+#include "octree.h"
+#include "scan.h"
 
 namespace gs {
 
@@ -54,12 +61,6 @@ __global__ void k_pm_wrap(double *__restrict__ x, size_t n3, double c0, double c
         const double f = fmod(p, w);
         x[q] = ((static_cast<int>(p / w) % 2 != 0) ? f - w : f) + c;
     }
-}
-
-__global__ void k_pm_clear(double *__restrict__ g, size_t count)
-{
-    const size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (q < count) g[q] = 0.0;
 }
 
 struct Cell {
@@ -109,21 +110,81 @@ __device__ __forceinline__ size_t node(const Cell &c, int dx, int dy, int dz, in
     return (a * N + b) * N + e;
 }
 
-__global__ void k_cic_deposit(const double *__restrict__ x, const double *__restrict__ m, int n,
-                              int N, double c0, double c1, double c2, double bw, double inv,
-                              double *__restrict__ grid)
+// Sort key of a particle: its base cell, wrapped into the grid (the cell
+// whose corner (0,0,0) the particle deposits to).
+__global__ void k_pm_cell_keys(const double *__restrict__ x, int n, int N, double c0, double c1,
+                               double c2, double bw, double inv, long long *__restrict__ key,
+                               int *__restrict__ idx)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const Cell c = cell_of(&x[3 * static_cast<size_t>(p)], c0, c1, c2, bw, inv);
-    const double mp = m[p];
+    key[p] = static_cast<long long>(node(c, 0, 0, 0, N));
+    idx[p] = p;
+}
+
+__global__ void k_pm_clear_ranges(int2 *__restrict__ r, size_t count)
+{
+    const size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (q < count) r[q] = make_int2(0, 0);
+}
+
+// [first, end) of each occupied cell's run in the sorted keys (other cells
+// keep the empty range [0, 0)).
+__global__ void k_pm_cell_ranges(const long long *__restrict__ key, int n, int2 *__restrict__ range)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long long k = key[p];
+    if (p == 0 || key[p - 1] != k) range[k].x = p;
+    if (p == n - 1 || key[p + 1] != k) range[k].y = p + 1;
+}
+
+// Weights and mass of each particle in sorted order: (wx, wy, wz, m), the
+// fractional offsets of cell_of (the weight of the far corner on each axis).
+__global__ void k_pm_sorted_weights(const double *__restrict__ x, const double *__restrict__ m,
+                                    const int *__restrict__ sidx, int n, double c0, double c1,
+                                    double c2, double bw, double inv, double4 *__restrict__ wm)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int p = sidx[t];
+    const Cell c = cell_of(&x[3 * static_cast<size_t>(p)], c0, c1, c2, bw, inv);
+    wm[t] = make_double4(c.w[0], c.w[1], c.w[2], m[p]);
+}
+
+// Cloud in cell (src/acceleration_PM.c:15-105) as a gather: the mass at
+// node q is the sum over the 8 corners d, in the reference's order, of the
+// particles whose base cell is q - d, each adding m * wx * wy * wz for that
+// corner (w or 1 - w per axis) -- the same product as the reference's for
+// that particle and node.  The 8 ranges are loaded before any particle.
+__global__ void k_cic_gather(const double4 *__restrict__ wm, const int2 *__restrict__ range, int N,
+                             double *__restrict__ grid)
+{
+    const unsigned n3 = static_cast<unsigned>(N) * N * N;
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n3) return;
+    const int a = static_cast<int>(q / (static_cast<unsigned>(N) * N));
+    const int b = static_cast<int>((q / N) % N);
+    const int e = static_cast<int>(q % N);
+    int2 r[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         int dx, dy, dz;
         corner(k, dx, dy, dz);
-        const double v = mp * (dx ? c.w[0] : c.wm[0]) * (dy ? c.w[1] : c.wm[1]) * (dz ? c.w[2] : c.wm[2]);
-        unsafeAtomicAdd(&grid[node(c, dx, dy, dz, N)], v);
+        r[k] = range[(wrap_index(a - dx, N) * N + wrap_index(b - dy, N)) * N + wrap_index(e - dz, N)];
     }
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int dx, dy, dz;
+        corner(k, dx, dy, dz);
+        for (int t = r[k].x; t < r[k].y; t++) {
+            const double4 w = wm[t];
+            sum += w.w * (dx ? w.x : 1.0 - w.x) * (dy ? w.y : 1.0 - w.y) * (dz ? w.z : 1.0 - w.z);
+        }
+    }
+    grid[q] = sum;
 }
 
 // Green's function of the Poisson equation on the half spectrum
@@ -246,8 +307,49 @@ struct PmWork {
     int N = 0;
     double *grid = nullptr;   // deposited density, then the potential (N^3)
     double2 *spec = nullptr;  // half spectrum N x N x (N/2 + 1)
+    int2 *range = nullptr;    // [first, end) of each cell's particles in sorted order (N^3)
+    int passes = 0;           // radix passes for keys below N^3
+    // (cell, particle) sort buffers for up to `cap` particles
+    int cap = 0;
+    long long *key[2] = {nullptr, nullptr};
+    int *idx[2] = {nullptr, nullptr};
+    double4 *wm = nullptr;    // (wx, wy, wz, m) in sorted order
+    unsigned *hist = nullptr, *stmp = nullptr;
     hipfftHandle r2c = 0, c2r = 0;
     bool have_r2c = false, have_c2r = false;
+
+    void free_sort()
+    {
+        for (int k = 0; k < 2; k++) {
+            if (key[k]) (void)hipFree(key[k]);
+            if (idx[k]) (void)hipFree(idx[k]);
+            key[k] = nullptr;
+            idx[k] = nullptr;
+        }
+        if (hist) (void)hipFree(hist);
+        if (stmp) (void)hipFree(stmp);
+        if (wm) (void)hipFree(wm);
+        hist = stmp = nullptr;
+        wm = nullptr;
+        cap = 0;
+    }
+
+    ErrorStatus ensure_particles(int n)
+    {
+        if (n <= cap) return make_success_error_status();
+        GS_HIP(hipStreamSynchronize(c->stream));
+        free_sort();
+        const size_t hn = radix_hist_elements(n);
+        for (int k = 0; k < 2; k++) {
+            GS_HIP(hipMalloc(reinterpret_cast<void **>(&key[k]), sizeof(long long) * n));
+            GS_HIP(hipMalloc(reinterpret_cast<void **>(&idx[k]), sizeof(int) * n));
+        }
+        GS_HIP(hipMalloc(reinterpret_cast<void **>(&wm), sizeof(double4) * n));
+        GS_HIP(hipMalloc(reinterpret_cast<void **>(&hist), sizeof(unsigned) * hn));
+        GS_HIP(hipMalloc(reinterpret_cast<void **>(&stmp), scan_temp_bytes(static_cast<long long>(hn)) + 64));
+        cap = n;
+        return make_success_error_status();
+    }
 
     ~PmWork()
     {
@@ -256,6 +358,8 @@ struct PmWork {
         if (have_c2r) (void)hipfftDestroy(c2r);
         if (grid) (void)hipFree(grid);
         if (spec) (void)hipFree(spec);
+        if (range) (void)hipFree(range);
+        free_sort();
     }
 
     ErrorStatus init(Context *ctx, int n)
@@ -266,6 +370,10 @@ struct PmWork {
         const size_t nh = static_cast<size_t>(n) * n * (n / 2 + 1);
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&grid), sizeof(double) * n3));
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&spec), sizeof(double2) * nh));
+        GS_HIP(hipMalloc(reinterpret_cast<void **>(&range), sizeof(int2) * n3));
+        int bits = 0;
+        while ((1ull << bits) < n3) bits++;
+        passes = bits > 8 ? (bits + 7) / 8 : 1;
         GS_FFT(hipfftPlan3d(&r2c, n, n, n, HIPFFT_D2Z));
         have_r2c = true;
         GS_FFT(hipfftPlan3d(&c2r, n, n, n, HIPFFT_Z2D));
@@ -314,10 +422,22 @@ ErrorStatus pm_acceleration(PmWork &W, const double *x, const double *m, int n,
     const double bw = box_length / 2.0;
     const double cell = box_length / N;
     const double inv = 1.0 / cell;
+    if (n <= 0) return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Number of particles must be positive. Got: %d", n);
+    GS_TRY(W.ensure_particles(n));
     int rec = timing_begin(c, "pm_deposit");
-    hipLaunchKernelGGL(k_pm_clear, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.grid, n3g);
-    hipLaunchKernelGGL(k_cic_deposit, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, m, n, N,
-                       center[0], center[1], center[2], bw, inv, W.grid);
+    hipLaunchKernelGGL(k_pm_cell_keys, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, n, N, center[0],
+                       center[1], center[2], bw, inv, W.key[0], W.idx[0]);
+    GS_HIP(hipGetLastError());
+    long long *skey;
+    int *sidx;
+    GS_TRY(radix_sort_pairs(c, W.key[0], W.idx[0], W.key[1], W.idx[1], n, W.passes, W.hist, W.stmp,
+                            &skey, &sidx));
+    hipLaunchKernelGGL(k_pm_clear_ranges, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.range, n3g);
+    hipLaunchKernelGGL(k_pm_cell_ranges, grid_for(n), dim3(PM_BLOCK), 0, c->stream, skey, n, W.range);
+    hipLaunchKernelGGL(k_pm_sorted_weights, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, m, sidx, n,
+                       center[0], center[1], center[2], bw, inv, W.wm);
+    hipLaunchKernelGGL(k_cic_gather, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.wm, W.range, N,
+                       W.grid);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     rec = timing_begin(c, "pm_fft");

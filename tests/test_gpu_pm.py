@@ -1,7 +1,7 @@
 """Particle-mesh gravity and the cosmological Leapfrog on the GPU, through
 the C-ABI, against the CPU restatement (oracle/pm.py).
 
-The deposit adds with FP64 atomics in no fixed order and the transforms are
+The deposit sums in another order than the oracle's and the transforms are
 hipFFT's, so forces agree with the oracle to rounding, not bitwise: the
 tolerance is 1e-10 of the largest acceleration.  The step arithmetic of the
 integrator is the reference's, so trajectories agree to the same order.
@@ -41,6 +41,16 @@ def test_acceleration_pm_matches_oracle(grid, center):
     # momentum: the total force vanishes
     tot = (m[:, None] * ag).sum(0)
     assert np.abs(tot).max() < 1e-11 * (m[:, None] * np.abs(ag)).sum()
+
+
+def test_acceleration_pm_is_bitwise_reproducible():
+    """The density is gathered per cell in a fixed order (no atomics), so
+    repeated evaluations give the same doubles."""
+    x, m = _clustered(50000, 5)
+    args = (x, m, (0.0, 0.0, 0.0), 1.0, 0.7, 0.3, m.sum() / 8.0, 64, 0.5)
+    a1 = L.acceleration_pm(*args)
+    a2 = L.acceleration_pm(*args)
+    assert np.array_equal(a1, a2)
 
 
 def test_point_mass_field_is_newtonian():
