@@ -157,34 +157,48 @@ __global__ void k_pm_sorted_weights(const double *__restrict__ x, const double *
 // node q is the sum over the 8 corners d, in the reference's order, of the
 // particles whose base cell is q - d, each adding m * wx * wy * wz for that
 // corner (w or 1 - w per axis) -- the same product as the reference's for
-// that particle and node.  The 8 ranges are loaded before any particle.
+// that particle and node.  A thread sums the nodes (a, b, 2j) and
+// (a, b, 2j + 1), whose corners share 4 of their base cells: 12 range
+// loads, all issued before any particle is read, and each node's sum in the
+// same order as alone.
 __global__ void k_cic_gather(const double4 *__restrict__ wm, const int2 *__restrict__ range, int N,
                              double *__restrict__ grid)
 {
-    const unsigned n3 = static_cast<unsigned>(N) * N * N;
+    const unsigned half = (static_cast<unsigned>(N) + 1) / 2;
     const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n3) return;
-    const int a = static_cast<int>(q / (static_cast<unsigned>(N) * N));
-    const int b = static_cast<int>((q / N) % N);
-    const int e = static_cast<int>(q % N);
-    int2 r[8];
+    if (q >= static_cast<unsigned>(N) * N * half) return;
+    const int a = static_cast<int>(q / (static_cast<unsigned>(N) * half));
+    const int b = static_cast<int>((q / half) % N);
+    const int e0 = 2 * static_cast<int>(q % half);
+    const bool two = e0 + 1 < N;
+    // r[dx + 2 dy][z]: base cells (a - dx, b - dy, e0 - 1 + z), z = 0, 1, 2
+    int2 r[4][3];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        int dx, dy, dz;
-        corner(k, dx, dy, dz);
-        r[k] = range[(wrap_index(a - dx, N) * N + wrap_index(b - dy, N)) * N + wrap_index(e - dz, N)];
+    for (int u = 0; u < 4; u++) {
+        const size_t row = (wrap_index(a - (u & 1), N) * N + wrap_index(b - (u >> 1), N)) * N;
+        r[u][0] = range[row + wrap_index(e0 - 1, N)];
+        r[u][1] = range[row + static_cast<size_t>(e0)];
+        r[u][2] = two ? range[row + static_cast<size_t>(e0 + 1)] : make_int2(0, 0);
     }
-    double sum = 0.0;
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         int dx, dy, dz;
         corner(k, dx, dy, dz);
-        for (int t = r[k].x; t < r[k].y; t++) {
+        const int u = dx + 2 * dy;
+        const int2 r0 = r[u][1 - dz], r1 = r[u][2 - dz];  // bases of e0 - dz, e0 + 1 - dz
+        for (int t = r0.x; t < r0.y; t++) {
             const double4 w = wm[t];
-            sum += w.w * (dx ? w.x : 1.0 - w.x) * (dy ? w.y : 1.0 - w.y) * (dz ? w.z : 1.0 - w.z);
+            s0 += w.w * (dx ? w.x : 1.0 - w.x) * (dy ? w.y : 1.0 - w.y) * (dz ? w.z : 1.0 - w.z);
+        }
+        for (int t = r1.x; t < r1.y; t++) {
+            const double4 w = wm[t];
+            s1 += w.w * (dx ? w.x : 1.0 - w.x) * (dy ? w.y : 1.0 - w.y) * (dz ? w.z : 1.0 - w.z);
         }
     }
-    grid[q] = sum;
+    const size_t q0 = (static_cast<size_t>(a) * N + b) * N + e0;
+    grid[q0] = s0;
+    if (two) grid[q0 + 1] = s1;
 }
 
 // Green's function of the Poisson equation on the half spectrum
@@ -436,8 +450,8 @@ ErrorStatus pm_acceleration(PmWork &W, const double *x, const double *m, int n,
     hipLaunchKernelGGL(k_pm_cell_ranges, grid_for(n), dim3(PM_BLOCK), 0, c->stream, skey, n, W.range);
     hipLaunchKernelGGL(k_pm_sorted_weights, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, m, sidx, n,
                        center[0], center[1], center[2], bw, inv, W.wm);
-    hipLaunchKernelGGL(k_cic_gather, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.wm, W.range, N,
-                       W.grid);
+    hipLaunchKernelGGL(k_cic_gather, grid_for(static_cast<size_t>(N) * N * ((N + 1) / 2)), dim3(PM_BLOCK), 0,
+                       c->stream, W.wm, W.range, N, W.grid);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     rec = timing_begin(c, "pm_fft");
