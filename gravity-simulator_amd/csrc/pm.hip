@@ -226,13 +226,16 @@ __global__ void k_pm_green(double2 *__restrict__ spec, int N, double two_pi_l2, 
 // same 4th-order expression per node and axis) and the CIC gather of those
 // values (:107-210), one particle per lane.  Along each axis the stencils of
 // the two nodes share one line of 6 potential values.
-__global__ void k_cic_grad_interp(const double *__restrict__ x, int n, int N, double c0,
-                                  double c1, double c2, double bw, double inv,
+__global__ void k_cic_grad_interp(const double *__restrict__ x, const int *__restrict__ sidx, int n,
+                                  int N, double c0, double c1, double c2, double bw, double inv,
                                   const double *__restrict__ phi, double twelve_h,
                                   double *__restrict__ a)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 >= n) return;
+    // particles in the deposit's cell order: neighbouring threads read
+    // neighbouring stencil points whatever the input order
+    const int p = sidx[t0];
     const Cell c = cell_of(&x[3 * static_cast<size_t>(p)], c0, c1, c2, bw, inv);
     const size_t N2 = static_cast<size_t>(N) * N;
     size_t li[6], lj[6], lk[6];  // node indices i-2 .. i+3 along each axis
@@ -465,7 +468,7 @@ ErrorStatus pm_acceleration(PmWork &W, const double *x, const double *m, int n,
     GS_FFT(hipfftExecZ2D(W.c2r, reinterpret_cast<hipfftDoubleComplex *>(W.spec), W.grid));
     timing_end(c, rec);
     rec = timing_begin(c, "pm_interp");
-    hipLaunchKernelGGL(k_cic_grad_interp, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, n, N,
+    hipLaunchKernelGGL(k_cic_grad_interp, grid_for(n), dim3(PM_BLOCK), 0, c->stream, x, sidx, n, N,
                        center[0], center[1], center[2], bw, inv, W.grid, 12.0 * cell, a);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
