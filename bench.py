@@ -315,13 +315,18 @@ def pm_lattice(nside, box, seed):
     return x, np.full(len(x), 1.0 / len(x))
 
 
-def run_pm(L, evals=5, cpu=True):
+def run_pm(L, evals=5, cpu=True, shuffle=False):
     """Particle mesh: one force evaluation for 128^3 particles on a 256^3 grid
     (per-kernel times over `evals` evaluations), 50 steps of the
     cosmological Leapfrog (LambdaCDM, a = 0.1 -> 0.2), and the numpy
-    restatement on a 64^3-particle, 64^3-grid sample as the CPU baseline."""
+    restatement on a 64^3-particle, 64^3-grid sample as the CPU baseline.
+    shuffle=True permutes the particles randomly (the arrays' order, not the
+    positions)."""
     nside, grid, box = 128, 256, 100.0
     x, m = pm_lattice(nside, box, 13)
+    if shuffle:
+        perm = np.random.default_rng(1).permutation(len(m))
+        x, m = x[perm], m[perm]
     c, bw = np.full(3, box / 2), box / 2
     rho = m.sum() / box ** 3
     H0, om, a = 70.0, 0.3, 0.5
@@ -405,6 +410,8 @@ def main():
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--no-whfast", action="store_true")
     ap.add_argument("--no-pm", action="store_true", help="skip the particle-mesh leg")
+    ap.add_argument("--pm-shuffle", action="store_true",
+                    help="particle-mesh leg with the lattice's particles in random order")
     ap.add_argument("--no-learned", action="store_true", help="Barnes-Hut without the learned correction")
     ap.add_argument("--no-integrators", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -486,7 +493,7 @@ def main():
         out["whfast"] = run_whfast(L, ic, args.whfast_steps)
         log(f"whfast: {out['whfast']['particle_steps_per_s']:.3e} particle-steps/s")
     if not args.no_pm:
-        out["pm"] = run_pm(L, cpu=not args.no_cpu)
+        out["pm"] = run_pm(L, cpu=not args.no_cpu, shuffle=args.pm_shuffle)
         log(f"pm: {out['pm']['particles_per_s']:.3e} particles/s ({out['pm']['device_ms_per_eval']:.3f} ms per "
             f"evaluation), cosmological Leapfrog {out['pm']['cosmology_leapfrog']['ms_per_step']:.3f} ms/step")
     if not args.no_cpu:
