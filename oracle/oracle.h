@@ -85,7 +85,8 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
 
 /* The walk of groups of at most `group` consecutive sorted particles (the
  * children of at most `group` particles of each node with more, packed in
- * order): a node is accepted for the whole group when it holds none of the
+ * order; a run wider than 4 width (64/n)^(1/3) in some axis is split into
+ * single particles): a node is accepted for the whole group when it holds none of the
  * group's particles and (l)^2 < theta^2 d^2 with d the distance from its
  * centre of mass to the group's bounding box; group = 1 is orc_bh_walk
  * (without features). */

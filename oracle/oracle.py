@@ -236,8 +236,9 @@ def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, fea
 
 
 def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None):
-    """Group walk (groups of `group` consecutive sorted particles); returns
-    (a, per-particle interaction counts).  group=1 is bh_walk."""
+    """Group walk (groups of at most `group` consecutive sorted particles,
+    formed from the tree as orc_bh_walk_group describes); returns (a,
+    per-particle interaction counts).  group=1 is bh_walk."""
     x, m = _c(x), _c(m)
     n = len(m)
     a = np.zeros_like(x)
