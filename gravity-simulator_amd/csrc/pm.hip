@@ -46,21 +46,27 @@ inline dim3 grid_for(size_t work) { return dim3(static_cast<unsigned>((work + PM
 
 // Periodic boundary conditions, the reference's fmod / parity formula
 // (src/system.c:1337-1380).
+__device__ __forceinline__ double wrap_coord(double xv, double c, double w)
+{
+    const double p = xv - c;
+    if (p < -w) {
+        const double f = fmod(p, w);
+        return ((static_cast<int>(p / w) % 2 != 0) ? f + w : f) + c;
+    }
+    if (p > w) {
+        const double f = fmod(p, w);
+        return ((static_cast<int>(p / w) % 2 != 0) ? f - w : f) + c;
+    }
+    return xv;
+}
+
 __global__ void k_pm_wrap(double *__restrict__ x, size_t n3, double c0, double c1, double c2,
                           double w)
 {
     const size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (q >= n3) return;
     const int d = static_cast<int>(q % 3);
-    const double c = d == 0 ? c0 : (d == 1 ? c1 : c2);
-    const double p = x[q] - c;
-    if (p < -w) {
-        const double f = fmod(p, w);
-        x[q] = ((static_cast<int>(p / w) % 2 != 0) ? f + w : f) + c;
-    } else if (p > w) {
-        const double f = fmod(p, w);
-        x[q] = ((static_cast<int>(p / w) % 2 != 0) ? f - w : f) + c;
-    }
+    x[q] = wrap_coord(x[q], d == 0 ? c0 : (d == 1 ? c1 : c2), w);
 }
 
 struct Cell {
@@ -297,11 +303,25 @@ __global__ void k_cos_kick(double *__restrict__ p, const double *__restrict__ a,
     if (q < n3) p[q] -= half_dt * a[q] / da;
 }
 
-__global__ void k_cos_drift(double *__restrict__ x, const double *__restrict__ p, double dt,
-                            double denom, size_t n3)
+// One pass per step of the cosmological Leapfrog
+// (src/integrator_cosmology_leapfrog.c:150-200): the previous step's closing
+// kick (when `pend`), this step's opening kick, the drift and the periodic
+// wrap, each with the same operations as the separate passes.
+__global__ void k_cos_kick_drift_wrap(double *__restrict__ p, double *__restrict__ x,
+                                      const double *__restrict__ a, int pend, double half_prev,
+                                      double da_prev, double half_dt, double da, double dt,
+                                      double denom, size_t n3, double c0, double c1, double c2,
+                                      double w)
 {
     const size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-    if (q < n3) x[q] += dt * p[q] / denom;
+    if (q >= n3) return;
+    const double aq = a[q];
+    double pq = p[q];
+    if (pend) pq -= half_prev * aq / da_prev;
+    pq -= half_dt * aq / da;
+    p[q] = pq;
+    const int d = static_cast<int>(q % 3);
+    x[q] = wrap_coord(x[q] + dt * pq / denom, d == 0 ? c0 : (d == 1 ? c1 : c2), w);
 }
 
 ErrorStatus fft_error(hipfftResult r, const char *what)
@@ -735,25 +755,34 @@ ErrorStatus leapfrog_cosmology(CosmologicalSystem *system, OutputParam *op,
         dt > 0.0 ? static_cast<long long>(ceil((a_final - a_begin) / dt)) : 0;
     status->dt = dt;
     status->num_steps = 0;
+    // a step's closing kick is applied in the next step's first pass (or
+    // after the loop)
+    int pend = 0;
+    double half_prev = 0.0, da_prev = 1.0;
     while (status->num_steps < total) {
         if (t + dt > a_final) dt = a_final - t;
         status->dt = dt;
-        double da = compute_da(t, H0, om, ol);
-        hipLaunchKernelGGL(k_cos_kick, g3, blk, 0, c->stream, p, a, 0.5 * dt, da, n3);
+        const double da0 = compute_da(t, H0, om, ol);
         t += 0.5 * dt;
-        da = compute_da(t, H0, om, ol);
-        hipLaunchKernelGGL(k_cos_drift, g3, blk, 0, c->stream, x, p, dt, (t * t) * da, n3);
+        const double da = compute_da(t, H0, om, ol);
+        hipLaunchKernelGGL(k_cos_kick_drift_wrap, g3, blk, 0, c->stream, p, x, a, pend, half_prev,
+                           da_prev, 0.5 * dt, da0, dt, (t * t) * da, n3, ctr[0], ctr[1], ctr[2],
+                           system->box_width);
         GS_HIP(hipGetLastError());
-        GS_TRY(pm_wrap(c, x, n, ctr, system->box_width));
         GS_TRY(pm_acceleration(*W, x, m, n, ctr, system->box_width, H0, om, mean_bkg_density, t, a));
-        hipLaunchKernelGGL(k_cos_kick, g3, blk, 0, c->stream, p, a, 0.5 * dt, da, n3);
-        GS_HIP(hipGetLastError());
+        pend = 1;
+        half_prev = 0.5 * dt;
+        da_prev = da;
         status->num_steps++;
         t = a_begin + status->num_steps * dt;
         status->t = t;
         if (settings->is_exit_ptr && *settings->is_exit_ptr) break;
         // keep the host a bounded number of steps ahead of the device
         if ((status->num_steps & 15) == 0) GS_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (pend) {
+        hipLaunchKernelGGL(k_cos_kick, g3, blk, 0, c->stream, p, a, half_prev, da_prev, n3);
+        GS_HIP(hipGetLastError());
     }
     GS_HIP(hipMemcpyAsync(system->x, x, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));
