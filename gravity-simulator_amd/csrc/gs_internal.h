@@ -51,7 +51,7 @@ enum Slot : int {
     SLOT_PARTIAL,
     // octree
     SLOT_BBOX, SLOT_KEYS, SLOT_KEYS_ALT, SLOT_IDX, SLOT_IDX_ALT, SLOT_HIST, SLOT_HIST_SCAN,
-    SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_SORTED_P, SLOT_MISC,
+    SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_SORTED_P, SLOT_MISC, SLOT_GROUP_PARENTS,
     // learned correction
     SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD, SLOT_MLP_X, SLOT_SPOT,
     SLOT_COUNT

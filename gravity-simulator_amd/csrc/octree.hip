@@ -923,54 +923,81 @@ __device__ __forceinline__ void quad_termf(const float *__restrict__ q, float r0
 // run whose box is wider than `cap` in some axis (a few times the width of a
 // 64-particle group at the mean density) gives one group per particle --
 // sparse outliers, for which that is the per-particle walk.
-__device__ void emit_group(const double4 *__restrict__ P, int f, int c, double cap,
-                           int2 *__restrict__ groups, unsigned *__restrict__ ngroups)
+// The nodes that make groups: the root, and every node of more than 64
+// particles (flag pass, scan, then the list at the scanned offsets).
+__global__ void k_group_parent_flags(const NodeRec *__restrict__ nodes, int num_nodes,
+                                     unsigned *__restrict__ flag)
 {
-    double lo0 = P[f].x, lo1 = P[f].y, lo2 = P[f].z, hi0 = lo0, hi1 = lo1, hi2 = lo2;
-    for (int q = f + 1; q < f + c; q++) {
-        const double4 p = P[q];
-        lo0 = fmin(lo0, p.x), lo1 = fmin(lo1, p.y), lo2 = fmin(lo2, p.z);
-        hi0 = fmax(hi0, p.x), hi1 = fmax(hi1, p.y), hi2 = fmax(hi2, p.z);
-    }
-    if (hi0 - lo0 > cap || hi1 - lo1 > cap || hi2 - lo2 > cap) {
-        for (int q = f; q < f + c; q++) groups[atomicAdd(ngroups, 1u)] = make_int2(q, 1);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < num_nodes) flag[p] = (p == 0 || nodes[p].count > 64) ? 1u : 0u;
+}
+__global__ void k_group_parent_list(const NodeRec *__restrict__ nodes, int num_nodes,
+                                    const unsigned *__restrict__ off, int *__restrict__ parents)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < num_nodes && (p == 0 || nodes[p].count > 64)) parents[off[p]] = p;
+}
+
+// One wavefront per parent node: its runs are formed in the same order by
+// every lane; each run's particles are loaded one per lane and its box is
+// reduced across the wave.  Groups are counted in a first launch (WRITE =
+// false: count[i] := their number) and written in a second from the
+// exclusive scan of the counts (WRITE = true: groups[off[i] ...]).
+template <bool WRITE>
+__device__ void emit_group(const double4 *__restrict__ P, int f, int c, double cap, int lane,
+                           int2 *__restrict__ groups, unsigned &at)
+{
+    const double4 q = P[f + (lane < c ? lane : 0)];
+    const double w0 = wave_max(q.x) - wave_min(q.x);
+    const double w1 = wave_max(q.y) - wave_min(q.y);
+    const double w2 = wave_max(q.z) - wave_min(q.z);
+    if (w0 > cap || w1 > cap || w2 > cap) {
+        if (WRITE && lane < c) groups[at + lane] = make_int2(f + lane, 1);
+        at += c;
     } else {
-        groups[atomicAdd(ngroups, 1u)] = make_int2(f, c);
+        if (WRITE && lane == 0) groups[at] = make_int2(f, c);
+        at += 1;
     }
 }
 
-__global__ void k_group_list(const NodeRec *__restrict__ nodes, int num_nodes, int n,
-                             const double4 *__restrict__ P, double cap,
-                             int2 *__restrict__ groups, unsigned *__restrict__ ngroups)
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_group_runs(const NodeRec *__restrict__ nodes,
+                                                    const int *__restrict__ parents, int num_parents,
+                                                    int n, const double4 *__restrict__ P, double cap,
+                                                    unsigned *__restrict__ count_or_off,
+                                                    int2 *__restrict__ groups)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= num_nodes) return;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (i >= num_parents) return;
+    const int p = parents[i];
+    unsigned at = WRITE ? count_or_off[i] : 0u;
     const NodeRec nd = nodes[p];
-    if (p == 0 && nd.count <= 64) {
-        groups[atomicAdd(ngroups, 1u)] = make_int2(0, n);
-        return;
-    }
-    if (nd.count <= 64) return;
-    if (nd.nchild == 0) {
+    if (nd.count <= 64) {
+        // (the root of at most 64 particles: one group)
+        if (WRITE && lane == 0) groups[at] = make_int2(0, n);
+        at += 1;
+    } else if (nd.nchild == 0) {
         // a leaf of more than 64 particles: consecutive runs of 64
         for (int f = nd.first; f < nd.first + nd.count; f += 64)
-            emit_group(P, f, min(64, nd.first + nd.count - f), cap, groups, ngroups);
-        return;
-    }
-    int start = -1, cnt = 0;
-    for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
-        const int cf = nodes[c].first, cc = nodes[c].count;
-        if (cc > 64 || cnt + cc > 64) {
-            if (cnt > 0) emit_group(P, start, cnt, cap, groups, ngroups);
-            start = -1;
-            cnt = 0;
+            emit_group<WRITE>(P, f, min(64, nd.first + nd.count - f), cap, lane, groups, at);
+    } else {
+        int start = -1, cnt = 0;
+        for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
+            const int cf = nodes[c].first, cc = nodes[c].count;
+            if (cc > 64 || cnt + cc > 64) {
+                if (cnt > 0) emit_group<WRITE>(P, start, cnt, cap, lane, groups, at);
+                start = -1;
+                cnt = 0;
+            }
+            if (cc <= 64) {
+                if (cnt == 0) start = cf;
+                cnt += cc;
+            }
         }
-        if (cc <= 64) {
-            if (cnt == 0) start = cf;
-            cnt += cc;
-        }
+        if (cnt > 0) emit_group<WRITE>(P, start, cnt, cap, lane, groups, at);
     }
-    if (cnt > 0) emit_group(P, start, cnt, cap, groups, ngroups);
+    if (!WRITE && lane == 0) count_or_off[i] = at;
 }
 
 template <bool QUAD>
@@ -1562,22 +1589,47 @@ ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, dou
     GS_TRY(slot(c, SLOT_MISC, 256, &p));
     unsigned long long *cnt = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
     GS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
-    // groups: at most one per particle
+    // groups: at most one per particle (they partition the particles)
     void *gp;
     GS_TRY(slot(c, SLOT_NODE_TMP, sizeof(int2) * static_cast<size_t>(t->n) + 256, &gp));
     int2 *groups = static_cast<int2 *>(gp);
     unsigned *ng = reinterpret_cast<unsigned *>(static_cast<char *>(p) + 128);
-    GS_HIP(hipMemsetAsync(ng, 0, sizeof(unsigned), c->stream));
+    // parent list and per-parent group counts, scanned into offsets (the
+    // sort's scratch slots; the parent list in the groups' slot's tail)
+    void *hp;
+    GS_TRY(slot(c, SLOT_HIST, sizeof(unsigned) * static_cast<size_t>(t->num_nodes) + 256, &hp));
+    unsigned *offs = static_cast<unsigned *>(hp);
+    GS_TRY(slot(c, SLOT_HIST_SCAN, scan_temp_bytes(t->num_nodes) + 64, &hp));
+    unsigned *stmp = static_cast<unsigned *>(hp);
+    GS_TRY(slot(c, SLOT_GROUP_PARENTS, sizeof(int) * static_cast<size_t>(t->num_nodes) + 256, &hp));
+    int *parents = static_cast<int *>(hp);
     const int rec = timing_begin(c, "bh_walk");
     const double cap = group_width_cap(t->width, t->n);
-    hipLaunchKernelGGL(k_group_list, dim3((t->num_nodes + 255) / 256), dim3(256), 0, c->stream,
-                       t->nodes, t->num_nodes, t->n, t->P, cap, groups, ng);
+    const dim3 ngrid((t->num_nodes + 255) / 256);
+    hipLaunchKernelGGL(k_group_parent_flags, ngrid, dim3(256), 0, c->stream, t->nodes, t->num_nodes, offs);
     GS_HIP(hipGetLastError());
+    GS_HIP(scan_exclusive(c->stream, offs, offs, t->num_nodes, stmp, ng));
+    hipLaunchKernelGGL(k_group_parent_list, ngrid, dim3(256), 0, c->stream, t->nodes, t->num_nodes, offs,
+                       parents);
+    GS_HIP(hipGetLastError());
+    unsigned num_parents = 0;
+    GS_HIP(hipMemcpyAsync(&num_parents, ng, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    if (num_parents > static_cast<unsigned>(t->num_nodes))
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group parents beyond the nodes");
+    const dim3 pgrid((num_parents + 3) / 4);  // 4 wavefronts per workgroup
+    hipLaunchKernelGGL(k_group_runs<false>, pgrid, dim3(256), 0, c->stream, t->nodes, parents,
+                       static_cast<int>(num_parents), t->n, t->P, cap, offs, groups);
+    GS_HIP(hipGetLastError());
+    GS_HIP(scan_exclusive(c->stream, offs, offs, num_parents, stmp, ng));
     unsigned num_groups = 0;
     GS_HIP(hipMemcpyAsync(&num_groups, ng, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));
     if (num_groups > static_cast<unsigned>(t->n))
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group list larger than the particles");
+    hipLaunchKernelGGL(k_group_runs<true>, pgrid, dim3(256), 0, c->stream, t->nodes, parents,
+                       static_cast<int>(num_parents), t->n, t->P, cap, offs, groups);
+    GS_HIP(hipGetLastError());
     const int grid = (static_cast<int>(num_groups) + GW_WAVES - 1) / GW_WAVES;
     if (order >= 2)
         hipLaunchKernelGGL((k_walk_group<true>), dim3(grid), dim3(GW_BLOCK), 0, c->stream, t->nodes,
