@@ -860,7 +860,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
 
 // ------------------------------------------------------------------ group walk
 // One wavefront walks the tree for a group of at most 64 consecutive
-// particles in sorted order (formed from the tree by k_group_list, below): a
+// particles in sorted order (formed from the tree by k_group_runs, below): a
 // node is accepted for the whole group when it holds none
 // of the group's particles and l^2 < theta^2 d^2, d the distance from its
 // centre of mass to the group's bounding box -- a test every particle of the
