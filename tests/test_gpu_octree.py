@@ -245,3 +245,18 @@ def test_group_walk_error_below_per_particle_walk_config4():
     assert group[1] < plain[1] and group[0] < plain[0]
     assert gq[1] < group[1]
     sim.close()
+
+
+def test_group_walk_interaction_set_at_full_size_config4():
+    """N=2^20 two-sphere (seed 4), theta 0.5: the GPU group walk's interaction
+    count equals the oracle's exactly (same groups, same accepted nodes), and
+    its forces agree with the oracle's FP64 sums on a sample to FP32 rounding."""
+    x, v, m = ic.two_spheres(1 << 20, 4)
+    t = O.build_tree(x, m, 1)
+    a_o, cnt = O.bh_walk_group(x, m, t, 0.5, 64, eps=1e-3)
+    print("oracle group walk done", int(cnt.sum()), flush=True)
+    a_g = L.acceleration(x, m, 20, eps=1e-3, theta=0.5, leaf=1)
+    assert L.last_walk_interactions() == int(cnt.sum())
+    idx = np.random.default_rng(4).choice(len(m), 20000, replace=False)
+    e = np.linalg.norm(a_g[idx] - a_o[idx], axis=1) / np.linalg.norm(a_o[idx], axis=1)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
