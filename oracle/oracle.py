@@ -19,9 +19,13 @@ tests/test_oracle.py, beyond the compiled reference where it builds):
 * ``octree``     node set == reference's internal-node set (depth, prefix, count,
                  first index); internal masses/COMs within 1e-14 of the reference;
                  leaf moments == direct sums in long double.
-* ``bh_walk``    theta=0 gives the direct sum; error vs direct sum decreases with
-                 theta; reference walk output is NOT a pin (leaf mass dropped and
-                 key mis-indexing, src/linear_octree.c:563-570,
+* ``bh_walk``    run on the reference's own tree over Morton-ordered input,
+                 bitwise == reference acceleration_barnes_hut (golden F7 and
+                 the compiled reference; theta doubled or halved, i.e. a cell
+                 side or depth slip, fails); theta=0 gives the direct sum;
+                 error vs direct sum decreases with theta.  On other inputs
+                 the reference's walk is not a pin (leaf mass dropped and key
+                 mis-indexing, src/linear_octree.c:563-570,
                  src/acceleration_barnes_hut.c:119-147).
 * ``leapfrog``   bitwise == reference launch_simulation(LEAPFROG) final state;
                  time reversibility.

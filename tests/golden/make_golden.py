@@ -18,6 +18,17 @@ exported functions on seeded inputs and stores inputs and outputs as data:
                     (seed 5) two-sphere inputs
   f4_octree.npz     N=4096 two-sphere internal-node set and monopoles for leaf
                     capacity 1 and 8 (src/linear_octree.c:590-823)
+  f7_bh_walk.npz    N=2048 two-sphere (seed 7) put in Morton order first, so
+                    the reference's sorted_indices is the identity and its
+                    walk's key look-up (src/acceleration_barnes_hut.c:119-147)
+                    reads the right keys: reference construct_octree node
+                    arrays for leaf capacity 1 and 8 (leaves without mass or
+                    centre of mass, src/linear_octree.c:563-570,662-678) and
+                    reference acceleration_barnes_hut at theta 0.3, 0.5, 0.8
+                    (eps 0) and theta 0.5 with eps 1e-2 for each capacity;
+                    plus, as a characterisation of the two quirks, the same
+                    particles in generator order with the reference output
+                    at theta 0.5, leaf 1 (src/acceleration_barnes_hut.c:33-248)
   f5_leapfrog.npz   N=1024 Plummer (seed 1), eps=0, dt=1e-3, 1000 steps:
                     reference launch_simulation(LEAPFROG) final x, v and
                     energies (src/integrator.c:894-1121, src/utils.c:27-59)
@@ -108,6 +119,26 @@ def f3_f4():
             t[f"cap{cap}_{k}"] = r[k][internal]
         t[f"cap{cap}_num_nodes"] = len(r["depth"])
     np.savez_compressed(os.path.join(HERE, "f4_octree.npz"), x=x, m=m, **t)
+
+
+F7_CASES = ((0.3, 0.0), (0.5, 0.0), (0.8, 0.0), (0.5, 1e-2))
+
+
+def f7():
+    x0, v0, m0 = ic.two_spheres(2048, 7)
+    perm = O.build_tree(x0, m0, 1)["sorted_idx"]
+    x, m = np.ascontiguousarray(x0[perm]), np.ascontiguousarray(m0[perm])
+    out = dict(x=x, m=m, x_generator_order=x0, m_generator_order=m0)
+    for cap in (1, 8):
+        r = O.ref_octree(x, m, cap)
+        assert np.array_equal(r["sorted_idx"], np.arange(len(m))), "presorted input must sort to the identity"
+        for k in ("width", "sorted_keys", "depth", "prefix", "count", "first", "nchild", "child0", "mass",
+                  "comx", "comy", "comz"):
+            out[f"cap{cap}_{k}"] = r[k]
+        for theta, eps in F7_CASES:
+            out[f"cap{cap}_a_theta{theta}_eps{eps}"] = O.ref_acceleration(x, m, 3, eps=eps, theta=theta, leaf=cap)
+    out["a_generator_order_theta0.5"] = O.ref_acceleration(x0, m0, 3, eps=0.0, theta=0.5, leaf=1)
+    np.savez_compressed(os.path.join(HERE, "f7_bh_walk.npz"), **out)
 
 
 def f5():
@@ -284,7 +315,7 @@ def f10():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f8", "f9", "f10"]
+    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f7", "f8", "f9", "f10"]
     for w in which:
-        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f8": f8, "f9": f9, "f10": f10}[w]()
+        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f7": f7, "f8": f8, "f9": f9, "f10": f10}[w]()
         print("wrote", w, flush=True)

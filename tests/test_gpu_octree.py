@@ -146,7 +146,10 @@ def test_bh_config4_error_vs_fp64_direct():
     med, p99, mx = sim.accel_error(L.param(3, theta=0.5, leaf=1), samples=4096)
     print(f"BH monopole theta=0.5 N=2^20: median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
     # monopole Barnes-Hut at theta=0.5 (the reference's multipole order)
-    assert med < 5e-3 and p99 < 3e-2
+    # measures median 6.7e-4, p99 4.1e-3 on this input: the bounds are 1.5x
+    # those.  A monopole walk cannot reach p99 1e-3 at theta=0.5 here (the
+    # quadrupole and group walks do, at theta 0.5 and 0.4).
+    assert med < 1.0e-3 and p99 < 6.2e-3
     sim.close()
 
 

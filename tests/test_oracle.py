@@ -215,6 +215,91 @@ def test_bh_error_decreases_with_theta():
     assert p99[0] < p99[1] < p99[2] < 0.2
 
 
+F7_CASES = ((0.3, 0.0), (0.5, 0.0), (0.8, 0.0), (0.5, 1e-2))
+
+
+def _f7_tree(f, cap):
+    """The reference's own node arrays from fixture F7 (leaves carry no mass
+    or centre of mass, src/linear_octree.c:563-570,662-678)."""
+    t = {k[len(f"cap{cap}_"):]: f[k] for k in f.files if k.startswith(f"cap{cap}_") and "_a_" not in k}
+    t["width"] = float(t["width"])
+    t["sorted_idx"] = np.arange(len(f["m"]), dtype=np.int32)
+    return t
+
+
+@pytest.mark.parametrize("cap", [1, 8])
+def test_bh_walk_equals_reference_walk_golden(golden, cap):
+    """The oracle's walk (opening rule, cell side per depth, inclusion test,
+    child order, leaf sums with the self skip and the force arithmetic) run
+    on the reference's tree equals the reference's acceleration_barnes_hut
+    bit for bit (src/acceleration_barnes_hut.c:78-248).  The input is in
+    Morton order, so the reference's key look-up by original index reads the
+    right keys; the reference's leaves are massless, so accepted leaves add
+    zero on both sides.  A cell side off by a factor of 2 or a depth off by
+    one is the same as theta doubled or halved: both fail."""
+    f = golden("f7_bh_walk.npz")
+    x, m = f["x"], f["m"]
+    t = _f7_tree(f, cap)
+    for theta, eps in F7_CASES:
+        a_ref = f[f"cap{cap}_a_theta{theta}_eps{eps}"]
+        assert np.array_equal(O.bh_walk(x, m, t, theta, eps=eps), a_ref), (cap, theta, eps)
+        for slip in (2.0, 0.5):
+            assert not np.array_equal(O.bh_walk(x, m, t, theta * slip, eps=eps), a_ref), (cap, theta, slip)
+
+
+def test_bh_walk_oracle_tree_differs_from_reference_only_by_leaf_monopoles(golden):
+    """On the oracle's own tree (leaves carry their mass and centre of mass)
+    the walk differs from the reference's by the accepted leaves' terms: with
+    those leaves' masses set to zero the results agree with the reference to
+    the rounding of the centres of mass (the two builds sum the moments in
+    different orders), and with them the error against the direct sum drops
+    (src/linear_octree.c:662-678 leaves the leaves' monopoles out)."""
+    f = golden("f7_bh_walk.npz")
+    x, m = f["x"], f["m"]
+    t = O.build_tree(x, m, 1)
+    assert np.array_equal(t["sorted_idx"], np.arange(len(m)))
+    q1 = dict(t)
+    leaf = t["nchild"] == 0
+    leaf[0] = False
+    for k in ("mass", "comx", "comy", "comz"):
+        q1[k] = np.where(leaf, 0.0, t[k])
+    a_ref = f["cap1_a_theta0.5_eps0.0"]
+    a_q1 = O.bh_walk(x, m, q1, 0.5)
+    scale = np.linalg.norm(a_ref, axis=1)
+    assert np.max(np.linalg.norm(a_q1 - a_ref, axis=1) / scale) < 1e-9
+    a_d = O.pairwise(x, m)
+    an = np.linalg.norm(a_d, axis=1)
+    e_own = np.percentile(np.linalg.norm(O.bh_walk(x, m, t, 0.5) - a_d, axis=1) / an, 99)
+    e_ref = np.percentile(np.linalg.norm(a_ref - a_d, axis=1) / an, 99)
+    assert e_own < 0.2 * e_ref
+
+
+def test_bh_reference_walk_depends_on_input_order(golden):
+    """Characterisation of src/acceleration_barnes_hut.c:119-147: the walk
+    reads the sorted key array with original particle indices, so the same
+    particles in generator order get other forces than in Morton order."""
+    f = golden("f7_bh_walk.npz")
+    x0 = f["x_generator_order"]
+    t = O.build_tree(x0, f["m_generator_order"], 1)
+    a_sorted_back = np.empty_like(x0)
+    a_sorted_back[t["sorted_idx"]] = f["cap1_a_theta0.5_eps0.0"]
+    d = np.linalg.norm(f["a_generator_order_theta0.5"] - a_sorted_back, axis=1) / np.linalg.norm(a_sorted_back, axis=1)
+    assert np.median(d) > 1e-3
+
+
+@ref_only
+@pytest.mark.parametrize("theta,cap", [(0.4, 1), (0.7, 3), (1.0, 16)])
+def test_bh_walk_equals_reference_build(theta, cap):
+    """As the golden test, on fresh inputs through the compiled reference."""
+    x0, v0, m0 = ic.uniform_cube(3000, 41)
+    perm = O.build_tree(x0, m0, 1)["sorted_idx"]
+    x, m = np.ascontiguousarray(x0[perm]), np.ascontiguousarray(m0[perm])
+    r = O.ref_octree(x, m, cap)
+    assert np.array_equal(r["sorted_idx"], np.arange(len(m)))
+    a_ref = O.ref_acceleration(x, m, 3, eps=1e-3, theta=theta, leaf=cap)
+    assert np.array_equal(O.bh_walk(x, m, r, theta, eps=1e-3), a_ref)
+
+
 # ---------------------------------------------------------------- integrators
 def test_leapfrog_golden_bitwise(golden):
     f = golden("f5_leapfrog.npz")
