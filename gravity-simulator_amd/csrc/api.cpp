@@ -648,8 +648,25 @@ ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs)
 }
 
 // ---------------------------------------------------------------- FP32 autotuner
-static const int kShapes[][2] = {{0, 1},   {0, 2},   {0, 4},   {256, 1}, {256, 2},
-                                  {256, 4}, {128, 2}, {128, 4}, {512, 1}, {512, 2}};
+// (block, targets per thread): block 4/8/16 = the shared-target
+// scalar-source kernel with that many wavefronts per 64 targets, 0 = the
+// scalar-source kernel, 128/256/512 = the LDS-tile kernel
+static const int kShapes[][2] = {{8, 1},   {4, 1},   {16, 1},  {0, 1},   {0, 2},   {0, 4}, {256, 1},
+                                  {256, 2}, {256, 4}, {128, 2}, {128, 4}, {512, 1}, {512, 2}};
+
+static bool shared_target_block(int block) { return block == 4 || block == 8 || block == 16; }
+
+static Fp32Config shape_config(int block, int ipt, int split)
+{
+    Fp32Config cfg;
+    cfg.kind = shared_target_block(block) ? 2 : block == 0 ? 1 : 0;
+    cfg.block = block == 0 ? 256 : block;
+    cfg.ipt = ipt;
+    cfg.jsplit = split;
+    return cfg;
+}
+
+static int shape_block(const Fp32Config &cfg) { return cfg.kind == 1 ? 0 : cfg.block; }
 
 ErrorStatus gs_set_fp32_config(int block, int ipt, int split)
 {
@@ -658,11 +675,16 @@ ErrorStatus gs_set_fp32_config(int block, int ipt, int split)
     if (!ok || split < 0)
         return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unsupported FP32 shape (%d, %d, %d)", block,
                                     ipt, split);
-    Fp32Config &cfg = fp32_config();
-    cfg.kind = block == 0 ? 1 : 0;
-    cfg.block = block == 0 ? 256 : block;
-    cfg.ipt = ipt;
-    cfg.jsplit = split;
+    fp32_config() = shape_config(block, ipt, split);
+    return make_success_error_status();
+}
+
+ErrorStatus gs_get_fp32_config(int *block, int *ipt, int *split)
+{
+    const Fp32Config &cfg = fp32_config();
+    if (block) *block = shape_block(cfg);
+    if (ipt) *ipt = cfg.ipt;
+    if (split) *split = cfg.jsplit;
     return make_success_error_status();
 }
 
@@ -695,11 +717,7 @@ ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
     (void)hipEventCreate(&e1);
     for (auto &sh : kShapes)
         for (int sp : splits) {
-            Fp32Config cfg;
-            cfg.kind = sh[0] == 0 ? 1 : 0;
-            cfg.block = sh[0] == 0 ? 256 : sh[0];
-            cfg.ipt = sh[1];
-            cfg.jsplit = sp;
+            const Fp32Config cfg = shape_config(sh[0], sh[1], sp);
             const size_t pb = direct_f32_partial_bytes(n, cfg, c->cu_count);
             void *part;
             ErrorStatus e2 = slot(c, SLOT_PARTIAL, pb, &part);
@@ -724,7 +742,7 @@ ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
     (void)hipEventDestroy(e1);
     gs_sim_destroy(s);
     fp32_config() = best;
-    if (block) *block = best.kind == 1 ? 0 : best.block;
+    if (block) *block = shape_block(best);
     if (ipt) *ipt = best.ipt;
     if (split) *split = best.jsplit;
     return make_success_error_status();

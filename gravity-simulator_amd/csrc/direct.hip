@@ -446,6 +446,115 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IPT <= 2 ? 
     }
 }
 
+// The scalar-source kernel with a shared target group and a software-
+// pipelined source stream.  The W wavefronts of a workgroup hold the same 64
+// targets and split the chunk's source tiles into W contiguous runs; at the
+// end wavefronts 1..W-1 leave their compensated sums in LDS and wavefront 0
+// adds the W sums in wavefront order (FP64, a fixed order), so the FP64
+// partial sums written per chunk are 1/W of k_direct_f32s's at the same
+// number of wavefronts.  Each wavefront reads its run of float4
+// sources in batches of PF_BATCH through wave-uniform loads and issues the
+// loads of batch b + 1 before the arithmetic of batch b; the last batch
+// re-reads its own sources instead of reading past the run.  Arithmetic, tile
+// frames and Kahan folds are k_direct_f32s's.
+constexpr int PF_BATCH = 4;
+
+template <bool SOFT>
+__device__ __forceinline__ void f32_batch(const float4 (&s)[PF_BATCH], float xi, float yi, float zi,
+                                          float eps2, float &px, float &py, float &pz)
+{
+#pragma unroll
+    for (int q = 0; q < PF_BATCH; q++) {
+        const float dx = s[q].x - xi;
+        const float dy = s[q].y - yi;
+        const float dz = s[q].z - zi;
+        float r2 = __builtin_fmaf(dx, dx, eps2);
+        r2 = __builtin_fmaf(dy, dy, r2);
+        r2 = __builtin_fmaf(dz, dz, r2);
+        float ri = __builtin_amdgcn_rsqf(r2);
+        if (!SOFT) ri = r2 > 0.0f ? ri : 0.0f;
+        const float mr = s[q].w * ri;
+        const float w = mr * (ri * ri);
+        px = __builtin_fmaf(dx, w, px);
+        py = __builtin_fmaf(dy, w, py);
+        pz = __builtin_fmaf(dz, w, pz);
+    }
+}
+
+template <int W, bool SOFT>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))) void k_direct_f32p(
+    const double *__restrict__ x, int n, const float4 *__restrict__ S,
+    const double4 *__restrict__ C, float eps2, int tiles_per_chunk, int ntiles,
+    double *__restrict__ partial)
+{
+    static_assert(ST_TILE % (2 * PF_BATCH) == 0, "a tile must hold whole batch pairs");
+    __shared__ double other[W - 1][3][64];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform by construction; readfirstlane lets the compiler see it,
+    // so the source addresses below stay scalar
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i0 = blockIdx.x * 64;  // the workgroup's 64 targets (uniform)
+    if (i0 >= n) return;
+    const int i = i0 + lane;
+    const int ii = i < n ? i : i0;
+    const double xd = x[3 * ii + 0], yd = x[3 * ii + 1], zd = x[3 * ii + 2];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    const int J0 = blockIdx.y * tiles_per_chunk;
+    const int Jc = min(ntiles, J0 + tiles_per_chunk);
+    const int tpw = (Jc - J0 + W - 1) / W;
+    const int Ja = min(Jc, J0 + wv * tpw);
+    const int Jb = min(Jc, Ja + tpw);
+    if (Ja < Jb) {
+        const float4 *__restrict__ base = S + static_cast<size_t>(Ja) * ST_TILE;
+        const int last = (Jb - Ja) * ST_TILE - PF_BATCH;  // start of the run's last batch
+        float4 a[PF_BATCH], b[PF_BATCH];
+#pragma unroll
+        for (int q = 0; q < PF_BATCH; q++) a[q] = base[q];
+        int pos = 0;
+        for (int J = Ja; J < Jb; J++) {
+            const double4 c = C[J];
+            const float xi = static_cast<float>(xd - c.x);
+            const float yi = static_cast<float>(yd - c.y);
+            const float zi = static_cast<float>(zd - c.z);
+            float px = 0.0f, py = 0.0f, pz = 0.0f;
+            for (int jj = 0; jj < ST_TILE; jj += 2 * PF_BATCH, pos += 2 * PF_BATCH) {
+                const float4 *nb = base + min(pos + PF_BATCH, last);
+#pragma unroll
+                for (int q = 0; q < PF_BATCH; q++) b[q] = nb[q];
+                f32_batch<SOFT>(a, xi, yi, zi, eps2, px, py, pz);
+                const float4 *na = base + min(pos + 2 * PF_BATCH, last);
+#pragma unroll
+                for (int q = 0; q < PF_BATCH; q++) a[q] = na[q];
+                f32_batch<SOFT>(b, xi, yi, zi, eps2, px, py, pz);
+            }
+            kahan_add(sx, cx, px);
+            kahan_add(sy, cy, py);
+            kahan_add(sz, cz, pz);
+        }
+    }
+    double rx = static_cast<double>(sx) - static_cast<double>(cx);
+    double ry = static_cast<double>(sy) - static_cast<double>(cy);
+    double rz = static_cast<double>(sz) - static_cast<double>(cz);
+    if (wv > 0) {
+        other[wv - 1][0][lane] = rx;
+        other[wv - 1][1][lane] = ry;
+        other[wv - 1][2][lane] = rz;
+    }
+    __syncthreads();
+    if (wv == 0 && i < n) {
+#pragma unroll
+        for (int w = 0; w < W - 1; w++) {
+            rx += other[w][0][lane];
+            ry += other[w][1][lane];
+            rz += other[w][2][lane];
+        }
+        double *out = partial + static_cast<size_t>(blockIdx.y) * 3 * n;
+        out[3 * i + 0] = rx;
+        out[3 * i + 1] = ry;
+        out[3 * i + 2] = rz;
+    }
+}
+
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nsplit, int n3,
                                   double G, double *__restrict__ a)
 {
@@ -467,11 +576,15 @@ Fp32Config &fp32_config()
 // MI355X at N = 2^17 with 256-source tiles (kernel only): 2048 per chunk
 // 4.36 ms, 1024 4.35 ms, 512 4.34 ms; each halving doubles the FP64 partial
 // sums (402 MB at 1024) that the reduction reads back (~0.09 ms).
+//
+// The shared-target kernel (kind 2) spreads a chunk over its W wavefronts:
+// 1024 W sources per chunk keep 1024 sources per wavefront (W = 8: 16 chunks
+// at N = 2^17, 50 MB of partial sums).
 static int resolve_split(int n, const Fp32Config &cfg, int cu_count)
 {
     (void)cu_count;
     if (cfg.jsplit > 0) return cfg.jsplit;
-    const int per_chunk = 1024;
+    const int per_chunk = cfg.kind == 2 ? 1024 * cfg.block : 1024;
     const int split = (n + per_chunk - 1) / per_chunk;
     return split < 1 ? 1 : split;
 }
@@ -481,7 +594,7 @@ static int f32s_tiles(int n) { return (n + ST_TILE - 1) / ST_TILE; }
 size_t direct_f32_partial_bytes(int n, const Fp32Config &cfg, int cu_count)
 {
     size_t b = static_cast<size_t>(resolve_split(n, cfg, cu_count)) * 3 * n * sizeof(double);
-    if (cfg.kind == 1) {
+    if (cfg.kind >= 1) {
         const size_t nt = f32s_tiles(n);
         b += 256 + nt * ST_TILE * sizeof(float4) + nt * sizeof(double4);
     }
@@ -513,17 +626,31 @@ static void launch_f32s_shape(Context *c, dim3 grid, const double *x, int n, con
                            C, eps2, tpc, ntiles, partial);
 }
 
+template <int W>
+static void launch_f32p_shape(Context *c, dim3 grid, const double *x, int n, const float4 *S,
+                              const double4 *C, float eps2, bool soft, int tpc, int ntiles,
+                              double *partial)
+{
+    if (soft)
+        hipLaunchKernelGGL((k_direct_f32p<W, true>), grid, dim3(64 * W), 0, c->stream, x, n, S, C,
+                           eps2, tpc, ntiles, partial);
+    else
+        hipLaunchKernelGGL((k_direct_f32p<W, false>), grid, dim3(64 * W), 0, c->stream, x, n, S,
+                           C, eps2, tpc, ntiles, partial);
+}
+
 hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n, double G,
                              double eps, double *a, void *partial_buf, size_t partial_bytes,
                              const Fp32Config &cfg)
 {
     const int split = resolve_split(n, cfg, c->cu_count);
     if (partial_bytes < direct_f32_partial_bytes(n, cfg, c->cu_count)) return hipErrorInvalidValue;
+    if (cfg.kind == 2 && cfg.ipt != 1) return hipErrorInvalidValue;
     const float eps2 = static_cast<float>(eps * eps);
     const bool soft = eps2 > 0.0f;
     double *partial = static_cast<double *>(partial_buf);
     int nsplit;
-    if (cfg.kind == 1) {
+    if (cfg.kind >= 1) {
         const int ntiles = f32s_tiles(n);
         const int n_pad = ntiles * ST_TILE;
         char *tail = static_cast<char *>(partial_buf) + static_cast<size_t>(split) * 3 * n * sizeof(double);
@@ -534,14 +661,24 @@ hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n
                            n, n_pad, S, C);
         const int tpc = (ntiles + split - 1) / split;
         nsplit = (ntiles + tpc - 1) / tpc;
-        const int waves = (n + 64 * cfg.ipt - 1) / (64 * cfg.ipt);
-        dim3 grid((waves + 3) / 4, nsplit);
         const int rec = timing_begin(c, "direct_f32");
-        switch (cfg.ipt) {
-            case 1: launch_f32s_shape<1>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
-            case 2: launch_f32s_shape<2>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
-            case 4: launch_f32s_shape<4>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
-            default: timing_end(c, rec); return hipErrorInvalidValue;
+        if (cfg.kind == 2) {
+            dim3 grid((n + 63) / 64, nsplit);
+            switch (cfg.block) {
+                case 4: launch_f32p_shape<4>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+                case 8: launch_f32p_shape<8>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+                case 16: launch_f32p_shape<16>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+                default: timing_end(c, rec); return hipErrorInvalidValue;
+            }
+        } else {
+            const int waves = (n + 64 * cfg.ipt - 1) / (64 * cfg.ipt);
+            dim3 grid((waves + 3) / 4, nsplit);
+            switch (cfg.ipt) {
+                case 1: launch_f32s_shape<1>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+                case 2: launch_f32s_shape<2>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+                case 4: launch_f32s_shape<4>(c, grid, x, n, S, C, eps2, soft, tpc, ntiles, partial); break;
+                default: timing_end(c, rec); return hipErrorInvalidValue;
+            }
         }
         timing_end(c, rec);
     } else {

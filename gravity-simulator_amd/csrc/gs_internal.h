@@ -106,8 +106,9 @@ hipError_t launch_direct_f64(Context *c, const double *x, const double *m, int n
 
 // FP32 tile-relative direct sum with tile-level Kahan compensation.
 struct Fp32Config {
-    int kind = 1;        // 0: LDS source tiles, 1: scalar-loaded source tiles
-    int block = 256;     // threads per workgroup (kind 0)
+    int kind = 2;        // 0: LDS source tiles, 1: scalar-loaded source tiles,
+                         // 2: scalar-loaded source tiles, shared target groups
+    int block = 16;      // threads per workgroup (kind 0), wavefronts per target group (kind 2)
     int ipt = 1;         // targets per thread
     int jsplit = 0;      // source chunks (0 = automatic)
 };

@@ -199,11 +199,17 @@ ErrorStatus gs_learned_status(double *last_p99, int *fell_back);
  * N-particle input resident on the device and keeps the fastest for later
  * calls.  Writes the chosen shape.  block = 0 denotes the scalar-source
  * kernel (independent wavefronts reading pre-converted source tiles through
- * wave-uniform loads); block = 128/256/512 the LDS-tile kernel.  A source
- * split of 0 picks the split automatically. */
+ * wave-uniform loads); block = 4, 8 or 16 the shared-target scalar-source
+ * kernel (that many wavefronts per 64 targets, each over its share of the
+ * source chunk, their sums combined in LDS in a fixed order;
+ * targets_per_thread 1 only);
+ * block = 128/256/512 the LDS-tile kernel.  A source split of 0 picks the
+ * split automatically. */
 ErrorStatus gs_autotune_fp32(int num_particles, int *block, int *targets_per_thread,
                              int *source_split);
 ErrorStatus gs_set_fp32_config(int block, int targets_per_thread, int source_split);
+/* The shape later FP32 direct sums use (default: block 16, 1, automatic split). */
+ErrorStatus gs_get_fp32_config(int *block, int *targets_per_thread, int *source_split);
 
 #ifdef __cplusplus
 }

@@ -93,17 +93,37 @@ def test_energy_matches_oracle(golden):
     assert abs(e - float(f["e0"])) <= 1e-12 * abs(float(f["e0"]))
 
 
-@pytest.mark.parametrize("block,ipt", [(0, 1), (0, 2), (0, 4), (256, 1), (256, 2), (256, 4), (128, 2),
+@pytest.mark.parametrize("block,ipt", [(4, 1), (8, 1), (16, 1), (0, 1), (0, 2), (0, 4), (256, 1), (256, 2), (256, 4), (128, 2),
                                        (128, 4), (512, 1), (512, 2)])
-@pytest.mark.parametrize("split", [0, 3])
+@pytest.mark.parametrize("split", [0, 3, 5, 12])
 def test_fp32_every_shape_vs_oracle(block, ipt, split):
     """Each kernel shape the autotuner may pick gives the same accuracy (ragged N)."""
     x, v, m = ic.plummer(3001, 13)
+    saved = L.get_fp32_config()
     try:
         L.set_fp32_config(block, ipt, split)
         a32 = L.acceleration(x, m, 16, eps=1e-3)
     finally:
-        L.set_fp32_config(0, 1, 0)
+        L.set_fp32_config(*saved)
     a64 = O.pairwise(x, m, eps=1e-3)
     e = rel_err(a32, a64)
     assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
+def test_autotuner_picks_a_valid_shape_that_meets_the_error_bound():
+    """gs_autotune_fp32 times every candidate shape and keeps the fastest; the
+    shape it keeps is one of the parity-tested ones and later evaluations with
+    it meet the FP32 bound against the oracle."""
+    saved = L.get_fp32_config()
+    try:
+        block, ipt, split = L.autotune_fp32(1 << 14)
+        assert (block, ipt) in {(4, 1), (8, 1), (16, 1), (0, 1), (0, 2), (0, 4), (256, 1), (256, 2), (256, 4),
+                                (128, 2), (128, 4), (512, 1), (512, 2)}
+        assert split in (0, 4, 8, 16, 32, 64)
+        assert L.get_fp32_config() == (block, ipt, split)
+        x, v, m = ic.plummer(3001, 17)
+        a32 = L.acceleration(x, m, 16, eps=1e-3)
+    finally:
+        L.set_fp32_config(*saved)
+    e = rel_err(a32, O.pairwise(x, m, eps=1e-3))
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (block, ipt, split, np.median(e), e.max())
