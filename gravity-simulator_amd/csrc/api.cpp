@@ -230,48 +230,83 @@ ErrorStatus construct_octree(LinearOctree *octree, const System *system,
     return tree_to_host(c, t, octree);
 }
 
-ErrorStatus benchmark_acceleration(const System *system, const AccelerationParam *params,
-                                   const int num_params, const int *num_times)
+static const char *benchmark_method_name(int method)
+{
+    switch (method) {
+        case ACCELERATION_METHOD_PAIRWISE: return "Pairwise";
+        case ACCELERATION_METHOD_MASSLESS: return "Massless";
+        case ACCELERATION_METHOD_BARNES_HUT: return "Barnes-Hut";
+        case ACCELERATION_METHOD_PAIRWISE_FP32: return "Pairwise (FP32 split)";
+        case ACCELERATION_METHOD_MASSLESS_FP32: return "Massless (FP32 split)";
+        case ACCELERATION_METHOD_BARNES_HUT_LEARNED: return "Barnes-Hut (learned correction)";
+        case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE: return "Barnes-Hut (quadrupole)";
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP: return "Barnes-Hut (group walk)";
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE:
+            return "Barnes-Hut (group walk, quadrupole)";
+        default: return nullptr;
+    }
+}
+
+// Directed pair interactions of one evaluation: all ordered pairs for the
+// pairwise sums, massive sources on every other particle for the massless
+// forms, the walk's own count for the tree methods.
+static double benchmark_interactions(const System *system, int method)
+{
+    const double n = system->num_particles;
+    switch (method) {
+        case ACCELERATION_METHOD_PAIRWISE:
+        case ACCELERATION_METHOD_PAIRWISE_FP32: return n * (n - 1.0);
+        case ACCELERATION_METHOD_MASSLESS:
+        case ACCELERATION_METHOD_MASSLESS_FP32: {
+            double massive = 0.0;
+            for (int i = 0; i < system->num_particles; i++) massive += system->m[i] != 0.0;
+            return massive * (n - 1.0);
+        }
+        default: return static_cast<double>(g_last_walk_interactions);
+    }
+}
+
+ErrorStatus gs_benchmark_acceleration(const System *system, const AccelerationParam *params,
+                                      const int num_params, const int *num_times,
+                                      GsBenchmarkRecord *records)
 {
     if (!system || !params || !num_times)
         return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "NULL argument");
     const int n = system->num_particles;
     std::vector<double> ref(static_cast<size_t>(3) * n), a(static_cast<size_t>(3) * n);
-    // the MAE reference is the first parameter set, always evaluated once
-    if (num_params > 0) GS_TRY(acceleration(ref.data(), system, &params[0]));
+    // The MAE reference is test 0's first run (src/acceleration.c:419-435).
+    // When test 0 is skipped the reference would compare against memory
+    // nothing wrote; here that reference is then evaluated once, untimed.
+    if (num_params > 0 && num_times[0] <= 0) GS_TRY(acceleration(ref.data(), system, &params[0]));
+    GsDeviceInfo info;
+    GS_TRY(gs_device_info(&info));
     fputs("Benchmarking acceleration...\n", stdout);
     for (int i = 0; i < num_params; i++) {
         const int times = num_times[i];
+        GsBenchmarkRecord rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.method = params[i].method;
+        rec.num_times = times;
         if (times <= 0) {
             printf("Test %d:    Skipped since num_times: %d <= 0\n\n", i, times);
+            if (records) records[i] = rec;
             continue;
         }
+        const char *name = benchmark_method_name(params[i].method);
+        if (!name)
+            return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Unknown acceleration method. Got: %d",
+                                        params[i].method);
         std::vector<double> rt(times);
         double mae = 0.0;
         for (int j = 0; j < times; j++) {
+            double *out = (i == 0 && j == 0) ? ref.data() : a.data();
             const double t0 = grav_get_current_time();
-            GS_TRY(acceleration(a.data(), system, &params[i]));
+            GS_TRY(acceleration(out, system, &params[i]));
             rt[j] = grav_get_current_time() - t0;
             if (i != 0 && j == 0) {
                 for (int k = 0; k < 3 * n; k++) mae += fabs(ref[k] - a[k]);
                 mae /= n;
             }
-        }
-        printf("Test %d:", i);
-        switch (params[i].method) {
-            case ACCELERATION_METHOD_PAIRWISE: fputs("    Method: Pairwise\n", stdout); break;
-            case ACCELERATION_METHOD_MASSLESS: fputs("    Method: Massless\n", stdout); break;
-            case ACCELERATION_METHOD_BARNES_HUT: fputs("    Method: Barnes-Hut\n", stdout); break;
-            case ACCELERATION_METHOD_PAIRWISE_FP32:
-                fputs("    Method: Pairwise (FP32 split)\n", stdout);
-                break;
-            case ACCELERATION_METHOD_MASSLESS_FP32:
-                fputs("    Method: Massless (FP32 split)\n", stdout);
-                break;
-            default:
-                return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
-                                            "Unknown acceleration method. Got: %d",
-                                            params[i].method);
         }
         double mean = 0.0;
         for (double t : rt) mean += t;
@@ -279,12 +314,33 @@ ErrorStatus benchmark_acceleration(const System *system, const AccelerationParam
         double var = 0.0;
         for (double t : rt) var += (t - mean) * (t - mean);
         const double sd = times > 1 ? sqrt(var / (times - 1)) : 0.0;
+        printf("Test %d:", i);
+        printf("    Method: %s\n", name);
         printf("    Number of times: %d\n", times);
         printf("    Avg time: %.3g (+- %.3g) s\n", mean, sd);
         printf("    MAE: %.3g\n", mae);
         printf("\n");
+        rec.mean_s = mean;
+        rec.std_s = sd;
+        rec.mae = mae;
+        rec.interactions = benchmark_interactions(system, params[i].method);
+        rec.interactions_per_s = mean > 0.0 ? rec.interactions / mean : 0.0;
+        rec.particles_per_s = mean > 0.0 ? n / mean : 0.0;
+        rec.flops_per_s = 20.0 * rec.interactions_per_s;
+        const bool fp64 = params[i].method == ACCELERATION_METHOD_PAIRWISE ||
+                          params[i].method == ACCELERATION_METHOD_MASSLESS;
+        rec.peak_flops_per_s = 1e12 * (fp64 ? info.fp64_vector_tflops : info.fp32_vector_tflops);
+        rec.roofline_frac = rec.flops_per_s / rec.peak_flops_per_s;
+        if (records) records[i] = rec;
     }
+    fflush(stdout);
     return make_success_error_status();
+}
+
+ErrorStatus benchmark_acceleration(const System *system, const AccelerationParam *params,
+                                   const int num_params, const int *num_times)
+{
+    return gs_benchmark_acceleration(system, params, num_params, num_times, nullptr);
 }
 
 // ---------------------------------------------------------------- energy

@@ -33,7 +33,7 @@ EXPORTS = [
     "free_linear_octree", "linear_octree_check_if_included", "get_new_integrator_param",
     "finalize_integration_param", "integrator_launch_simulation", "launch_simulation", "compute_energy",
     "grav_get_current_time", "launch_simulation_python", "compute_energy_python",
-    "gs_device_info", "gs_synchronize", "gs_timing_enable", "gs_timing_reset", "gs_timing_query",
+    "gs_device_info", "gs_benchmark_acceleration", "gs_synchronize", "gs_timing_enable", "gs_timing_reset", "gs_timing_query",
     "gs_sim_create", "gs_sim_destroy", "gs_sim_upload", "gs_sim_download", "gs_sim_acceleration",
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
@@ -49,6 +49,13 @@ EXPORTS = [
     "finalize_cosmological_system", "free_cosmological_system", "set_periodic_boundary_conditions",
     "acceleration_PM", "leapfrog_cosmology", "output_snapshot_cosmology", "launch_cosmological_simulation",
 ]
+
+
+class GsBenchmarkRecord(C.Structure):
+    _fields_ = [("method", C.c_int), ("num_times", C.c_int), ("mean_s", C.c_double), ("std_s", C.c_double),
+                ("mae", C.c_double), ("interactions", C.c_double), ("interactions_per_s", C.c_double),
+                ("particles_per_s", C.c_double), ("flops_per_s", C.c_double), ("peak_flops_per_s", C.c_double),
+                ("roofline_frac", C.c_double)]
 
 
 class GsDeviceInfo(C.Structure):
@@ -91,6 +98,8 @@ def load(path=None):
         "compute_energy": (C.c_double, [pS]),
         "free_traceback": (None, [C.POINTER(ES)]),
         "gs_device_info": (ES, [C.POINTER(GsDeviceInfo)]),
+        "gs_benchmark_acceleration": (ES, [C.POINTER(System), C.POINTER(AccelerationParam), C.c_int, PI,
+                                           C.POINTER(GsBenchmarkRecord)]),
         "gs_synchronize": (ES, []),
         "gs_timing_enable": (ES, [C.c_int]),
         "gs_timing_reset": (ES, []),
@@ -401,6 +410,22 @@ def whfast_ensemble(offsets, x, v, m, G, dt, tf, method=2, eps=0.0, remove_inval
 
 
 # ---------------------------------------------------------------- learned correction
+def benchmark_acceleration(x, m, sets, G=1.0):
+    """gs_benchmark_acceleration over sets of (method, theta, eps, leaf,
+    num_times); returns the records as dicts (the text goes to stdout)."""
+    L = load()
+    x, m = _c(x), _c(m)
+    v = np.zeros_like(x)
+    s = _abi.make_system(x, v, m, G)
+    k = len(sets)
+    params = (AccelerationParam * k)(*[AccelerationParam(me, th, ep, lf) for me, th, ep, lf, _ in sets])
+    times = np.array([t for *_, t in sets], dtype=np.int32)
+    recs = (GsBenchmarkRecord * k)()
+    check(L.gs_benchmark_acceleration(C.byref(s), params, k, times.ctypes.data_as(PI), recs),
+          "gs_benchmark_acceleration")
+    return [{f: getattr(r, f) for f, _ in GsBenchmarkRecord._fields_} for r in recs]
+
+
 def learned_num_params():
     return int(load().gs_learned_num_params())
 

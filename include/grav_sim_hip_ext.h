@@ -62,6 +62,37 @@ typedef struct GsDeviceInfo
 
 ErrorStatus gs_device_info(GsDeviceInfo *info);
 
+/* ---- benchmark_acceleration with a machine-readable record ----------------
+ * benchmark_acceleration (src/acceleration.c:369-507) prints the same text
+ * and, when records is not NULL, fills records[i] for each parameter set:
+ * timings of the host call (device copies included), the MAE against test 0,
+ * the directed pair interactions of one evaluation (all ordered pairs for the
+ * pairwise sums, massive sources on every other particle for the massless
+ * forms, the walk's count for the tree methods), their rate, particles/s,
+ * 20 flops per interaction and the fraction of the FP64 (methods 1, 2) or FP32
+ * (every other method) vector peak.  Skipped tests keep method and num_times
+ * and zeros elsewhere. */
+typedef struct GsBenchmarkRecord
+{
+    int method;
+    int num_times;
+    double mean_s;
+    double std_s;
+    double mae;
+    double interactions;
+    double interactions_per_s;
+    double particles_per_s;
+    double flops_per_s;
+    double peak_flops_per_s;
+    double roofline_frac;
+} GsBenchmarkRecord;
+
+ErrorStatus gs_benchmark_acceleration(const System *system,
+                                      const AccelerationParam *acceleration_params,
+                                      int num_acceleration_params,
+                                      const int *num_times_acceleration_param,
+                                      GsBenchmarkRecord *records);
+
 /* Wait for all work the library has queued. */
 ErrorStatus gs_synchronize(void);
 
@@ -210,6 +241,7 @@ ErrorStatus gs_autotune_fp32(int num_particles, int *block, int *targets_per_thr
 ErrorStatus gs_set_fp32_config(int block, int targets_per_thread, int source_split);
 /* The shape later FP32 direct sums use (default: block 16, 1, automatic split). */
 ErrorStatus gs_get_fp32_config(int *block, int *targets_per_thread, int *source_split);
+
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,10 @@ exported functions on seeded inputs and stores inputs and outputs as data:
                     plus, as a characterisation of the two quirks, the same
                     particles in generator order with the reference output
                     at theta 0.5, leaf 1 (src/acceleration_barnes_hut.c:33-248)
+  f11_benchmark.npz the text reference benchmark_acceleration
+                    (src/acceleration.c:369-507) prints for N=2048 two-sphere
+                    (seed 11): pairwise x2, massless x1, Barnes-Hut theta 0.5
+                    leaf 1 x2 and a skipped set (num_times 0)
   f5_leapfrog.npz   N=1024 Plummer (seed 1), eps=0, dt=1e-3, 1000 steps:
                     reference launch_simulation(LEAPFROG) final x, v and
                     energies (src/integrator.c:894-1121, src/utils.c:27-59)
@@ -139,6 +143,42 @@ def f7():
             out[f"cap{cap}_a_theta{theta}_eps{eps}"] = O.ref_acceleration(x, m, 3, eps=eps, theta=theta, leaf=cap)
     out["a_generator_order_theta0.5"] = O.ref_acceleration(x0, m0, 3, eps=0.0, theta=0.5, leaf=1)
     np.savez_compressed(os.path.join(HERE, "f7_bh_walk.npz"), **out)
+
+
+BENCH_SETS = ((1, 0.0, 1e-3, -1, 2), (2, 0.0, 1e-3, -1, 1), (3, 0.5, 1e-3, 1, 2), (1, 0.0, 1e-3, -1, 0))
+
+
+def capture_stdout(fn):
+    """Run fn() and return what C code wrote to file descriptor 1."""
+    import ctypes
+    import tempfile
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    with tempfile.TemporaryFile(mode="w+b") as tmp:
+        saved = os.dup(1)
+        os.dup2(tmp.fileno(), 1)
+        try:
+            fn()
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        return tmp.read().decode()
+
+
+def f11():
+    import ctypes
+    from grav_sim_amd import _abi
+    x, v, m = ic.two_spheres(2048, 11)
+    s = _abi.make_system(x, v, m, 1.0)
+    params = (_abi.AccelerationParam * len(BENCH_SETS))(*[_abi.AccelerationParam(meth, th, eps, leaf)
+                                                           for meth, th, eps, leaf, _ in BENCH_SETS])
+    times = (ctypes.c_int * len(BENCH_SETS))(*[t for *_, t in BENCH_SETS])
+    R = O.ref()
+    R.benchmark_acceleration.restype = _abi.ErrorStatus
+    text = capture_stdout(lambda: R.benchmark_acceleration(ctypes.byref(s), params, len(BENCH_SETS), times))
+    np.savez_compressed(os.path.join(HERE, "f11_benchmark.npz"), text=np.array(text), x=x, m=m)
 
 
 def f5():
@@ -315,7 +355,8 @@ def f10():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f7", "f8", "f9", "f10"]
+    which = sys.argv[1:] or ["f1", "f2", "f3", "f5", "f6", "f7", "f8", "f9", "f10", "f11"]
     for w in which:
-        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f7": f7, "f8": f8, "f9": f9, "f10": f10}[w]()
+        {"f1": f1, "f2": f2, "f3": f3_f4, "f5": f5, "f6": f6, "f7": f7, "f8": f8, "f9": f9, "f10": f10,
+         "f11": f11}[w]()
         print("wrote", w, flush=True)
