@@ -50,6 +50,9 @@ ErrorStatus context(Context **out)
         }
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking);
+    // the no-allocation entry point's static scratch: its device address on
+    // this device (a failure here is reported by that entry point)
+    if (e == hipSuccess) (void)noalloc_scratch_init(dev);
     if (e != hipSuccess) {
         g_init_error.return_code = GRAV_FAILURE;
         return hip_error(e, "device initialisation", __FILE__, __LINE__, __func__);

@@ -85,6 +85,9 @@ struct Context {
 
 // Returns the initialised context or an error (no HIP device, init failure).
 ErrorStatus context(Context **out);
+// Looks up, once per device with that device current, the device address of
+// the no-allocation entry point's static scratch array.
+hipError_t noalloc_scratch_init(int device);
 
 // Grow-only device buffer for a slot.
 ErrorStatus slot(Context *c, Slot s, size_t bytes, void **out);

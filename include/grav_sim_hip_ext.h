@@ -241,6 +241,19 @@ ErrorStatus gs_autotune_fp32(int num_particles, int *block, int *targets_per_thr
 ErrorStatus gs_set_fp32_config(int block, int targets_per_thread, int source_split);
 /* The shape later FP32 direct sums use (default: block 16, 1, automatic split). */
 ErrorStatus gs_get_fp32_config(int *block, int *targets_per_thread, int *source_split);
+/* ---- direct sums without device allocation ---------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  acceleration() for methods 1, 2, 16 and 17 with
+ * system->x, system->m and a in device memory of the library's device; the
+ * result is complete when the call returns.  It calls no hipMalloc or
+ * hipFree: the FP32 sums' scratch is a static device array of 32 MiB that
+ * the GPU holds until the process ends, enough for about 800 000 particles
+ * (larger N: GRAV_VALUE_ERROR; past the default shape's source-chunk count
+ * the chunks are fewer, which changes the FP32 rounding, not the bound).
+ * Calls that overlap on one GPU share that array and race. */
+ErrorStatus gs_acceleration_noalloc(double *a, const System *system,
+                                    const AccelerationParam *acceleration_param);
 
 
 #ifdef __cplusplus
