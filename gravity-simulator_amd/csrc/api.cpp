@@ -113,6 +113,9 @@ struct GsSim {
     double G = 1.0;
     double *x = nullptr, *v = nullptr, *m = nullptr, *a = nullptr;
     double *xe = nullptr, *ve = nullptr;
+    // the other half of the ping-pong of the fused FP64 Leapfrog step; x, xe
+    // and v hold the state whenever no step call is running
+    double *x2 = nullptr, *xe2 = nullptr, *v2 = nullptr;
     // captured Leapfrog step (drift, forces, kick) and what it was captured for
     hipGraphExec_t lf_exec = nullptr;
     struct Key {
@@ -120,11 +123,12 @@ struct GsSim {
         double eps = 0.0, dt = 0.0;
         Fp32Config cfg;
         void *partial = nullptr;
+        double *x = nullptr;  // the state buffers current when captured
         bool operator==(const Key &o) const
         {
             return method == o.method && eps == o.eps && dt == o.dt && cfg.kind == o.cfg.kind &&
                    cfg.block == o.cfg.block && cfg.ipt == o.cfg.ipt && cfg.jsplit == o.cfg.jsplit &&
-                   partial == o.partial;
+                   partial == o.partial && x == o.x;
         }
     } lf_key;
 };
@@ -141,13 +145,38 @@ static bool graph_method(int method)
            method == ACCELERATION_METHOD_MASSLESS_FP32;
 }
 
-static ErrorStatus lf_step(Context *c, GsSim *s, const AccelerationParam *p, double dt)
+// The FP64 direct sums run the whole step as one kernel (drift, forces and
+// kick fused, into the other half of the ping-pong); the other methods as
+// drift, force evaluation and kick.
+static bool fused_step_method(int method)
 {
+    return method == ACCELERATION_METHOD_PAIRWISE || method == ACCELERATION_METHOD_MASSLESS;
+}
+
+// parity: which half of the fused step's ping-pong holds the state (0: x,
+// xe, v; 1: x2, xe2, v2); the fused step flips it.
+static ErrorStatus lf_step(Context *c, GsSim *s, const AccelerationParam *p, double dt,
+                           int &parity)
+{
+    if (fused_step_method(p->method)) {
+        double *x = parity ? s->x2 : s->x, *xe = parity ? s->xe2 : s->xe, *v = parity ? s->v2 : s->v;
+        double *xo = parity ? s->x : s->x2, *xeo = parity ? s->xe : s->xe2, *vo = parity ? s->v : s->v2;
+        GS_HIP(launch_lf_step_f64(c, x, xe, v, s->m, s->n, s->G, p->softening_length,
+                                  p->method == ACCELERATION_METHOD_MASSLESS, dt, s->a, xo, xeo, vo,
+                                  s->ve));
+        parity ^= 1;
+        return make_success_error_status();
+    }
     GS_HIP(launch_lf_drift(c, s->x, s->xe, s->v, s->n, dt));
     GS_TRY(device_acceleration(c, s->x, s->m, s->n, s->G, p, s->a));
     GS_HIP(launch_lf_kick(c, s->v, s->ve, s->a, s->n, dt));
     return make_success_error_status();
 }
+
+// Steps one graph replay performs: two for the fused step (there and back
+// through the ping-pong, starting from and ending at parity 1), one
+// otherwise.
+static int lf_graph_steps(const AccelerationParam *p) { return fused_step_method(p->method) ? 2 : 1; }
 
 // Capture one step into s->lf_exec unless the cached graph was captured for
 // the same parameters and buffers.  Called after an eager step, so every
@@ -160,13 +189,17 @@ static ErrorStatus lf_graph(Context *c, GsSim *s, const AccelerationParam *p, do
     key.dt = dt;
     key.cfg = fp32_config();
     key.partial = c->slots[SLOT_PARTIAL].ptr;
+    key.x = s->x;
     if (s->lf_exec && key == s->lf_key) return make_success_error_status();
+    int parity = 1;  // a captured fused pair starts after one eager step
     if (s->lf_exec) {
         GS_HIP(hipGraphExecDestroy(s->lf_exec));
         s->lf_exec = nullptr;
     }
     GS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    ErrorStatus st = lf_step(c, s, p, dt);
+    ErrorStatus st = make_success_error_status();
+    for (int r = 0; r < lf_graph_steps(p) && st.return_code == GRAV_SUCCESS; r++)
+        st = lf_step(c, s, p, dt, parity);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (st.return_code != GRAV_SUCCESS) {
@@ -421,6 +454,9 @@ ErrorStatus gs_sim_create(GsSim **out, int n, double G)
     if (!e) e = hipMalloc(&s->a, b3);
     if (!e) e = hipMalloc(&s->xe, b3);
     if (!e) e = hipMalloc(&s->ve, b3);
+    if (!e) e = hipMalloc(&s->x2, b3);
+    if (!e) e = hipMalloc(&s->xe2, b3);
+    if (!e) e = hipMalloc(&s->v2, b3);
     if (!e) e = hipMemsetAsync(s->xe, 0, b3, c->stream);
     if (!e) e = hipMemsetAsync(s->ve, 0, b3, c->stream);
     if (!e) e = hipMemsetAsync(s->a, 0, b3, c->stream);
@@ -447,6 +483,9 @@ ErrorStatus gs_sim_destroy(GsSim *s)
     (void)hipFree(s->a);
     (void)hipFree(s->xe);
     (void)hipFree(s->ve);
+    (void)hipFree(s->x2);
+    (void)hipFree(s->xe2);
+    (void)hipFree(s->v2);
     delete s;
     return make_success_error_status();
 }
@@ -539,15 +578,24 @@ ErrorStatus gs_sim_leapfrog_steps(GsSim *s, const AccelerationParam *p, double d
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     // with per-kernel timing on, every step runs eagerly so that its kernels
     // carry their own events
-    const bool graph = k >= 3 && graph_method(p->method) && !c->timing;
+    const int per = lf_graph_steps(p);
+    const bool graph = k >= 1 + 2 * per && graph_method(p->method) && !c->timing;
     int64 i = 0;
+    int parity = 0;
     if (graph) {
-        GS_TRY(lf_step(c, s, p, dt));  // allocates what the step needs
+        GS_TRY(lf_step(c, s, p, dt, parity));  // allocates what the step needs
         i = 1;
         GS_TRY(lf_graph(c, s, p, dt));
-        for (; i < k; i++) GS_HIP(hipGraphLaunch(s->lf_exec, c->stream));
+        for (; i + per <= k; i += per) GS_HIP(hipGraphLaunch(s->lf_exec, c->stream));
     }
-    for (; i < k; i++) GS_TRY(lf_step(c, s, p, dt));
+    for (; i < k; i++) GS_TRY(lf_step(c, s, p, dt, parity));
+    if (parity) {
+        // an odd number of fused steps: the state is in the second half
+        const size_t b3 = sizeof(double) * 3 * s->n;
+        GS_HIP(hipMemcpyAsync(s->x, s->x2, b3, hipMemcpyDeviceToDevice, c->stream));
+        GS_HIP(hipMemcpyAsync(s->xe, s->xe2, b3, hipMemcpyDeviceToDevice, c->stream));
+        GS_HIP(hipMemcpyAsync(s->v, s->v2, b3, hipMemcpyDeviceToDevice, c->stream));
+    }
     return make_success_error_status();
 }
 
@@ -765,9 +813,12 @@ ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
         return es;
     }
     const int splits[] = {0, 4, 8, 16, 32, 64};
-    const Fp32Config saved = fp32_config();
-    Fp32Config best = saved;
-    float best_ms = 1e30f;
+    struct Timed {
+        Fp32Config cfg;
+        float ms;
+        size_t bytes;
+    };
+    std::vector<Timed> timed;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -789,10 +840,22 @@ ErrorStatus gs_autotune_fp32(int n, int *block, int *ipt, int *split)
             (void)hipEventSynchronize(e1);
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = cfg;
-            }
+            timed.push_back({cfg, ms, pb});
+        }
+    // the fastest shape, except that among the shapes within 1 % of it (the
+    // run-to-run spread of these timings) the one with the fewest scratch
+    // bytes -- the FP64 partial sums it writes and reads back -- wins
+    Fp32Config best = fp32_config();
+    float best_ms = 1e30f;
+    for (const Timed &t : timed) best_ms = std::min(best_ms, t.ms);
+    size_t best_bytes = ~size_t(0);
+    float best_tie_ms = 1e30f;
+    for (const Timed &t : timed)
+        if (t.ms <= 1.01f * best_ms &&
+            (t.bytes < best_bytes || (t.bytes == best_bytes && t.ms < best_tie_ms))) {
+            best = t.cfg;
+            best_bytes = t.bytes;
+            best_tie_ms = t.ms;
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -27,37 +27,72 @@ constexpr int F64_BLOCK = 256;
 // Bitwise reference sums: every target adds its terms in source-index order,
 // term = fl(fl(G / fl(fl(r*r)*r) * fl(x_l - x_k)) * m_l), l != k
 // (src/acceleration.c:206-229 as restated per target).  A workgroup takes
-// F64_TB targets; per tile of 256 sources every thread computes its source's
-// terms for all F64_TB targets into LDS (the expensive sqrt and division run
-// on all lanes), then one thread per (target, component) adds the tile's
-// terms in index order.  The additions are the reference's, in the
-// reference's order, so the result is bit for bit the serial one.
-constexpr int F64_TB = 8;
+// TB targets; per tile of 256 sources every thread computes its source's
+// terms for all TB targets into LDS (the expensive sqrt and division run on
+// all lanes), then one thread per (target, component) adds the tile's terms
+// in index order.  The additions are the reference's, in the reference's
+// order, so the result is bit for bit the serial one.  A source the
+// reference skips (the target itself; massless sources in the massless form;
+// past n) stores +0.0 instead: the running sum starts at +0.0 and a sum in
+// round-to-nearest is -0.0 only when both operands are, so the sum is never
+// -0.0 and adding +0.0 leaves it unchanged bit for bit.  The adds then form
+// one dependent chain of plain FP64 additions per (target, component), with
+// no selects.  TB = 4 for small N (more workgroups to fill the chip), 8
+// otherwise.
+// With LF the kernel is one whole Leapfrog step of the FP64 direct sum: every
+// position it reads (source or target) is first drifted as k_lf_drift drifts
+// it (compensated, src/integrator.c:1008-1018), and the thread that holds a
+// target component's sum then kicks that velocity component as k_lf_kick
+// does (src/integrator.c:1030-1039) and writes the drifted position, so the
+// three kernels' arithmetic is unchanged and one launch replaces three.
+// Positions, their compensation and velocities are read from one set of
+// buffers and written to another (every workgroup reads all positions while
+// others write theirs); a velocity's compensation is read and written only
+// by its own thread, in place.
+struct LfStep {
+    const double *xe = nullptr, *v = nullptr;  // state at the start of the step (with x)
+    double *xo = nullptr, *xeo = nullptr, *vo = nullptr;  // drifted positions, kicked velocities
+    double *ve = nullptr;                                  // velocity compensation
+    double dt = 0.0;
+};
 
-template <bool SKIP_MASSLESS>
+__device__ __forceinline__ double lf_drifted(double x0, double xe, double v, double dt, double &eo)
+{
+    double e = xe + v * dt;
+    const double x1 = x0 + e;
+    e += x0 - x1;
+    eo = e;
+    return x1;
+}
+
+template <int TB, bool SKIP_MASSLESS, bool LF>
 __global__ __launch_bounds__(F64_BLOCK) void k_direct_f64(const double *__restrict__ x,
                                                           const double *__restrict__ m, int n,
                                                           double G, double eps2,
                                                           const int *__restrict__ targets,
                                                           int num_targets,
-                                                          double *__restrict__ a)
+                                                          double *__restrict__ a, LfStep lf)
 {
-    __shared__ double term[F64_TB][3][F64_BLOCK];
-    __shared__ unsigned long long skip[F64_TB][F64_BLOCK / 64];  // one bit per source
-    __shared__ double tx[F64_TB][3];
-    __shared__ int tk[F64_TB];
+    __shared__ double term[TB][3][F64_BLOCK];
+    __shared__ double tx[TB][3];
+    __shared__ int tk[TB];
     const int tid = threadIdx.x;
-    const int t0 = blockIdx.x * F64_TB;
-    if (tid < F64_TB) {
+    const int t0 = blockIdx.x * TB;
+    auto pos = [&](int q) {
+        if (!LF) return x[q];
+        double e;
+        return lf_drifted(x[q], lf.xe[q], lf.v[q], lf.dt, e);
+    };
+    if (tid < TB) {
         const int t = t0 + tid;
         const int k = t < num_targets ? (targets ? targets[t] : t) : -1;
         tk[tid] = k;
-        tx[tid][0] = k >= 0 ? x[3 * k + 0] : 0.0;
-        tx[tid][1] = k >= 0 ? x[3 * k + 1] : 0.0;
-        tx[tid][2] = k >= 0 ? x[3 * k + 2] : 0.0;
+        tx[tid][0] = k >= 0 ? pos(3 * k + 0) : 0.0;
+        tx[tid][1] = k >= 0 ? pos(3 * k + 1) : 0.0;
+        tx[tid][2] = k >= 0 ? pos(3 * k + 2) : 0.0;
     }
     __syncthreads();
-    // accumulator of (target tid / 3, component tid % 3) for tid < 3 F64_TB
+    // accumulator of (target tid / 3, component tid % 3) for tid < 3 TB
     double acc = 0.0;
     const int at = tid / 3, ac = tid % 3;
     for (int base = 0; base < n; base += F64_BLOCK) {
@@ -65,59 +100,100 @@ __global__ __launch_bounds__(F64_BLOCK) void k_direct_f64(const double *__restri
         const bool src = j < n;
         double xj0 = 0.0, xj1 = 0.0, xj2 = 0.0, mj = 0.0;
         if (src) {
-            xj0 = x[3 * j + 0];
-            xj1 = x[3 * j + 1];
-            xj2 = x[3 * j + 2];
+            xj0 = pos(3 * j + 0);
+            xj1 = pos(3 * j + 1);
+            xj2 = pos(3 * j + 2);
             mj = m[j];
         }
 #pragma unroll 2
-        for (int tt = 0; tt < F64_TB; tt++) {
+        for (int tt = 0; tt < TB; tt++) {
             const int k = tk[tt];
             const bool sk = !src || k < 0 || j == k || (SKIP_MASSLESS && mj == 0.0);
-            const unsigned long long bits = __ballot(sk);
-            if ((tid & 63) == 0) skip[tt][tid >> 6] = bits;
+            double t0v = 0.0, t1v = 0.0, t2v = 0.0;
             if (!sk) {
                 const double R0 = xj0 - tx[tt][0];
                 const double R1 = xj1 - tx[tt][1];
                 const double R2 = xj2 - tx[tt][2];
                 const double r = __builtin_sqrt(R0 * R0 + R1 * R1 + R2 * R2 + eps2);
                 const double tv = G / (r * r * r);
-                term[tt][0][tid] = (tv * R0) * mj;
-                term[tt][1][tid] = (tv * R1) * mj;
-                term[tt][2][tid] = (tv * R2) * mj;
+                t0v = (tv * R0) * mj;
+                t1v = (tv * R1) * mj;
+                t2v = (tv * R2) * mj;
             }
+            term[tt][0][tid] = t0v;
+            term[tt][1][tid] = t1v;
+            term[tt][2][tid] = t2v;
         }
         __syncthreads();
-        if (tid < 3 * F64_TB) {
-            // in index order; a skipped source (self, massless, past n) keeps
-            // acc unchanged through a select, so the loads and adds pipeline
+        if (tid < 3 * TB) {
             const double *tp = term[at][ac];
-            for (int w = 0; w < F64_BLOCK / 64; w++) {
-                const unsigned long long bits = skip[at][w];
-#pragma unroll 16
-                for (int jj = 0; jj < 64; jj++) {
-                    const double t = tp[64 * w + jj];
-                    const double s = acc + t;
-                    acc = ((bits >> jj) & 1ull) ? acc : s;
-                }
-            }
+#pragma unroll 32
+            for (int jj = 0; jj < F64_BLOCK; jj++) acc = acc + tp[jj];
         }
         __syncthreads();
     }
-    if (tid < 3 * F64_TB && t0 + at < num_targets) a[3 * (t0 + at) + ac] = acc;
+    if (tid < 3 * TB && t0 + at < num_targets) {
+        const int q = 3 * (t0 + at) + ac;
+        a[q] = acc;
+        if (LF) {
+            double ex;
+            lf.xo[q] = lf_drifted(x[q], lf.xe[q], lf.v[q], lf.dt, ex);
+            lf.xeo[q] = ex;
+            const double v0 = lf.v[q];
+            double e = lf.ve[q] + acc * lf.dt;
+            const double v1 = v0 + e;
+            e += v0 - v1;
+            lf.vo[q] = v1;
+            lf.ve[q] = e;
+        }
+    }
+}
+
+template <bool SKIP_MASSLESS, bool LF = false>
+static void launch_f64_shape(Context *c, int n, const double *x, const double *m, double G,
+                             double eps2, const int *targets, int num_targets, double *out,
+                             const LfStep &lf = LfStep())
+{
+    // 4 targets per workgroup while 8 would leave CUs idle
+    if ((num_targets + 7) / 8 < 2 * c->cu_count)
+        hipLaunchKernelGGL((k_direct_f64<4, SKIP_MASSLESS, LF>), dim3((num_targets + 3) / 4),
+                           dim3(F64_BLOCK), 0, c->stream, x, m, n, G, eps2, targets, num_targets,
+                           out, lf);
+    else
+        hipLaunchKernelGGL((k_direct_f64<8, SKIP_MASSLESS, LF>), dim3((num_targets + 7) / 8),
+                           dim3(F64_BLOCK), 0, c->stream, x, m, n, G, eps2, targets, num_targets,
+                           out, lf);
+}
+
+hipError_t launch_lf_step_f64(Context *c, const double *x, const double *xe, const double *v,
+                              const double *m, int n, double G, double eps, bool skip_massless,
+                              double dt, double *a, double *xo, double *xeo, double *vo, double *ve)
+{
+    LfStep lf;
+    lf.xe = xe;
+    lf.v = v;
+    lf.xo = xo;
+    lf.xeo = xeo;
+    lf.vo = vo;
+    lf.ve = ve;
+    lf.dt = dt;
+    const int rec = timing_begin(c, "direct_f64");
+    if (skip_massless)
+        launch_f64_shape<true, true>(c, n, x, m, G, eps * eps, nullptr, n, a, lf);
+    else
+        launch_f64_shape<false, true>(c, n, x, m, G, eps * eps, nullptr, n, a, lf);
+    timing_end(c, rec);
+    return hipGetLastError();
 }
 
 hipError_t launch_direct_f64(Context *c, const double *x, const double *m, int n, double G,
                              double eps, bool skip_massless, double *a)
 {
-    const int grid = (n + F64_TB - 1) / F64_TB;
     const int rec = timing_begin(c, "direct_f64");
     if (skip_massless)
-        hipLaunchKernelGGL(k_direct_f64<true>, dim3(grid), dim3(F64_BLOCK), 0, c->stream, x, m,
-                           n, G, eps * eps, nullptr, n, a);
+        launch_f64_shape<true>(c, n, x, m, G, eps * eps, nullptr, n, a);
     else
-        hipLaunchKernelGGL(k_direct_f64<false>, dim3(grid), dim3(F64_BLOCK), 0, c->stream, x, m,
-                           n, G, eps * eps, nullptr, n, a);
+        launch_f64_shape<false>(c, n, x, m, G, eps * eps, nullptr, n, a);
     timing_end(c, rec);
     return hipGetLastError();
 }
@@ -128,13 +204,10 @@ hipError_t launch_direct_f64_targets(Context *c, const double *x, const double *
                                      double G, double eps, bool skip_massless,
                                      const int *targets, int num_targets, double *out)
 {
-    const int grid = (num_targets + F64_TB - 1) / F64_TB;
     if (skip_massless)
-        hipLaunchKernelGGL(k_direct_f64<true>, dim3(grid), dim3(F64_BLOCK), 0, c->stream, x, m,
-                           n, G, eps * eps, targets, num_targets, out);
+        launch_f64_shape<true>(c, n, x, m, G, eps * eps, targets, num_targets, out);
     else
-        hipLaunchKernelGGL(k_direct_f64<false>, dim3(grid), dim3(F64_BLOCK), 0, c->stream, x, m,
-                           n, G, eps * eps, targets, num_targets, out);
+        launch_f64_shape<false>(c, n, x, m, G, eps * eps, targets, num_targets, out);
     return hipGetLastError();
 }
 

@@ -51,7 +51,8 @@ enum Slot : int {
     SLOT_PARTIAL,
     // octree
     SLOT_BBOX, SLOT_KEYS, SLOT_KEYS_ALT, SLOT_IDX, SLOT_IDX_ALT, SLOT_HIST, SLOT_HIST_SCAN,
-    SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_SORTED_P, SLOT_MISC, SLOT_GROUP_PARENTS,
+    SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_PACKED_P, SLOT_SORTED_P, SLOT_MISC,
+    SLOT_GROUP_PARENTS,
     // learned correction
     SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD, SLOT_MLP_X, SLOT_SPOT,
     SLOT_COUNT
@@ -141,6 +142,15 @@ Fp32Config &fp32_config();
 // FP64 energy (src/utils.c:27-59 formula; reduction order differs).
 hipError_t launch_energy(Context *c, const double *x, const double *v, const double *m, int n,
                          double G, double *partials, int *num_partials);
+
+// One Leapfrog step with the FP64 direct sum in one launch: drift (x, xe, v)
+// into (xo, xeo), forces of the drifted positions into a, kick v into vo with
+// ve updated in place -- the arithmetic of launch_lf_drift, launch_direct_f64
+// and launch_lf_kick in that order.  The output buffers must not alias the
+// inputs.
+hipError_t launch_lf_step_f64(Context *c, const double *x, const double *xe, const double *v,
+                              const double *m, int n, double G, double eps, bool skip_massless,
+                              double dt, double *a, double *xo, double *xeo, double *vo, double *ve);
 
 // Leapfrog updates (src/integrator.c:973-1094 op order).
 hipError_t launch_lf_half_kick(Context *c, double *v, double *ve, const double *a, int n,

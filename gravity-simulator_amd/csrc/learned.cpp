@@ -112,14 +112,20 @@ ErrorStatus learned_acceleration(Context *c, const double *x, const double *m, i
     timing_end(c, rec);
     M.last_fell_back = 0;
     M.last_p99 = -1.0;
+    M.last_p99_coarse = -1.0;
     if (M.spot_samples > 0) {
         const int ns = std::min(M.spot_samples, n);
         GS_TRY(spot_check(c, x, m, n, G, eps, a, ns, &M.last_p99));
+        M.last_p99_coarse = M.last_p99;
         if (!(M.last_p99 <= M.spot_tol)) {
-            M.last_fell_back = 1;
+            // the group walk with quadrupole moments at the fallback angle
+            // (the tree already holds the second moments), checked again on
+            // the same particles; last_p99 is then the fallback's error
             long long c2 = 0;
-            GS_TRY(tree_walk(c, t, x, G, eps, M.fallback_theta, a, &c2, 1));
+            GS_TRY(tree_walk_group(c, t, G, eps, M.fallback_theta, a, &c2, 2));
             cnt += c2;
+            GS_TRY(spot_check(c, x, m, n, G, eps, a, ns, &M.last_p99));
+            M.last_fell_back = M.last_p99 <= M.spot_tol ? 1 : 2;
         }
     }
     if (interactions) *interactions = cnt;
@@ -214,6 +220,95 @@ extern "C" {
 
 int gs_learned_num_params(void) { return MLP_NPARAM; }
 
+// Training / calibration samples of one particle set: ns particles drawn
+// without replacement (partial Fisher-Yates, 64-bit LCG), their FP32
+// split-precision direct-sum labels, the coarse walk's accelerations of all
+// particles and the raw and walk features of the samples (host copies).
+struct Samples {
+    int n = 0, ns = 0;
+    std::vector<int> idx;
+    std::vector<double> ref, ac;
+    std::vector<float> fs, ws;
+};
+
+static ErrorStatus build_samples(Context *c, const System *system, double eps, double theta,
+                                 int leaf, int samples, unsigned int seed, Samples &S)
+{
+    const int n = system->num_particles;
+    const int ns = std::min(samples, n);
+    const double G = system->G;
+    S.n = n;
+    S.ns = ns;
+    double *x, *m, *a;
+    GS_TRY(upload_xm(c, system, &x, &m, &a));
+    float *F, *wf;
+    GS_TRY(coarse_walk(c, x, m, n, G, eps, theta, leaf_or_default(leaf), a, false, &F, nullptr, &wf));
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    unsigned long long st = 0x853C49E6748FEA9Bull ^ (static_cast<unsigned long long>(seed) << 1);
+    for (int k = 0; k < ns; k++) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        const int j = k + static_cast<int>((st >> 33) % static_cast<unsigned long long>(n - k));
+        std::swap(perm[k], perm[j]);
+    }
+    S.idx.assign(perm.begin(), perm.begin() + ns);
+    // labels from the FP32 split-precision direct sum
+    void *sp, *buf;
+    const size_t lb = direct_f32_targets_bytes(n, ns);
+    GS_TRY(slot(c, SLOT_PARTIAL, lb, &buf));
+    auto up = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t off_ref = up(sizeof(int) * ns);
+    const size_t off_feat = off_ref + up(sizeof(double) * 3 * ns);
+    const size_t off_wf = off_feat + up(sizeof(float) * MLP_F * static_cast<size_t>(ns));
+    GS_TRY(slot(c, SLOT_SPOT, off_wf + sizeof(float) * MLP_WALK_FEAT * static_cast<size_t>(ns),
+                &sp));
+    int *didx = static_cast<int *>(sp);
+    double *dref = reinterpret_cast<double *>(static_cast<char *>(sp) + off_ref);
+    float *dfs = reinterpret_cast<float *>(static_cast<char *>(sp) + off_feat);
+    float *dwf = reinterpret_cast<float *>(static_cast<char *>(sp) + off_wf);
+    GS_HIP(hipMemcpyAsync(didx, S.idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
+    GS_HIP(launch_direct_f32_targets(c, x, m, n, G, eps, didx, ns, dref, buf, lb));
+    GS_HIP(mlp_gather_rows(c, F, didx, ns, MLP_F, dfs));
+    GS_HIP(mlp_gather_rows(c, wf, didx, ns, MLP_WALK_FEAT, dwf));
+    S.ref.resize(3 * static_cast<size_t>(ns));
+    S.ac.resize(3 * static_cast<size_t>(n));
+    S.fs.resize(static_cast<size_t>(MLP_F) * ns);
+    S.ws.resize(static_cast<size_t>(MLP_WALK_FEAT) * ns);
+    GS_HIP(hipMemcpyAsync(S.ref.data(), dref, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipMemcpyAsync(S.ac.data(), a, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipMemcpyAsync(S.fs.data(), dfs, sizeof(float) * MLP_F * ns, hipMemcpyDeviceToHost,
+                          c->stream));
+    GS_HIP(hipMemcpyAsync(S.ws.data(), dwf, sizeof(float) * MLP_WALK_FEAT * ns,
+                          hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return make_success_error_status();
+}
+
+// Network outputs (host) of samples [k0, k0 + nh) with the model's feature
+// normalisation.
+static ErrorStatus forward_samples(Context *c, const Samples &S, int k0, int nh,
+                                   std::vector<float> &Yh)
+{
+    MlpModel &M = mlp_model();
+    std::vector<float> Xh(static_cast<size_t>(nh) * MLP_F);
+    for (int h = 0; h < nh; h++)
+        for (int f = 0; f < MLP_F; f++)
+            Xh[static_cast<size_t>(MLP_F) * h + f] =
+                (S.fs[static_cast<size_t>(MLP_F) * (k0 + h) + f] - M.fmean[f]) / M.fstd[f];
+    const size_t xb = ((sizeof(float) * Xh.size() + 255) / 256) * 256;
+    void *p;
+    GS_TRY(slot(c, SLOT_MLP_X, xb + sizeof(float) * 3 * static_cast<size_t>(nh), &p));
+    float *dX = static_cast<float *>(p);
+    float *dY = reinterpret_cast<float *>(static_cast<char *>(p) + xb);
+    GS_HIP(hipMemcpyAsync(dX, Xh.data(), sizeof(float) * Xh.size(), hipMemcpyHostToDevice, c->stream));
+    GS_TRY(mlp_forward_rows(c, dX, nh, dY));
+    Yh.resize(3 * static_cast<size_t>(nh));
+    GS_HIP(hipMemcpyAsync(Yh.data(), dY, sizeof(float) * 3 * nh, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    return make_success_error_status();
+}
+
 ErrorStatus gs_learned_train(const System *system, double softening_length, double theta_coarse,
                              int max_num_particles_per_leaf, int samples, int epochs, int batch,
                              double lr, unsigned int seed, double *final_loss)
@@ -229,57 +324,14 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     GS_TRY(mlp_ensure(c));
     MlpModel &M = mlp_model();
-    const int n = system->num_particles;
-    const int ns = std::min(samples, n);
     const double G = system->G;
-    double *x, *m, *a;
-    GS_TRY(upload_xm(c, system, &x, &m, &a));
-    float *F, *wf;
-    GS_TRY(coarse_walk(c, x, m, n, G, softening_length, theta_coarse,
-                       leaf_or_default(max_num_particles_per_leaf), a, false, &F, nullptr, &wf));
-
-    // sample without replacement (partial Fisher-Yates, 64-bit LCG)
-    std::vector<int> perm(n);
-    for (int i = 0; i < n; i++) perm[i] = i;
-    unsigned long long st = 0x853C49E6748FEA9Bull ^ (static_cast<unsigned long long>(seed) << 1);
-    for (int k = 0; k < ns; k++) {
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        const int j = k + static_cast<int>((st >> 33) % static_cast<unsigned long long>(n - k));
-        std::swap(perm[k], perm[j]);
-    }
-    std::vector<int> idx(perm.begin(), perm.begin() + ns);
-    perm.clear();
-    perm.shrink_to_fit();
-
-    // labels from the FP32 split-precision direct sum
-    void *sp, *buf;
-    const size_t lb = direct_f32_targets_bytes(n, ns);
-    GS_TRY(slot(c, SLOT_PARTIAL, lb, &buf));
-    auto up = [](size_t b) { return ((b + 255) / 256) * 256; };
-    const size_t off_ref = up(sizeof(int) * ns);
-    const size_t off_feat = off_ref + up(sizeof(double) * 3 * ns);
-    const size_t off_wf = off_feat + up(sizeof(float) * MLP_F * static_cast<size_t>(ns));
-    GS_TRY(slot(c, SLOT_SPOT, off_wf + sizeof(float) * MLP_WALK_FEAT * static_cast<size_t>(ns),
-                &sp));
-    int *didx = static_cast<int *>(sp);
-    double *dref = reinterpret_cast<double *>(static_cast<char *>(sp) + off_ref);
-    float *dfs = reinterpret_cast<float *>(static_cast<char *>(sp) + off_feat);
-    float *dwf = reinterpret_cast<float *>(static_cast<char *>(sp) + off_wf);
-    GS_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
-    GS_HIP(launch_direct_f32_targets(c, x, m, n, G, softening_length, didx, ns, dref, buf, lb));
-    GS_HIP(mlp_gather_rows(c, F, didx, ns, MLP_F, dfs));
-    GS_HIP(mlp_gather_rows(c, wf, didx, ns, MLP_WALK_FEAT, dwf));
-    std::vector<double> ref(3 * static_cast<size_t>(ns)), ac(3 * static_cast<size_t>(n));
-    std::vector<float> fs(static_cast<size_t>(MLP_F) * ns),
-        ws(static_cast<size_t>(MLP_WALK_FEAT) * ns);
-    GS_HIP(hipMemcpyAsync(ref.data(), dref, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
-                          c->stream));
-    GS_HIP(hipMemcpyAsync(ac.data(), a, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
-    GS_HIP(hipMemcpyAsync(fs.data(), dfs, sizeof(float) * MLP_F * ns, hipMemcpyDeviceToHost,
-                          c->stream));
-    GS_HIP(hipMemcpyAsync(ws.data(), dwf, sizeof(float) * MLP_WALK_FEAT * ns,
-                          hipMemcpyDeviceToHost, c->stream));
-    GS_HIP(hipStreamSynchronize(c->stream));
+    Samples S;
+    GS_TRY(build_samples(c, system, softening_length, theta_coarse, max_num_particles_per_leaf,
+                         samples, seed, S));
+    const int ns = S.ns;
+    const std::vector<int> &idx = S.idx;
+    const std::vector<double> &ref = S.ref, &ac = S.ac;
+    const std::vector<float> &fs = S.fs, &ws = S.ws;
 
     // targets: the residual left after the quadrupole sum, in units of the
     // next order's scale G s3 and in the particle's local frame,
@@ -358,9 +410,37 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
         GS_HIP(hipStreamSynchronize(c->stream));
         GS_TRY(choose_output_scale(ac, ref, ws, idx, nt, nh, Yh, G, tscale, &alpha));
     }
+    M.tscale_base = static_cast<float>(tscale);
     M.tscale = static_cast<float>(tscale * alpha);
     GS_HIP(hipStreamSynchronize(c->stream));
     M.trained = true;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_learned_calibrate(const System *system, double softening_length, double theta_coarse,
+                                 int max_num_particles_per_leaf, int samples, unsigned int seed)
+{
+    GS_TRY(check_system(system));
+    if (samples < 20 || theta_coarse < 0.0 || softening_length < 0.0)
+        return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR,
+                                "samples must be at least 20; theta and softening non-negative");
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    MlpModel &M = mlp_model();
+    if (!M.trained)
+        return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR,
+                                "The learned Barnes-Hut correction has no trained network to "
+                                "calibrate");
+    Samples S;
+    GS_TRY(build_samples(c, system, softening_length, theta_coarse, max_num_particles_per_leaf,
+                         samples, seed, S));
+    std::vector<float> Yh;
+    GS_TRY(forward_samples(c, S, 0, S.ns, Yh));
+    double alpha = 0.0;
+    GS_TRY(choose_output_scale(S.ac, S.ref, S.ws, S.idx, 0, S.ns, Yh, system->G, M.tscale_base,
+                               &alpha));
+    M.tscale = static_cast<float>(M.tscale_base * alpha);
     return make_success_error_status();
 }
 
@@ -407,6 +487,7 @@ ErrorStatus gs_learned_set_params(const float *params, const float *feature_mean
     std::memcpy(M.fmean, feature_mean, sizeof(M.fmean));
     std::memcpy(M.fstd, feature_std, sizeof(M.fstd));
     M.tscale = target_scale;
+    M.tscale_base = target_scale;
     M.trained = true;
     return make_success_error_status();
 }

@@ -81,7 +81,8 @@ struct MlpModel {
     float *v = nullptr;  // device: Adam second moments
     float fmean[MLP_F];  // feature normalisation (host copy)
     float fstd[MLP_F];
-    float tscale = 1.0f;  // scale of the targets (residual / (G s3))
+    float tscale = 1.0f;  // scale of the targets (residual / (G s3)) times the output scale
+    float tscale_base = 1.0f;  // the training's target scale (output scale 1)
     bool trained = false;
     // held-out evaluation of the last training run (relative errors against
     // the direct sum): output scale chosen, p99 / median with the network at
@@ -91,9 +92,12 @@ struct MlpModel {
     // runtime policy of method BARNES_HUT_LEARNED
     double spot_tol = 1e-3;      // p99 relative error that triggers the fallback
     int spot_samples = 1024;     // particles checked against the FP32 direct sum
-    double fallback_theta = 0.5;  // plain monopole walk used on fallback
-    // status of the last evaluation
+    double fallback_theta = 0.5;  // group walk with quadrupoles used on fallback
+    // status of the last evaluation: p99 of the returned result and of the
+    // coarse corrected walk; 0 no fallback, 1 fell back and the fallback met
+    // the tolerance, 2 fell back and the fallback missed it too
     double last_p99 = -1.0;
+    double last_p99_coarse = -1.0;
     int last_fell_back = 0;
 };
 

@@ -51,7 +51,11 @@ struct DeviceTree {
 size_t radix_hist_elements(int n);
 ErrorStatus radix_sort_pairs(Context *c, long long *k0, int *i0, long long *k1, int *i1, int n,
                              int passes, unsigned *hist, unsigned *stmp, long long **ko, int **io);
+// Bounding box, Morton keys and the stable sort.  With m and packed not
+// NULL the key kernel also writes packed[i] = (x_i, y_i, z_i, m_i) in the
+// input order, for the gather into sorted order.
 ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_center,
-                      double box_width, DeviceTree *t);
+                      double box_width, DeviceTree *t, const double *m = nullptr,
+                      double4 *packed = nullptr);
 
 }  // namespace gs

@@ -128,17 +128,23 @@ __device__ __forceinline__ long long quantise(double xv, double c, double w)
     return static_cast<long long>(s);
 }
 
+// With packed != NULL it also writes (x, y, z, m) per particle as one
+// 32-byte record: the gather into sorted order then reads one aligned record
+// per particle instead of a 24-byte and an 8-byte piece of two arrays.
 __global__ void k_morton(const double *__restrict__ x, int n, const double *__restrict__ box,
-                         long long *__restrict__ keys, int *__restrict__ idx)
+                         long long *__restrict__ keys, int *__restrict__ idx,
+                         const double *__restrict__ m, double4 *__restrict__ packed)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double w = box[3];
-    const long long nx = spread21(quantise(x[3 * i + 0], box[0], w));
-    const long long ny = spread21(quantise(x[3 * i + 1], box[1], w));
-    const long long nz = spread21(quantise(x[3 * i + 2], box[2], w));
+    const double x0 = x[3 * i + 0], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+    const long long nx = spread21(quantise(x0, box[0], w));
+    const long long ny = spread21(quantise(x1, box[1], w));
+    const long long nz = spread21(quantise(x2, box[2], w));
     keys[i] = nx | (ny << 1) | (nz << 2);
     idx[i] = i;
+    if (packed) packed[i] = make_double4(x0, x1, x2, m[i]);
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -262,14 +268,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__rest
 }
 
 // ------------------------------------------------------------------ gather
-__global__ void k_gather_particles(const double *__restrict__ x, const double *__restrict__ m,
+__global__ void k_gather_particles(const double4 *__restrict__ packed,
                                    const int *__restrict__ sidx, int n,
                                    double4 *__restrict__ P)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const int i = sidx[s];
-    P[s] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], m[i]);
+    P[s] = packed[sidx[s]];
 }
 
 // ------------------------------------------------------------------ build
@@ -1307,8 +1312,9 @@ ErrorStatus radix_sort_pairs(Context *c, long long *k0, int *i0, long long *k1, 
 }
 
 ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_center,
-                      double box_width, DeviceTree *t)
+                      double box_width, DeviceTree *t, const double *m, double4 *packed)
 {
+    if (!m) packed = nullptr;
     if (n <= 0) return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "Number of particles must be positive");
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
     if (static_cast<long long>(ntiles) * RS_BINS > scan_max_elements())
@@ -1342,7 +1348,7 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
         hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_BLOCK), 0, c->stream, bpart, nb, box);
     }
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, n, box, k0,
-                       i0);
+                       i0, m, packed);
     GS_HIP(hipGetLastError());
     long long *ks;
     int *is;
@@ -1445,12 +1451,14 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
     // an unfinalised parameter (-1) means the reference's default of one
     // particle per leaf (src/acceleration.c:73-129)
     if (leaf_cap < 1) leaf_cap = 1;
-    GS_TRY(tree_sort(c, x, n, box_center, box_width, t));
-    t->leaf_cap = leaf_cap;
     void *p;
+    GS_TRY(slot(c, SLOT_PACKED_P, sizeof(double4) * n, &p));
+    double4 *packed = static_cast<double4 *>(p);
+    GS_TRY(tree_sort(c, x, n, box_center, box_width, t, m, packed));
+    t->leaf_cap = leaf_cap;
     GS_TRY(slot(c, SLOT_SORTED_P, sizeof(double4) * n, &p));
     t->P = static_cast<double4 *>(p);
-    hipLaunchKernelGGL(k_gather_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, x, m,
+    hipLaunchKernelGGL(k_gather_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, packed,
                        t->sidx, n, t->P);
     GS_HIP(hipGetLastError());
 

@@ -91,12 +91,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const unsigned *__res
     }
 }
 
-long long scan_max_elements() { return static_cast<long long>(SCAN_MAX_TILES) * SCAN_TILE; }
+// Beyond SCAN_MAX_TILES tiles the tile sums are scanned by the same three
+// kernels one level up (their temporary storage follows this level's).
+long long scan_max_elements() { return static_cast<long long>(SCAN_TILE) * 0x7fffffffLL; }
 
 size_t scan_temp_bytes(long long n)
 {
     const long long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    return static_cast<size_t>(tiles + 1) * sizeof(unsigned);
+    size_t b = static_cast<size_t>(tiles + 1) * sizeof(unsigned);
+    if (tiles > SCAN_MAX_TILES) b += scan_temp_bytes(tiles);
+    return b;
 }
 
 hipError_t scan_exclusive(hipStream_t stream, const unsigned *in, unsigned *out, long long n,
@@ -104,11 +108,16 @@ hipError_t scan_exclusive(hipStream_t stream, const unsigned *in, unsigned *out,
 {
     if (n <= 0) return hipSuccess;
     const long long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (tiles > SCAN_MAX_TILES) return hipErrorInvalidValue;
+    if (n > scan_max_elements()) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scan_tilesum, dim3(static_cast<unsigned>(tiles)), dim3(SCAN_BLOCK), 0,
                        stream, in, n, temp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, stream, temp,
-                       static_cast<int>(tiles), total_out);
+    if (tiles <= SCAN_MAX_TILES) {
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, stream, temp,
+                           static_cast<int>(tiles), total_out);
+    } else {
+        const hipError_t e = scan_exclusive(stream, temp, temp, tiles, temp + tiles + 1, total_out);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(tiles)), dim3(SCAN_BLOCK), 0,
                        stream, in, out, n, temp);
     return hipGetLastError();

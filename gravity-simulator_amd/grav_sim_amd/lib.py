@@ -40,7 +40,8 @@ EXPORTS = [
     "gs_set_fp32_config", "gs_get_fp32_config", "gs_acceleration_noalloc", "gs_copy_calibrate", "gs_whfast_ensemble",
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
     "gs_learned_reset", "gs_learned_forward", "gs_learned_gradients", "gs_learned_features",
-    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "ias15", "whfast",
+    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "gs_learned_calibrate",
+    "ias15", "whfast",
     "rk_embedded",
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
     "keplerian_to_cartesian_python", "compute_linear_momentum_python", "compute_angular_momentum_python",
@@ -135,6 +136,7 @@ def load(path=None):
         "gs_learned_set_policy": (ES, [C.c_double, C.c_int, C.c_double]),
         "gs_learned_status": (ES, [PD, PI]),
         "gs_learned_training_report": (ES, [PD, PD, PD, PD, PD]),
+        "gs_learned_calibrate": (ES, [pS, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint]),
         "ias15": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
                        C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
         "whfast": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
@@ -444,6 +446,16 @@ def learned_train(x, m, G=1.0, eps=0.0, theta=0.8, leaf=1, samples=1 << 20, epoc
     return loss.value
 
 
+def learned_calibrate(x, m, G=1.0, eps=0.0, theta=0.8, leaf=1, samples=1 << 16, seed=1):
+    """Choose the network's output scale on another particle set (a
+    validation realisation); the report then describes these samples."""
+    L = load()
+    x, m = _c(x), _c(m)
+    s = _abi.make_system(x, np.zeros_like(x), m, G)
+    check(L.gs_learned_calibrate(C.byref(s), float(eps), float(theta), int(leaf), int(samples), int(seed)),
+          "gs_learned_calibrate")
+
+
 def learned_get_params():
     L = load()
     w = np.zeros(learned_num_params(), dtype=np.float32)
@@ -501,10 +513,12 @@ def learned_set_policy(tol=1e-3, samples=1024, fallback_theta=0.5):
 
 
 def learned_status():
+    """(p99 of the result returned, fallback status): 0 no fallback, 1 the
+    fallback met the tolerance, 2 it did not."""
     p = C.c_double(0.0)
     f = C.c_int(0)
     check(load().gs_learned_status(C.byref(p), C.byref(f)), "gs_learned_status")
-    return p.value, bool(f.value)
+    return p.value, f.value
 
 
 def learned_training_report():

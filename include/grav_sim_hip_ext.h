@@ -28,8 +28,8 @@ extern "C" {
 /* As above, massive sources only (src/acceleration.c:236-367 physics). */
 #define ACCELERATION_METHOD_MASSLESS_FP32 17
 /* Barnes-Hut walk at a coarse opening angle plus a learned residual
- * correction; falls back to the plain walk at theta = 0.5 when a direct-sum
- * spot check fails (see gs_learned_*). */
+ * correction; falls back to the group walk with quadrupoles at theta = 0.5
+ * when a direct-sum spot check fails (see gs_learned_*). */
 #define ACCELERATION_METHOD_BARNES_HUT_LEARNED 18
 /* Barnes-Hut with quadrupole moments (same tree and opening criterion as
  * ACCELERATION_METHOD_BARNES_HUT, multipole order 2). */
@@ -172,7 +172,8 @@ ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs);
  * coarse result.  Every evaluation then spot-checks
  * spot_samples evenly strided particles against the FP32 split-precision
  * direct sum; if their 99th-percentile relative error exceeds spot_tol the
- * result is replaced by the plain monopole walk at fallback_theta.
+ * result is replaced by the group walk with quadrupole terms at
+ * fallback_theta, which is checked again on the same particles.
  * Parameters: weights W1 (16x128), b1, W2 (128x128), b2, W3 (128x128), b3,
  * W4 (128x3), b4, each matrix row-major as (inputs, outputs), in one flat
  * FP32 array of gs_learned_num_params() values.                              */
@@ -200,6 +201,12 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
  * scale and without it.  Any pointer may be NULL. */
 ErrorStatus gs_learned_training_report(double *alpha, double *p99_net, double *p99_without,
                                        double *median_net, double *median_without);
+/* Chooses the output scale again, on `samples` particles of another particle
+ * set (a validation realisation): the same candidates and rule as in
+ * gs_learned_train, the trained weights unchanged; the report then holds
+ * these samples' statistics. */
+ErrorStatus gs_learned_calibrate(const System *system, double softening_length, double theta_coarse,
+                                 int max_num_particles_per_leaf, int samples, unsigned int seed);
 /* Copies out / installs the parameters, the feature normalisation (16 means
  * and 16 standard deviations) and the target scale.  set marks the network
  * trained; reset marks it untrained (method 18 then fails with a value
@@ -221,14 +228,18 @@ ErrorStatus gs_learned_features(const System *system, double softening_length, d
                                 int max_num_particles_per_leaf, double *a_coarse,
                                 float *features);
 /* Spot-check policy (defaults 1e-3, 1024, 0.5; spot_samples 0 disables the
- * check) and the outcome of the last evaluation. */
+ * check) and the outcome of the last evaluation: last_p99 the spot check's
+ * 99th-percentile error of the result returned (of the fallback when there
+ * was one); fell_back 0 none, 1 fallback within spot_tol, 2 fallback still
+ * above spot_tol. */
 ErrorStatus gs_learned_set_policy(double spot_tol, int spot_samples, double fallback_theta);
 ErrorStatus gs_learned_status(double *last_p99, int *fell_back);
 
 /* ---- FP32 direct-sum autotuner ---------------------------------------------
  * Times the candidate (block, targets-per-thread, source-split) shapes on an
  * N-particle input resident on the device and keeps the fastest for later
- * calls.  Writes the chosen shape.  block = 0 denotes the scalar-source
+ * calls -- among the shapes within 1 % of the fastest, the one with the
+ * smallest scratch (partial sums) -- and writes the chosen shape.  block = 0 denotes the scalar-source
  * kernel (independent wavefronts reading pre-converted source tiles through
  * wave-uniform loads); block = 4, 8 or 16 the shared-target scalar-source
  * kernel (that many wavefronts per 64 targets, each over its share of the
@@ -241,6 +252,7 @@ ErrorStatus gs_autotune_fp32(int num_particles, int *block, int *targets_per_thr
 ErrorStatus gs_set_fp32_config(int block, int targets_per_thread, int source_split);
 /* The shape later FP32 direct sums use (default: block 16, 1, automatic split). */
 ErrorStatus gs_get_fp32_config(int *block, int *targets_per_thread, int *source_split);
+
 /* ---- direct sums without device allocation ---------------------------------
  * No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -254,7 +266,6 @@ ErrorStatus gs_get_fp32_config(int *block, int *targets_per_thread, int *source_
  * Calls that overlap on one GPU share that array and race. */
 ErrorStatus gs_acceleration_noalloc(double *a, const System *system,
                                     const AccelerationParam *acceleration_param);
-
 
 #ifdef __cplusplus
 }

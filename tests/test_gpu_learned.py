@@ -120,15 +120,31 @@ def test_untrained_network_fails_loudly():
         L.acceleration(x, m, 18, theta=0.8, leaf=1)
 
 
-def test_forced_fallback_is_the_plain_walk():
+def test_forced_fallback_is_the_group_quadrupole_walk_checked_again():
+    """A spot check that fails replaces the result by the group walk with
+    quadrupoles at the fallback angle and checks that again: the status
+    then reports the fallback's own p99, and 2 when it misses the tolerance
+    too (here a tolerance nothing meets)."""
     x, v, m = ic.two_spheres(1 << 15, 43)
     L.learned_set_params(NN.random_params(13).astype(np.float32) * 1e-3)
     L.learned_set_policy(1e-12, 256, 0.5)
     a = L.acceleration(x, m, 18, theta=0.8, leaf=1)
     p99, fell_back = L.learned_status()
-    assert fell_back and p99 > 1e-12
-    a_plain = L.acceleration(x, m, 3, theta=0.5, leaf=1)
-    assert np.array_equal(a, a_plain)
+    assert fell_back == 2 and p99 > 1e-12
+    a_fb = L.acceleration(x, m, 21, theta=0.5, leaf=1)
+    assert np.array_equal(a, a_fb)
+    # the reported p99 is the fallback's: the spot check of the group walk itself
+    L.learned_set_policy(1.0, 256, 0.5)
+    sim = L.Sim(x, v, m, 1.0)
+    med, p99_fb, mx = sim.accel_error(L.param(21, theta=0.5, leaf=1), samples=256)
+    sim.close()
+    assert abs(p99 - p99_fb) <= 1e-3 * p99_fb + 1e-6, (p99, p99_fb)
+    # a tolerance the fallback meets: status 1
+    L.learned_set_policy(1.5 * p99, 256, 0.5)
+    L.learned_set_params(NN.random_params(13).astype(np.float32) * 1e3)  # a network far off: fails the check
+    L.acceleration(x, m, 18, theta=0.8, leaf=1)
+    p99b, fell_back = L.learned_status()
+    assert fell_back == 1 and p99b <= 1.5 * p99
 
 
 @pytest.fixture(scope="module")
@@ -148,9 +164,16 @@ def test_training_reduces_loss(trained):
 
 
 def test_learned_gain_over_coarse_walk(trained):
-    """Ablation on a particle set the network has not seen: the learned
-    correction against the coarse walk alone and against its quadrupole sum
-    alone (the network switched off)."""
+    """Ablation on a particle set neither training nor calibration has seen.
+    The output scale is chosen on a second, independent realisation (seed
+    54); on a third (seed 52) the learned correction must not be worse than
+    its quadrupole sum alone (the network switched off, method 19) in median
+    or p99 -- with a network that does not generalise the calibration picks
+    scale 0, which is then that same quadrupole result -- and must beat the
+    coarse monopole walk."""
+    xc, vc, mc = ic.two_spheres(1 << 17, 54)
+    L.learned_calibrate(xc, mc, eps=1e-3, theta=0.8, leaf=1, samples=1 << 16, seed=9)
+    rep = L.learned_training_report()
     x, v, m = ic.two_spheres(1 << 17, 52)
     L.learned_set_policy(1.0, 0, 0.5)  # no spot check: the network's own result
     sim = L.Sim(x, v, m, 1.0)
@@ -158,10 +181,13 @@ def test_learned_gain_over_coarse_walk(trained):
     learned = sim.accel_error(L.param(18, theta=0.8, eps=1e-3, leaf=1), samples=4096)
     quad = sim.accel_error(L.param(19, theta=0.8, eps=1e-3, leaf=1), samples=4096)
     sim.close()
-    print(f"theta=0.8 median/p99: coarse {coarse[0]:.2e}/{coarse[1]:.2e}, "
+    print(f"calibration {rep}; theta=0.8 median/p99: coarse {coarse[0]:.2e}/{coarse[1]:.2e}, "
           f"learned {learned[0]:.2e}/{learned[1]:.2e}, quadrupole sum only {quad[0]:.2e}/{quad[1]:.2e}")
     assert learned[0] < 0.6 * coarse[0] and learned[1] < 0.6 * coarse[1]
-    assert learned[0] <= 1.05 * quad[0] and learned[1] <= 1.05 * quad[1]
+    # (1e-4: the FP32 accumulation of the quadrupole sum differs between the two paths)
+    assert learned[0] <= (1 + 1e-4) * quad[0] and learned[1] <= (1 + 1e-4) * quad[1]
+    if rep["alpha"] == 0.0:
+        assert rep["p99_net"] == rep["p99_without"]
 
 
 def test_spot_check_reports_true_error(trained):

@@ -263,3 +263,18 @@ def test_group_walk_interaction_set_at_full_size_config4():
     idx = np.random.default_rng(4).choice(len(m), 20000, replace=False)
     e = np.linalg.norm(a_g[idx] - a_o[idx], axis=1) / np.linalg.norm(a_o[idx], axis=1)
     assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
+def test_group_walk_beyond_one_scan_level():
+    """More tree nodes than one level of the device-wide scan covers (2048 x
+    4096): the group walk's parent scan then takes a second level.  The forces
+    must still agree with the per-particle walk's accuracy class."""
+    x, v, m = ic.uniform_cube(6_000_000, 61)
+    t = L.construct_octree(x, m, leaf=1)
+    assert t["num_nodes"] > 2048 * 4096, t["num_nodes"]
+    del t
+    sim = L.Sim(x, v, m, 1.0)
+    med, p99, mx = sim.accel_error(L.param(20, theta=0.5, eps=1e-3, leaf=1), samples=2048)
+    sim.close()
+    print(f"group walk N=6e6 uniform: median {med:.2e} p99 {p99:.2e}")
+    assert med < 1e-3 and p99 < 5e-3
