@@ -167,10 +167,11 @@ struct DeviceTree;  // defined in octree.hip
 // moments, which the quadrupole and feature walks need.
 ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
                        const double *box_center, double box_width, bool with_quad,
-                       DeviceTree **out);
+                       DeviceTree **out, bool with_oct = false);
 // Walk; writes a (device, AoS, original order).  order 1: monopoles (the
 // reference's), 2: with quadrupole terms.  feat != NULL: monopole walk that
-// also writes 8 FP32 features per particle for the learned correction.
+// also writes 16 FP32 features per particle for the learned correction (the
+// tree built with second and third moments).
 ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, double eps,
                       double theta, double *a, long long *interactions, int order = 1,
                       float *feat = nullptr);

@@ -77,7 +77,7 @@ static ErrorStatus coarse_walk(Context *c, const double *x, const double *m, int
                                float **F_out, DeviceTree **t_out, float **wf_out = nullptr)
 {
     DeviceTree *t;
-    GS_TRY(tree_build(c, x, m, n, leaf, nullptr, -1.0, true, &t));
+    GS_TRY(tree_build(c, x, m, n, leaf, nullptr, -1.0, true, &t, true));
     void *wf, *F;
     GS_TRY(slot(c, SLOT_FEAT, sizeof(float) * MLP_WALK_FEAT * static_cast<size_t>(n), &wf));
     GS_TRY(slot(c, SLOT_MLP_X, sizeof(float) * MLP_F * static_cast<size_t>(n), &F));
@@ -186,10 +186,11 @@ static ErrorStatus choose_output_scale(const std::vector<double> &ac, const std:
             double e1[3], e2[3], e3[3];
             mlp_local_frame(ak, q, v, e1, e2, e3);
             const double s = G * static_cast<double>(wk[7]) * tscale * alpha;
+            const double o[3] = {wk[10], wk[11], wk[12]};
             const float *y = &Yh[3 * static_cast<size_t>(h)];
             double dn = 0.0, rn = 0.0;
             for (int d = 0; d < 3; d++) {
-                const double pd = ak[d] + G * q[d] + s * (y[0] * e1[d] + y[1] * e2[d] + y[2] * e3[d]);
+                const double pd = ak[d] + G * (q[d] + o[d]) + s * (y[0] * e1[d] + y[1] * e2[d] + y[2] * e3[d]);
                 const double rf = ref[3 * static_cast<size_t>(k) + d];
                 dn += (pd - rf) * (pd - rf);
                 rn += rf * rf;
@@ -333,9 +334,9 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
     const std::vector<double> &ref = S.ref, &ac = S.ac;
     const std::vector<float> &fs = S.fs, &ws = S.ws;
 
-    // targets: the residual left after the quadrupole sum, in units of the
-    // next order's scale G s3 and in the particle's local frame,
-    //   r = (a_direct - a_coarse - G q) / (G s3),  T = (r.e1, r.e2, r.e3),
+    // targets: the residual left after the quadrupole and octupole sums, in
+    // units of the next orders' scale G s3 and in the particle's local frame,
+    //   r = (a_direct - a_coarse - G (q + o)) / (G s3),  T = (r.e1, r.e2, r.e3),
     // with each sample's norm clipped at the 99.9th percentile, scaled to unit
     // RMS per component
     std::vector<double> r(3 * static_cast<size_t>(ns)), rn(ns);
@@ -346,9 +347,10 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
         double e1[3], e2[3], e3[3];
         mlp_local_frame(ak, q, v, e1, e2, e3);
         const double s3 = G * static_cast<double>(wk[7]);
+        const double o[3] = {wk[10], wk[11], wk[12]};
         double rg[3];
         for (int d = 0; d < 3; d++)
-            rg[d] = s3 > 0.0 ? (ref[3 * k + d] - ak[d] - G * q[d]) / s3 : 0.0;
+            rg[d] = s3 > 0.0 ? (ref[3 * k + d] - ak[d] - G * (q[d] + o[d])) / s3 : 0.0;
         r[3 * k + 0] = rg[0] * e1[0] + rg[1] * e1[1] + rg[2] * e1[2];
         r[3 * k + 1] = rg[0] * e2[0] + rg[1] * e2[1] + rg[2] * e2[2];
         r[3 * k + 2] = rg[0] * e3[0] + rg[1] * e3[1] + rg[2] * e3[2];

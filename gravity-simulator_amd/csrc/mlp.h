@@ -13,7 +13,7 @@ namespace gs {
 constexpr int MLP_F = 16;   // input features
 constexpr int MLP_H = 128;  // hidden width
 constexpr int MLP_O = 3;    // outputs (residual / (|a| * tscale))
-constexpr int MLP_WALK_FEAT = 12;  // walk accumulators per particle
+constexpr int MLP_WALK_FEAT = 16;  // walk accumulators per particle
 
 constexpr int MLP_OFF_W1 = 0;
 constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_F * MLP_H;

@@ -15,10 +15,10 @@
 //   k_adam        Adam on the FP32 master weights
 //   k_features    per-particle input features from the coarse walk
 //   k_mlp_infer   fused inference + correction for method BARNES_HUT_LEARNED:
-//                 a = a_coarse + G q + G s3 * tscale * (y in the local frame)
-//                 (q: the walk's quadrupole sum, the physics part of the
-//                 predicted residual; s3 = sum M l^3 / d^5, the scale of the
-//                 next multipole order)
+//                 a = a_coarse + G (q + o) + G s3 * tscale * (y in the local frame)
+//                 (q, o: the walk's quadrupole and octupole sums, the physics
+//                 part of the predicted residual; s3 = sum M l^3 / d^5, the
+//                 scale of the next multipole orders)
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -343,7 +343,7 @@ __global__ void k_features(const double *__restrict__ a, const float *__restrict
     const double g = an > 0.0 ? G / an : 0.0;
     auto dot = [](const double *x, const double *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
     const double s2 = g * w[6], s3 = g * w[7];
-    const double qn = g * sqrt(dot(q, q)), vn = sqrt(dot(v, v));
+    const double qn = g * sqrt(dot(q, q));
     float f[MLP_F];
     f[0] = static_cast<float>(g * dot(q, e1));
     f[1] = static_cast<float>(g * dot(q, e2));
@@ -358,9 +358,14 @@ __global__ void k_features(const double *__restrict__ a, const float *__restrict
     f[10] = static_cast<float>(log10(fmax(s3, 1e-12)));
     f[11] = static_cast<float>(log10(fmax(s2, 1e-12)));
     f[12] = static_cast<float>(log10(fmax(qn, 1e-12)));
-    f[13] = w[7] > 0.0f ? static_cast<float>(vn / static_cast<double>(w[7])) : 0.0f;
-    f[14] = 0.0f;
-    f[15] = 0.0f;
+    // the walk's octupole sum in the local frame, in units of the target's
+    // scale s3 (both without G): the physics of the next order, whose
+    // residual the network predicts
+    const double oc[3] = {w[10], w[11], w[12]};
+    const double is3 = w[7] > 0.0f ? 1.0 / static_cast<double>(w[7]) : 0.0;
+    f[13] = static_cast<float>(dot(oc, e1) * is3);
+    f[14] = static_cast<float>(dot(oc, e2) * is3);
+    f[15] = static_cast<float>(dot(oc, e3) * is3);
     float *o = F + MLP_F * static_cast<size_t>(i);
 #pragma unroll
     for (int k = 0; k < MLP_F; k++) o[k] = mean ? (f[k] - mean[k]) / stdv[k] : f[k];
@@ -373,7 +378,7 @@ __global__ void k_features(const double *__restrict__ a, const float *__restrict
 // lane's eight k-values of a B fragment are one 16-byte read), keeps the
 // 16 x 128 activations of its particles in its own LDS tile between layers,
 // and finally applies
-//   a += G q + G s3 * tscale * (y0 e1 + y1 e2 + y2 e3)
+//   a += G (q + o) + G s3 * tscale * (y0 e1 + y1 e2 + y2 e3)
 // in the particle's local frame.  Rows of the LDS tiles are padded by 8
 // bf16 (16 B) so that the 16 rows a fragment read touches fall in
 // different banks.
@@ -514,8 +519,9 @@ __global__ __launch_bounds__(64 * MI_WAVES) void k_mlp_infer(const float *__rest
             mlp_local_frame(ai, qv, vv, e1, e2, e3);
             const double s = G * static_cast<double>(wi[7]) * static_cast<double>(tscale);
             const double y0 = y[lane * 4], y1 = y[lane * 4 + 1], y2 = y[lane * 4 + 2];
+            const double ov[3] = {wi[10], wi[11], wi[12]};
             for (int d = 0; d < 3; d++)
-                a[3 * i + d] = ai[d] + G * qv[d] + s * (y0 * e1[d] + y1 * e2[d] + y2 * e3[d]);
+                a[3 * i + d] = ai[d] + G * (qv[d] + ov[d]) + s * (y0 * e1[d] + y1 * e2[d] + y2 * e3[d]);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     }

@@ -40,6 +40,11 @@ struct DeviceTree {
     float *quadf = nullptr;         // device: 8 per node, the walk's FP32 copy
                                     // (xx yy zz xy xz yz, 1.5 tr, 0)
     bool has_quad = false;          // quad holds this tree's second moments
+    double *oct = nullptr;          // device: 12 per node (xxx yyy zzz xxy xxz xyy yyz
+                                    // xzz yzz xyz, 0, 0): third central moments
+    float *octf = nullptr;          // device: 12 per node, the walk's FP32 copy
+    int oct_cap = 0;                // nodes oct / octf hold room for
+    bool has_oct = false;           // oct holds this tree's third moments
     unsigned *tmp = nullptr;        // device: per-parent child counts
     unsigned *scan_tmp = nullptr;   // device: scan scratch
 };

@@ -372,10 +372,11 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
 
 // moments: mom[node] = (sum m, sum m x, sum m y, sum m z); with QUAD also
 // the second central moments (only the quadrupole and feature walks read them)
-template <bool QUAD>
+template <bool QUAD, bool OCT>
 __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
                             const double4 *__restrict__ P, double4 *__restrict__ mom,
-                            double *__restrict__ quad, float *__restrict__ quadf)
+                            double *__restrict__ quad, float *__restrict__ quadf,
+                            double *__restrict__ oct, float *__restrict__ octf)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= le - lb) return;
@@ -460,6 +461,60 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
     qf[0] = make_float4(fxx, fyy, fzz, static_cast<float>(q3));
     qf[1] = make_float4(static_cast<float>(q4), static_cast<float>(q5), 1.5f * (fxx + fyy + fzz),
                         0.0f);
+    if (!OCT) return;
+    // third central moments T_abc = sum m s_a s_b s_c (xxx yyy zzz xxy xxz
+    // xyy yyz xzz yzz xyz): leaves from their particles, internal nodes from
+    // their children shifted to this centre of mass,
+    //   T_abc += T^c_abc + Q^c_ab d_c + Q^c_ac d_b + Q^c_bc d_a + M_c d_a d_b d_c
+    double T[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) T[k] = 0.0;
+    if (nd.nchild == 0) {
+        for (int s = nd.first; s < nd.first + nd.count; s++) {
+            const double4 p = P[s];
+            const double d0 = p.x - c0, d1 = p.y - c1, d2 = p.z - c2;
+            const double mp = p.w;
+            T[0] += mp * d0 * d0 * d0;
+            T[1] += mp * d1 * d1 * d1;
+            T[2] += mp * d2 * d2 * d2;
+            T[3] += mp * d0 * d0 * d1;
+            T[4] += mp * d0 * d0 * d2;
+            T[5] += mp * d0 * d1 * d1;
+            T[6] += mp * d1 * d1 * d2;
+            T[7] += mp * d0 * d2 * d2;
+            T[8] += mp * d1 * d2 * d2;
+            T[9] += mp * d0 * d1 * d2;
+        }
+    } else {
+        for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
+            const NodeRec &ch = nodes[c];
+            const double d0 = ch.cx - c0, d1 = ch.cy - c1, d2 = ch.cz - c2;
+            const double Mc = mom[c].x;
+            const double *q = quad + 8 * static_cast<size_t>(c);
+            const double *tc = oct + 12 * static_cast<size_t>(c);
+            T[0] += tc[0] + 3.0 * q[0] * d0 + Mc * d0 * d0 * d0;
+            T[1] += tc[1] + 3.0 * q[1] * d1 + Mc * d1 * d1 * d1;
+            T[2] += tc[2] + 3.0 * q[2] * d2 + Mc * d2 * d2 * d2;
+            T[3] += tc[3] + (q[0] * d1 + 2.0 * q[3] * d0) + Mc * d0 * d0 * d1;
+            T[4] += tc[4] + (q[0] * d2 + 2.0 * q[4] * d0) + Mc * d0 * d0 * d2;
+            T[5] += tc[5] + (2.0 * q[3] * d1 + q[1] * d0) + Mc * d0 * d1 * d1;
+            T[6] += tc[6] + (q[1] * d2 + 2.0 * q[5] * d1) + Mc * d1 * d1 * d2;
+            T[7] += tc[7] + (2.0 * q[4] * d2 + q[2] * d0) + Mc * d0 * d2 * d2;
+            T[8] += tc[8] + (2.0 * q[5] * d2 + q[2] * d1) + Mc * d1 * d2 * d2;
+            T[9] += tc[9] + (q[3] * d2 + q[4] * d1 + q[5] * d0) + Mc * d0 * d1 * d2;
+        }
+    }
+    double *tn = oct + 12 * static_cast<size_t>(node);
+#pragma unroll
+    for (int k = 0; k < 10; k++) tn[k] = T[k];
+    tn[10] = 0.0;
+    tn[11] = 0.0;
+    float4 *tf = reinterpret_cast<float4 *>(octf + 12 * static_cast<size_t>(node));
+    tf[0] = make_float4(static_cast<float>(T[0]), static_cast<float>(T[1]), static_cast<float>(T[2]),
+                        static_cast<float>(T[3]));
+    tf[1] = make_float4(static_cast<float>(T[4]), static_cast<float>(T[5]), static_cast<float>(T[6]),
+                        static_cast<float>(T[7]));
+    tf[2] = make_float4(static_cast<float>(T[8]), static_cast<float>(T[9]), 0.0f, 0.0f);
 }
 
 // ------------------------------------------------------------------ build, all levels at once
@@ -685,12 +740,41 @@ __device__ __forceinline__ void quad_term(const float *__restrict__ q, double R0
     o2 = __builtin_fmaf(k3, QR2, sc * r2);
 }
 
+// Octupole term of an accepted node (R = x_i - com, r2 = |R|^2 + eps^2),
+// without the factor G, from its third central moments T (FP32, xxx yyy zzz
+// xxy xxz xyy yyz xzz yzz xyz):
+//   -(35/2) u R / r^9 + (15/2) ((t.R) R + w) / r^7 - (3/2) t / r^5,
+// w_d = T_abd R_a R_b, u = w.R, t_d = T_aad.
+__device__ __forceinline__ void oct_term(const float *__restrict__ T, double R0, double R1,
+                                         double R2, float ri, float &o0, float &o1, float &o2)
+{
+    const float r0 = static_cast<float>(R0), r1 = static_cast<float>(R1),
+                r2 = static_cast<float>(R2);
+    const float Txxx = T[0], Tyyy = T[1], Tzzz = T[2], Txxy = T[3], Txxz = T[4], Txyy = T[5],
+                Tyyz = T[6], Txzz = T[7], Tyzz = T[8], Txyz = T[9];
+    const float r00 = r0 * r0, r11 = r1 * r1, r22 = r2 * r2;
+    const float r01 = 2.0f * r0 * r1, r02 = 2.0f * r0 * r2, r12 = 2.0f * r1 * r2;
+    const float w0 = Txxx * r00 + Txyy * r11 + Txzz * r22 + Txxy * r01 + Txxz * r02 + Txyz * r12;
+    const float w1 = Txxy * r00 + Tyyy * r11 + Tyzz * r22 + Txyy * r01 + Txyz * r02 + Tyyz * r12;
+    const float w2 = Txxz * r00 + Tyyz * r11 + Tzzz * r22 + Txyz * r01 + Txzz * r02 + Tyzz * r12;
+    const float t0 = Txxx + Txyy + Txzz, t1 = Txxy + Tyyy + Tyzz, t2 = Txxz + Tyyz + Tzzz;
+    const float u = w0 * r0 + w1 * r1 + w2 * r2;
+    const float tR = t0 * r0 + t1 * r1 + t2 * r2;
+    const float ir2 = ri * ri;
+    const float ir5 = ir2 * ir2 * ri;
+    const float ir7 = ir5 * ir2;
+    const float su = (-17.5f * u * ir2 + 7.5f * tR) * ir7;
+    o0 = su * r0 + 7.5f * w0 * ir7 - 1.5f * t0 * ir5;
+    o1 = su * r1 + 7.5f * w1 * ir7 - 1.5f * t1 * ir5;
+    o2 = su * r2 + 7.5f * w2 * ir7 - 1.5f * t2 * ir5;
+}
+
 // QUAD: add the quadrupole term of accepted nodes.  FEAT: do not add it but
 // accumulate per-target features of the walk for the learned correction,
-// 12 FP32 per particle in original order, without the factor G:
+// 16 FP32 per particle in original order, without the factor G:
 // (quadrupole sum x3, sum (M l^3 / d^6) R x3, sum M l^2 / d^4,
-// sum M l^3 / d^5, accepted nodes, particle interactions, 0, 0) with l the
-// node's side, R = x_i - com and d = |R| (softened).
+// sum M l^3 / d^5, accepted nodes, particle interactions, octupole sum x3,
+// 0, 0, 0) with l the node's side, R = x_i - com and d = |R| (softened).
 template <bool QUAD, bool FEAT>
 __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ nodes,
                                                    const double4 *__restrict__ P,
@@ -699,6 +783,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                                                    WalkArgs args, double *__restrict__ a,
                                                    unsigned long long *__restrict__ n_int,
                                                    const float *__restrict__ quad,
+                                                   const float *__restrict__ oct,
                                                    float *__restrict__ feat)
 {
     __shared__ int stk_node[WK_WAVES][WK_STACK];
@@ -724,6 +809,7 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
     unsigned long long wcnt = 0;  // the wavefront's other interactions (uniform)
     float fq0 = 0.0f, fq1 = 0.0f, fq2 = 0.0f, fs2 = 0.0f, fs3 = 0.0f;
     float fv0 = 0.0f, fv1 = 0.0f, fv2 = 0.0f;
+    float fo0 = 0.0f, fo1 = 0.0f, fo2 = 0.0f;
     unsigned nacc = 0, npp = 0;
     int *snode = stk_node[wave];
     unsigned long long *smask = stk_mask[wave];
@@ -806,10 +892,15 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
                     const float mr2 = mr3 * static_cast<float>(__builtin_sqrt(d2 + args.eps2));
                     const float lr3 = lr * lr * lr;
                     const float k3 = mr3 * lr3;
+                    float t0, t1, t2;
+                    oct_term(oct + 12 * static_cast<size_t>(node), R0, R1, R2, ri, t0, t1, t2);
                     if (accept) {
                         fq0 += o0;
                         fq1 += o1;
                         fq2 += o2;
+                        fo0 += t0;
+                        fo1 += t1;
+                        fo2 += t2;
                         fs2 = __builtin_fmaf(mr2, lr * lr, fs2);
                         fs3 = __builtin_fmaf(mr2, lr3, fs3);
                         fv0 = __builtin_fmaf(k3, static_cast<float>(R0), fv0);
@@ -851,10 +942,11 @@ __global__ __launch_bounds__(WK_BLOCK) void k_walk(const NodeRec *__restrict__ n
         a[3 * i + 1] = args.G * a1;
         a[3 * i + 2] = args.G * a2;
         if (FEAT) {
-            float4 *f = reinterpret_cast<float4 *>(feat + 12 * static_cast<size_t>(i));
+            float4 *f = reinterpret_cast<float4 *>(feat + 16 * static_cast<size_t>(i));
             f[0] = make_float4(fq0, fq1, fq2, fv0);
             f[1] = make_float4(fv1, fv2, fs2, fs3);
-            f[2] = make_float4(static_cast<float>(nacc), static_cast<float>(npp), 0.0f, 0.0f);
+            f[2] = make_float4(static_cast<float>(nacc), static_cast<float>(npp), fo0, fo1);
+            f[3] = make_float4(fo2, 0.0f, 0.0f, 0.0f);
         }
     }
     // interaction count: one atomic per wave
@@ -1369,18 +1461,34 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
     return make_success_error_status();
 }
 
-// Moments bottom-up, one launch per level (deepest first).
-static ErrorStatus tree_moments(Context *c, DeviceTree *t, bool with_quad)
+// Moments bottom-up, one launch per level (deepest first).  Third moments
+// (with_oct, which implies with_quad) in arrays allocated on first use.
+static ErrorStatus tree_moments(Context *c, DeviceTree *t, bool with_quad, bool with_oct)
 {
+    if (with_oct && t->oct_cap < t->node_cap) {
+        GS_HIP(hipStreamSynchronize(c->stream));
+        if (t->oct) GS_HIP(hipFree(t->oct));
+        if (t->octf) GS_HIP(hipFree(t->octf));
+        t->oct = nullptr;
+        t->octf = nullptr;
+        t->oct_cap = 0;
+        GS_HIP(hipMalloc(reinterpret_cast<void **>(&t->oct), sizeof(double) * 12 * t->node_cap));
+        GS_HIP(hipMalloc(reinterpret_cast<void **>(&t->octf), sizeof(float) * 12 * t->node_cap));
+        t->oct_cap = t->node_cap;
+    }
     for (int d = t->num_levels - 1; d >= 0; d--) {
         const int lb = t->level_begin[d], le = t->level_end[d];
         if (le <= lb) continue;
-        if (with_quad)
-            hipLaunchKernelGGL(k_multipole<true>, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad, t->quadf);
+        const dim3 grid((le - lb + 255) / 256);
+        if (with_oct)
+            hipLaunchKernelGGL((k_multipole<true, true>), grid, dim3(256), 0, c->stream, t->nodes, lb,
+                               le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf);
+        else if (with_quad)
+            hipLaunchKernelGGL((k_multipole<true, false>), grid, dim3(256), 0, c->stream, t->nodes,
+                               lb, le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf);
         else
-            hipLaunchKernelGGL(k_multipole<false>, dim3((le - lb + 255) / 256), dim3(256), 0,
-                               c->stream, t->nodes, lb, le, t->P, t->mom, t->quad, t->quadf);
+            hipLaunchKernelGGL((k_multipole<false, false>), grid, dim3(256), 0, c->stream, t->nodes,
+                               lb, le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf);
     }
     GS_HIP(hipGetLastError());
     return make_success_error_status();
@@ -1389,7 +1497,8 @@ static ErrorStatus tree_moments(Context *c, DeviceTree *t, bool with_quad)
 // All levels at once (leaf_cap <= FB_MAX_LEAF): digits and leaf depths per
 // particle, node counts per (depth, tile), one scan and the emission; one
 // host round trip (the level sizes), then the moments level by level.
-static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_cap, bool with_quad)
+static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_cap, bool with_quad,
+                                   bool with_oct)
 {
     const int ntiles = (n + FB_TILE - 1) / FB_TILE;
     const long long hn = static_cast<long long>(FB_DEPTHS) * ntiles;
@@ -1439,15 +1548,17 @@ static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_ca
     }
     t->num_levels = levels;
     t->num_nodes = nn;
-    return tree_moments(c, t, with_quad);
+    return tree_moments(c, t, with_quad, with_oct);
 }
 
 ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int leaf_cap,
                        const double *box_center, double box_width, bool with_quad,
-                       DeviceTree **out)
+                       DeviceTree **out, bool with_oct)
 {
+    with_quad = with_quad || with_oct;
     DeviceTree *t = &g_tree;
     t->has_quad = false;
+    t->has_oct = false;
     // an unfinalised parameter (-1) means the reference's default of one
     // particle per leaf (src/acceleration.c:73-129)
     if (leaf_cap < 1) leaf_cap = 1;
@@ -1464,8 +1575,9 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
 
     const int rec = timing_begin(c, "tree_build");
     if (leaf_cap <= FB_MAX_LEAF) {
-        GS_TRY(tree_build_flat(c, t, n, leaf_cap, with_quad));
+        GS_TRY(tree_build_flat(c, t, n, leaf_cap, with_quad, with_oct));
         t->has_quad = with_quad;
+        t->has_oct = with_oct;
         timing_end(c, rec);
         *out = t;
         return make_success_error_status();
@@ -1528,9 +1640,10 @@ ErrorStatus tree_build(Context *c, const double *x, const double *m, int n, int 
     }
     t->num_levels = levels;
     t->num_nodes = t->level_end[levels - 1];
-    GS_TRY(tree_moments(c, t, with_quad));
+    GS_TRY(tree_moments(c, t, with_quad, with_oct));
     GS_HIP(hipGetLastError());
     t->has_quad = with_quad;
+    t->has_oct = with_oct;
     timing_end(c, rec);
     *out = t;
     return make_success_error_status();
@@ -1543,6 +1656,8 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
     if ((order == 2 || feat) && !t->has_quad)
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Quadrupole or feature walk of a tree built without "
                                               "second moments");
+    if (feat && !t->has_oct)
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Feature walk of a tree built without third moments");
     WalkArgs args;
     args.theta2 = theta * theta;
     args.eps2 = eps * eps;
@@ -1556,13 +1671,16 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
     const int rec = timing_begin(c, "bh_walk");
     if (feat)
         hipLaunchKernelGGL((k_walk<false, true>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
-                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, feat);
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, t->octf,
+                           feat);
     else if (order >= 2)
         hipLaunchKernelGGL((k_walk<true, false>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
-                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, feat);
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, t->octf,
+                           feat);
     else
         hipLaunchKernelGGL((k_walk<false, false>), dim3(grid), dim3(WK_BLOCK), 0, c->stream,
-                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, feat);
+                           t->nodes, t->P, t->skeys, t->sidx, t->n, args, a, cnt, t->quadf, t->octf,
+                           feat);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
     if (interactions) {

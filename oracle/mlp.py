@@ -154,12 +154,13 @@ def local_frame(a, q, v):
 
 def features16(a, wf, G):
     """The 16 raw network inputs from coarse accelerations a (N, 3) and the
-    walk features wf (N, 12) (oracle.bh_walk(..., features=True)), all
-    rotation invariant, in the frame of local_frame(a, q, v):
+    walk features wf (N, 16) (oracle.bh_walk(..., features=True, oct=...)),
+    all rotation invariant, in the frame of local_frame(a, q, v):
     0 G q.e1/|a|, 1 G q.e2/|a|, 2-4 G v.e1, v.e2, v.e3 /|a|, 5 G s2/|a|,
     6 G s3/|a|, 7 log10|a|, 8 log(1 + accepted nodes), 9 log(1 + particle
     interactions), 10 log10 max(G s3/|a|, 1e-12), 11 the same of s2,
-    12 the same of |q|, 13 |v|/s3 (0 if s3 = 0), 14-15 zero."""
+    12 the same of |q|, 13-15 the octupole sum o.e1, o.e2, o.e3 over s3 (0 if
+    s3 = 0)."""
     a = np.asarray(a, dtype=np.float64)
     wf = np.asarray(wf, dtype=np.float64)
     q, v = wf[:, 0:3], wf[:, 3:6]
@@ -180,16 +181,20 @@ def features16(a, wf, G):
     f[:, 10] = np.log10(np.maximum(s3, 1e-12))
     f[:, 11] = np.log10(np.maximum(s2, 1e-12))
     f[:, 12] = np.log10(np.maximum(g * np.linalg.norm(q, axis=1), 1e-12))
-    vn = np.linalg.norm(v, axis=1)
-    f[:, 13] = np.where(wf[:, 7] > 0, vn / np.where(wf[:, 7] > 0, wf[:, 7], 1.0), 0.0)
+    o = wf[:, 10:13]
+    is3 = np.where(wf[:, 7] > 0, 1.0 / np.where(wf[:, 7] > 0, wf[:, 7], 1.0), 0.0)
+    f[:, 13] = np.sum(o * e1, axis=1) * is3
+    f[:, 14] = np.sum(o * e2, axis=1) * is3
+    f[:, 15] = np.sum(o * e3, axis=1) * is3
     return f
 
 
 def apply_correction(a, wf, Y, G, tscale):
-    """a + G q + G s3 tscale (Y0 e1 + Y1 e2 + Y2 e3)."""
+    """a + G (q + o) + G s3 tscale (Y0 e1 + Y1 e2 + Y2 e3): the quadrupole and
+    octupole sums exactly, the network's output in the local frame."""
     a = np.asarray(a, dtype=np.float64)
     wf = np.asarray(wf, dtype=np.float64)
     e1, e2, e3 = local_frame(a, wf[:, 0:3], wf[:, 3:6])
     Y = np.asarray(Y, dtype=np.float64)
     s = (G * wf[:, 7] * tscale)[:, None]
-    return a + G * wf[:, 0:3] + s * (Y[:, 0:1] * e1 + Y[:, 1:2] * e2 + Y[:, 2:3] * e3)
+    return a + G * (wf[:, 0:3] + wf[:, 10:13]) + s * (Y[:, 0:1] * e1 + Y[:, 1:2] * e2 + Y[:, 2:3] * e3)

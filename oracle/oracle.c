@@ -262,7 +262,8 @@ typedef struct
     const int64_t *prefix;
     const double *mass, *comx, *comy, *comz;
     const double *quad; /* 6 per node (xx yy zz xy xz yz) or NULL: monopole only */
-    int features;       /* nonzero: quad feeds the features, not the acceleration */
+    const double *oct;  /* 10 per node (xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz) or NULL */
+    int features;       /* nonzero: quad and oct feed the features, not the acceleration */
 } Walk;
 
 /* Walk features of one target (the learned correction's inputs, without G):
@@ -273,7 +274,7 @@ typedef struct
  * single particle always counts as a particle interaction. */
 typedef struct
 {
-    double q[3], v[3], s2, s3, nacc, npp;
+    double q[3], v[3], s2, s3, nacc, npp, o[3];
 } WalkFeat;
 
 /* Quadrupole term of an accepted node at R = x_i - com (r^2 = |R|^2 + eps^2):
@@ -295,6 +296,35 @@ static void quad_accel(double acc[3], const double *q, double R0, double R1, dou
     acc[0] += G * (3.0 * QR0 * ir5 + s * R0);
     acc[1] += G * (3.0 * QR1 * ir5 + s * R1);
     acc[2] += G * (3.0 * QR2 * ir5 + s * R2);
+}
+
+/* Octupole term of an accepted node at R = x_i - com (r^2 = |R|^2 + eps^2),
+ * the next order of the expansion of -G sum m (R - s) / |R - s|^3 in the
+ * offsets s from the centre of mass:
+ *   a_d += G [ -(35/2) u R_d / r^9 + (15/2) ((t.R) R_d + w_d) / r^7 - (3/2) t_d / r^5 ]
+ * with T_abc = sum m s_a s_b s_c, w_d = T_abd R_a R_b, u = w.R, t_d = T_aad
+ * (-(1/6) T_abc d_a d_b d_c d_d (1/r) written out). */
+static void oct_accel(double acc[3], const double *T, double R0, double R1, double R2, double r2,
+                      double G)
+{
+    const double Txxx = T[0], Tyyy = T[1], Tzzz = T[2], Txxy = T[3], Txxz = T[4], Txyy = T[5],
+                 Tyyz = T[6], Txzz = T[7], Tyzz = T[8], Txyz = T[9];
+    const double w0 = Txxx * R0 * R0 + Txyy * R1 * R1 + Txzz * R2 * R2 +
+                      2.0 * (Txxy * R0 * R1 + Txxz * R0 * R2 + Txyz * R1 * R2);
+    const double w1 = Txxy * R0 * R0 + Tyyy * R1 * R1 + Tyzz * R2 * R2 +
+                      2.0 * (Txyy * R0 * R1 + Txyz * R0 * R2 + Tyyz * R1 * R2);
+    const double w2 = Txxz * R0 * R0 + Tyyz * R1 * R1 + Tzzz * R2 * R2 +
+                      2.0 * (Txyz * R0 * R1 + Txzz * R0 * R2 + Tyzz * R1 * R2);
+    const double t0 = Txxx + Txyy + Txzz, t1 = Txxy + Tyyy + Tyzz, t2 = Txxz + Tyyz + Tzzz;
+    const double u = w0 * R0 + w1 * R1 + w2 * R2;
+    const double tR = t0 * R0 + t1 * R1 + t2 * R2;
+    const double ir2 = 1.0 / r2;
+    const double ir5 = ir2 * ir2 / sqrt(r2);
+    const double ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+    const double su = -17.5 * u * ir9 + 7.5 * tR * ir7;
+    acc[0] += G * (su * R0 + 7.5 * w0 * ir7 - 1.5 * t0 * ir5);
+    acc[1] += G * (su * R1 + 7.5 * w1 * ir7 - 1.5 * t1 * ir5);
+    acc[2] += G * (su * R2 + 7.5 * w2 * ir7 - 1.5 * t2 * ir5);
 }
 
 static void walk_node(const Walk *w, int node, int i, int64_t key_i, const double xi[3],
@@ -320,6 +350,8 @@ static void walk_node(const Walk *w, int node, int i, int64_t key_i, const doubl
             acc[2] -= t * R2;
             if (w->quad && !w->features)
                 quad_accel(acc, &w->quad[6 * node], R0, R1, R2, d2 + w->eps2, w->G);
+            if (w->oct && !w->features)
+                oct_accel(acc, &w->oct[10 * node], R0, R1, R2, d2 + w->eps2, w->G);
             if (f && single)
                 f->npp += 1.0;
             else if (f)
@@ -327,6 +359,8 @@ static void walk_node(const Walk *w, int node, int i, int64_t key_i, const doubl
                 const double r2 = d2 + w->eps2;
                 const double M = w->mass[node];
                 quad_accel(f->q, &w->quad[6 * node], R0, R1, R2, r2, 1.0);
+                if (w->oct)
+                    oct_accel(f->o, &w->oct[10 * node], R0, R1, R2, r2, 1.0);
                 f->s2 += M * l * l / (r2 * r2);
                 f->s3 += M * l * l * l / (r2 * r2 * r);
                 const double k3 = M * l * l * l / (r2 * r2 * r2);
@@ -370,11 +404,11 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
                  const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
                  const int *count, const int *first, const int *nchild, const int *child0,
                  const double *mass, const double *comx, const double *comy, const double *comz,
-                 const double *quad, double *feat)
+                 const double *quad, const double *oct, double *feat)
 {
     Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
               depth, count, first,  nchild,    child0,        prefix, mass,        comx,
-              comy,  comz,  quad,   feat != NULL};
+              comy,  comz,  quad,   oct,       feat != NULL};
 #pragma omp parallel for schedule(dynamic, 64)
     for (int s = 0; s < n; s++)
     {
@@ -382,17 +416,18 @@ void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double
         const double xi[3] = {x[i * 3 + 0], x[i * 3 + 1], x[i * 3 + 2]};
         double acc[3] = {0.0, 0.0, 0.0};
         int64_t cnt = 0;
-        WalkFeat f = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, 0.0};
+        WalkFeat f = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}};
         /* the root is always internal: start from its children */
         for (int c = child0[0]; c < child0[0] + nchild[0]; c++)
             walk_node(&w, c, i, sorted_keys[s], xi, acc, &cnt, feat ? &f : NULL);
         if (feat)
         {
-            double *o = &feat[(size_t)i * 12];
+            double *o = &feat[(size_t)i * 16];
             o[0] = f.q[0], o[1] = f.q[1], o[2] = f.q[2];
             o[3] = f.v[0], o[4] = f.v[1], o[5] = f.v[2];
             o[6] = f.s2, o[7] = f.s3, o[8] = f.nacc, o[9] = f.npp;
-            o[10] = 0.0, o[11] = 0.0;
+            o[10] = f.o[0], o[11] = f.o[1], o[12] = f.o[2];
+            o[13] = 0.0, o[14] = 0.0, o[15] = 0.0;
         }
         a[i * 3 + 0] = acc[0];
         a[i * 3 + 1] = acc[1];
@@ -516,7 +551,7 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
 {
     Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
               depth, count, first,  nchild,    child0,        prefix, mass,        comx,
-              comy,  comz,  quad,   0};
+              comy,  comz,  quad,   NULL,      0};
     /* groups from the tree: children of at most `group` particles of each node
      * with more are packed in order into runs of at most `group` particles; a
      * root within the size is one group; larger leaves give runs of `group`.
@@ -652,6 +687,66 @@ void orc_octree_quad(int num, const double *x, const double *m, const int32_t *s
         }
         for (int k = 0; k < 6; k++)
             quad[6 * node + k] = q[k];
+    }
+}
+
+/* Third central moments T_abc = sum m s_a s_b s_c (s = x - com), 10 per node
+ * (xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz): leaves over their particles,
+ * internal nodes from their children shifted to this centre of mass,
+ *   T_abc += T^c_abc + Q^c_ab d_c + Q^c_ac d_b + Q^c_bc d_a + M_c d_a d_b d_c
+ * (d = com_child - com; the children's first moments about their own centres
+ * vanish).  quad: the second moments of orc_octree_quad. */
+void orc_octree_oct(int num, const double *x, const double *m, const int32_t *sorted_idx,
+                    const int *count, const int *first, const int *nchild, const int *child0,
+                    const double *mass, const double *comx, const double *comy,
+                    const double *comz, const double *quad, double *oct)
+{
+    for (int node = num - 1; node >= 0; node--)
+    {
+        double T[10] = {0.0};
+        const double c[3] = {comx[node], comy[node], comz[node]};
+        if (nchild[node] == 0)
+        {
+            for (int t = first[node]; t < first[node] + count[node]; t++)
+            {
+                const int p = sorted_idx[t];
+                const double d0 = x[p * 3 + 0] - c[0], d1 = x[p * 3 + 1] - c[1],
+                             d2 = x[p * 3 + 2] - c[2];
+                const double mp = m[p];
+                T[0] += mp * d0 * d0 * d0;
+                T[1] += mp * d1 * d1 * d1;
+                T[2] += mp * d2 * d2 * d2;
+                T[3] += mp * d0 * d0 * d1;
+                T[4] += mp * d0 * d0 * d2;
+                T[5] += mp * d0 * d1 * d1;
+                T[6] += mp * d1 * d1 * d2;
+                T[7] += mp * d0 * d2 * d2;
+                T[8] += mp * d1 * d2 * d2;
+                T[9] += mp * d0 * d1 * d2;
+            }
+        }
+        else
+        {
+            for (int ch = child0[node]; ch < child0[node] + nchild[node]; ch++)
+            {
+                const double d0 = comx[ch] - c[0], d1 = comy[ch] - c[1], d2 = comz[ch] - c[2];
+                const double M = mass[ch];
+                const double *q = &quad[6 * ch]; /* xx yy zz xy xz yz */
+                const double *tc = &oct[10 * ch];
+                T[0] += tc[0] + 3.0 * q[0] * d0 + M * d0 * d0 * d0;
+                T[1] += tc[1] + 3.0 * q[1] * d1 + M * d1 * d1 * d1;
+                T[2] += tc[2] + 3.0 * q[2] * d2 + M * d2 * d2 * d2;
+                T[3] += tc[3] + (q[0] * d1 + 2.0 * q[3] * d0) + M * d0 * d0 * d1;
+                T[4] += tc[4] + (q[0] * d2 + 2.0 * q[4] * d0) + M * d0 * d0 * d2;
+                T[5] += tc[5] + (2.0 * q[3] * d1 + q[1] * d0) + M * d0 * d1 * d1;
+                T[6] += tc[6] + (q[1] * d2 + 2.0 * q[5] * d1) + M * d1 * d1 * d2;
+                T[7] += tc[7] + (2.0 * q[4] * d2 + q[2] * d0) + M * d0 * d2 * d2;
+                T[8] += tc[8] + (2.0 * q[5] * d2 + q[2] * d1) + M * d1 * d2 * d2;
+                T[9] += tc[9] + (q[3] * d2 + q[4] * d1 + q[5] * d0) + M * d0 * d1 * d2;
+            }
+        }
+        for (int k = 0; k < 10; k++)
+            oct[10 * node + k] = T[k];
     }
 }
 

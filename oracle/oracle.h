@@ -70,18 +70,20 @@ int orc_octree(int max_nodes, int n, const double *x, const double *m,
  *   otherwise -> descend.
  * Writes a[idx*3..] for every particle and, if n_int != NULL, the number of
  * accepted nodes + leaf-particle interactions per particle.
- * With feat != NULL (and quad given) the walk stays monopole-only and writes
- * 12 features per particle (original order) for the learned correction:
- * quadrupole-acceleration sum (3), sum (M l^3 / r^6) R (3), sum M l^2 / r^4,
- * sum M l^3 / r^5, accepted nodes, particle interactions, 0, 0 -- without G;
- * l is the node side, R = x_i - com, r^2 = |R|^2 + eps^2; a leaf holding one
- * particle counts as a particle interaction. */
+ * With quad / oct != NULL the quadrupole / octupole terms of accepted nodes
+ * are added.  With feat != NULL (and quad given) the walk stays
+ * monopole-only and writes 16 features per particle (original order) for the
+ * learned correction: quadrupole-acceleration sum (3), sum (M l^3 / r^6) R
+ * (3), sum M l^2 / r^4, sum M l^3 / r^5, accepted nodes, particle
+ * interactions, octupole-acceleration sum (3, zero without oct), 0, 0, 0 --
+ * without G; l is the node side, R = x_i - com, r^2 = |R|^2 + eps^2; a leaf
+ * holding one particle counts as a particle interaction. */
 void orc_bh_walk(double *a, int64_t *n_int, int n, const double *x, const double *m, double G,
                  double eps, double theta, double width, const int64_t *sorted_keys,
                  const int32_t *sorted_idx, const int *depth, const int64_t *prefix,
                  const int *count, const int *first, const int *nchild, const int *child0,
                  const double *mass, const double *comx, const double *comy, const double *comz,
-                 const double *quad, double *feat);
+                 const double *quad, const double *oct, double *feat);
 
 /* The walk of groups of at most `group` consecutive sorted particles (the
  * children of at most `group` particles of each node with more, packed in
@@ -107,6 +109,15 @@ void orc_octree_quad(int num, const double *x, const double *m, const int32_t *s
                      const int *count, const int *first, const int *nchild, const int *child0,
                      const double *mass, const double *comx, const double *comy,
                      const double *comz, double *quad);
+
+/* Third central moments per node, T_abc = sum m s_a s_b s_c (s = x - com),
+ * in the order xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz: leaves over their
+ * particles, internal nodes from their children (parallel-axis shift with
+ * the children's second moments, quad from orc_octree_quad). */
+void orc_octree_oct(int num, const double *x, const double *m, const int32_t *sorted_idx,
+                    const int *count, const int *first, const int *nchild, const int *child0,
+                    const double *mass, const double *comx, const double *comy,
+                    const double *comz, const double *quad, double *oct);
 
 /* Leapfrog KDK with compensated summation (src/integrator.c:894-1121), using
  * orc_pairwise (method 1) or orc_massless (method 2) for the accelerations.

@@ -68,10 +68,11 @@ def lib():
         L.orc_octree.argtypes = [i, i, PD, PD, PI64, PI, i] + [PI, PI64, PI, PI, PI, PI] + [PD] * 7
         L.orc_octree.restype = i
         L.orc_bh_walk.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI, PI,
-                                  PD, PD, PD, PD, PD, PD]
+                                  PD, PD, PD, PD, PD, PD, PD]
         L.orc_bh_walk_group.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI, PI,
                                         PD, PD, PD, PD, PD, i]
         L.orc_octree_quad.argtypes = [i, PD, PD, PI, PI, PI, PI, PI, PD, PD, PD, PD, PD]
+        L.orc_octree_oct.argtypes = [i, PD, PD, PI, PI, PI, PI, PI, PD, PD, PD, PD, PD, PD]
         L.orc_leapfrog.argtypes = [PD, PD, i, PD, d, i, d, d, d]
         L.orc_leapfrog.restype = i64
         L.orc_energy.argtypes = [i, PD, PD, PD, d]
@@ -212,11 +213,27 @@ def octree_quad(x, m, tree):
     return q
 
 
-def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, features=False):
-    """Barnes-Hut walk; quad=(num_nodes, 6) array adds the quadrupole terms.
+def octree_oct(x, m, tree, quad):
+    """Third central moments (num_nodes, 10) of a tree from build_tree, given
+    its second moments from octree_quad."""
+    x, m = _c(x), _c(m)
+    t = tree
+    num = len(t["depth"])
+    o = np.zeros((num, 10))
+    lib().orc_octree_oct(num, _abi.dptr(x), _abi.dptr(m), _abi.iptr(_c(t["sorted_idx"], np.int32)),
+                         _abi.iptr(t["count"]), _abi.iptr(t["first"]), _abi.iptr(t["nchild"]),
+                         _abi.iptr(t["child0"]), _abi.dptr(t["mass"]), _abi.dptr(t["comx"]),
+                         _abi.dptr(t["comy"]), _abi.dptr(t["comz"]), _abi.dptr(_c(quad)), _abi.dptr(o))
+    return o
+
+
+def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, features=False, oct=None):
+    """Barnes-Hut walk; quad=(num_nodes, 6) / oct=(num_nodes, 10) arrays add
+    the quadrupole / octupole terms.
 
     features=True (quad required): monopole walk that also returns the
-    (N, 12) walk features of the learned correction as a last element."""
+    (N, 16) walk features of the learned correction as a last element (the
+    octupole sums only when oct is given)."""
     x, m = _c(x), _c(m)
     n = len(m)
     a = np.zeros_like(x)
@@ -225,13 +242,14 @@ def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, fea
     if features and quad is None:
         raise ValueError("features need the quadrupole moments")
     qp = _abi.dptr(_c(quad)) if quad is not None else None
-    feat = np.zeros((n, 12)) if features else None
+    op = _abi.dptr(_c(oct)) if oct is not None else None
+    feat = np.zeros((n, 16)) if features else None
     lib().orc_bh_walk(_abi.dptr(a), cnt.ctypes.data_as(PI64) if want_counts else None, n, _abi.dptr(x),
                       _abi.dptr(m), G, eps, theta, t["width"], _c(t["sorted_keys"], np.int64).ctypes.data_as(PI64),
                       _abi.iptr(_c(t["sorted_idx"], np.int32)), _abi.iptr(t["depth"]),
                       t["prefix"].ctypes.data_as(PI64), _abi.iptr(t["count"]), _abi.iptr(t["first"]),
                       _abi.iptr(t["nchild"]), _abi.iptr(t["child0"]), _abi.dptr(t["mass"]),
-                      _abi.dptr(t["comx"]), _abi.dptr(t["comy"]), _abi.dptr(t["comz"]), qp,
+                      _abi.dptr(t["comx"]), _abi.dptr(t["comy"]), _abi.dptr(t["comz"]), qp, op,
                       _abi.dptr(feat) if features else None)
     out = (a, cnt) if want_counts else (a,)
     if features:

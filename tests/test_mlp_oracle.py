@@ -87,20 +87,27 @@ def test_walk_feature_identities():
     x, v, m = ic.two_spheres(3000, 31)
     t = O.build_tree(x, m, 1)
     q = O.octree_quad(x, m, t)
+    o = O.octree_oct(x, m, t, q)
     for theta, eps in ((0.8, 0.0), (0.5, 1e-2)):
         G = 2.5
         a_m, cnt = O.bh_walk(x, m, t, theta, G=G, eps=eps, want_counts=True)
-        a_f, cnt_f, wf = O.bh_walk(x, m, t, theta, G=G, eps=eps, want_counts=True, quad=q, features=True)
+        a_f, cnt_f, wf = O.bh_walk(x, m, t, theta, G=G, eps=eps, want_counts=True, quad=q, features=True, oct=o)
         a_q = O.bh_walk(x, m, t, theta, G=G, eps=eps, quad=q)
+        a_o = O.bh_walk(x, m, t, theta, G=G, eps=eps, quad=q, oct=o)
         # features mode keeps the monopole acceleration unchanged
         assert np.array_equal(a_f, a_m) and np.array_equal(cnt_f, cnt)
         # quadrupole sum = quadrupole walk - monopole walk (divided by G)
         assert np.allclose(G * wf[:, 0:3], a_q - a_m, rtol=1e-9, atol=1e-12 * np.abs(a_m).max())
+        # octupole sum = octupole walk - quadrupole walk (divided by G)
+        assert np.allclose(G * wf[:, 10:13], a_o - a_q, rtol=1e-7, atol=1e-12 * np.abs(a_m).max())
         # every interaction is either an accepted node or a particle pair
         assert np.array_equal(wf[:, 8] + wf[:, 9], cnt.astype(float))
         # |v| <= s3 (per node |term| = M l^3 |R| / r^6 <= M l^3 / r^5)
         assert np.all(np.linalg.norm(wf[:, 3:6], axis=1) <= wf[:, 7] * (1 + 1e-12))
-        assert np.all(wf[:, 10:] == 0.0)
+        assert np.all(wf[:, 13:] == 0.0)
+        # without third moments the octupole sums stay zero
+        _, wq = O.bh_walk(x, m, t, theta, G=G, eps=eps, quad=q, features=True)
+        assert np.all(wq[:, 10:] == 0.0) and np.array_equal(wq[:, :10], wf[:, :10])
 
 
 def test_walk_features_theta_zero_all_pairs():
@@ -122,8 +129,9 @@ def test_walk_features_single_far_node_closed_form():
     m = np.concatenate([rng.uniform(0.5, 1.5, 100), [1.0]])
     t = O.build_tree(x, m, 1)
     q = O.octree_quad(x, m, t)
+    o = O.octree_oct(x, m, t, q)
     eps = 0.3
-    a, cnt, wf = O.bh_walk(x, m, t, 0.5, eps=eps, want_counts=True, quad=q, features=True)
+    a, cnt, wf = O.bh_walk(x, m, t, 0.5, eps=eps, want_counts=True, quad=q, features=True, oct=o)
     f = wf[100]
     assert f[8] == 1.0 and f[9] == 0.0 and cnt[100] == 1
     M = m[:100].sum()
@@ -142,6 +150,13 @@ def test_walk_features_single_far_node_closed_form():
     Q = (m[:100, None, None] * d[:, :, None] * d[:, None, :]).sum(axis=0)
     qa = 3 * Q @ R / r ** 5 - 7.5 * (R @ Q @ R) * R / r ** 7 + 1.5 * np.trace(Q) * R / r ** 5
     assert np.allclose(f[0:3], qa, rtol=1e-9, atol=1e-12 * np.abs(qa).max())
+    # the octupole term: -(1/6) T_abc d_a d_b d_c d_d (1/r) of the cluster's
+    # third moments, written with numpy's einsum
+    T = np.einsum("i,ia,ib,ic->abc", m[:100], d, d, d)
+    w = np.einsum("abd,a,b->d", T, R, R)
+    tv = np.einsum("aad->d", T)
+    oa = -17.5 * (w @ R) * R / r ** 9 + 7.5 * ((tv @ R) * R + w) / r ** 7 - 1.5 * tv / r ** 5
+    assert np.allclose(f[10:13], oa, rtol=1e-9, atol=1e-12 * np.abs(oa).max())
 
 
 def _rotation(seed):
@@ -171,15 +186,17 @@ def test_local_frame_orthonormal_right_handed():
 def test_features16_rotation_invariant_and_definitions():
     rng = np.random.default_rng(9)
     a = rng.normal(size=(100, 3))
-    wf = np.zeros((100, 12))
+    wf = np.zeros((100, 16))
     wf[:, 0:6] = rng.normal(size=(100, 6)) * 1e-2
     wf[:, 6:10] = np.abs(rng.normal(size=(100, 4)))
     wf[:, 7] = np.linalg.norm(wf[:, 3:6], axis=1) * 1.5  # |v| <= s3
+    wf[:, 10:13] = rng.normal(size=(100, 3)) * 1e-3
     f = NN.features16(a, wf, 2.0)
     R = _rotation(10)
     wr = wf.copy()
     wr[:, 0:3] = wf[:, 0:3] @ R.T
     wr[:, 3:6] = wf[:, 3:6] @ R.T
+    wr[:, 10:13] = wf[:, 10:13] @ R.T
     fr = NN.features16(a @ R.T, wr, 2.0)
     assert np.allclose(fr, f, rtol=1e-12, atol=1e-13)
     an = np.linalg.norm(a, axis=1)
@@ -187,22 +204,25 @@ def test_features16_rotation_invariant_and_definitions():
     assert np.allclose(np.hypot(f[:, 0], f[:, 1]) * an, 2.0 * np.linalg.norm(wf[:, 0:3], axis=1))
     assert np.allclose(np.linalg.norm(f[:, 2:5], axis=1) * an, 2.0 * np.linalg.norm(wf[:, 3:6], axis=1))
     assert np.allclose(f[:, 0] * an ** 2, 2.0 * np.sum(wf[:, 0:3] * a, axis=1))
-    assert np.allclose(f[:, 7], np.log10(an)) and np.allclose(f[:, 13], 1 / 1.5)
+    assert np.allclose(f[:, 7], np.log10(an))
+    # the octupole components reassemble |o| / s3
+    assert np.allclose(np.linalg.norm(f[:, 13:16], axis=1), np.linalg.norm(wf[:, 10:13], axis=1) / wf[:, 7])
     assert np.allclose(10 ** f[:, 10], 2.0 * wf[:, 7] / an)
 
 
 def test_apply_correction_rotates_back():
     rng = np.random.default_rng(11)
     a = rng.normal(size=(50, 3))
-    wf = np.abs(rng.normal(size=(50, 12)))
+    wf = np.abs(rng.normal(size=(50, 16)))
     Y = rng.normal(size=(50, 3))
     out = NN.apply_correction(a, wf, Y, 1.5, 0.25)
     e1, e2, e3 = NN.local_frame(a, wf[:, 0:3], wf[:, 3:6])
-    d = (out - a - 1.5 * wf[:, 0:3]) / (1.5 * 0.25 * wf[:, 7])[:, None]
+    d = (out - a - 1.5 * (wf[:, 0:3] + wf[:, 10:13])) / (1.5 * 0.25 * wf[:, 7])[:, None]
     assert np.allclose(np.stack([np.sum(d * e, axis=1) for e in (e1, e2, e3)], axis=1), Y)
     # equivariance: rotating the inputs rotates the correction
     R = _rotation(12)
     wr = wf.copy()
     wr[:, 0:3] = wf[:, 0:3] @ R.T
     wr[:, 3:6] = wf[:, 3:6] @ R.T
+    wr[:, 10:13] = wf[:, 10:13] @ R.T
     assert np.allclose(NN.apply_correction(a @ R.T, wr, Y, 1.5, 0.25), out @ R.T)

@@ -429,6 +429,65 @@ def test_quadrupole_walk_beats_monopole():
     assert np.median(e2) < 0.5 * np.median(e1)
 
 
+OCT_AXES = ((0, 0, 0), (1, 1, 1), (2, 2, 2), (0, 0, 1), (0, 0, 2), (0, 1, 1), (1, 1, 2), (0, 2, 2), (1, 2, 2),
+            (0, 1, 2))
+
+
+def test_octupole_moments_long_double():
+    """Every node's third central moments (the parallel-axis recursion over
+    children) against the direct long-double sum over the node's particles."""
+    x, v, m = ic.plummer(800, 15)
+    m = np.random.default_rng(3).uniform(0.5, 1.5, 800)
+    t = O.build_tree(x, m, 4)
+    q = O.octree_quad(x, m, t)
+    o = O.octree_oct(x, m, t, q)
+    xl, ml = x.astype(np.longdouble), m.astype(np.longdouble)
+    for nd in range(0, len(t["depth"]), 5):
+        sel = t["sorted_idx"][t["first"][nd]:t["first"][nd] + t["count"][nd]]
+        c = np.array([t["comx"][nd], t["comy"][nd], t["comz"][nd]], dtype=np.longdouble)
+        d = xl[sel] - c
+        want = [(ml[sel] * d[:, a] * d[:, b] * d[:, cc]).sum() for a, b, cc in OCT_AXES]
+        scale = ml[sel].sum() * t["width"] ** 3
+        assert np.all(np.abs(o[nd] - np.array(want, dtype=np.float64)) <= 1e-12 * float(scale)), nd
+
+
+def test_octupole_force_remainder_order():
+    """A cluster far from a target, accepted as one node: with the octupole
+    term the error against the exact sum falls as distance^-6 (the next
+    order), with the quadrupole only as distance^-5 -- a wrong sign or
+    coefficient in the octupole term leaves the distance^-5 remainder."""
+    rng = np.random.default_rng(4)
+    cl = rng.normal(size=(7, 3)) * 0.05
+    ratios = {}
+    for order in (2, 3):
+        errs = []
+        for L in (2.0, 4.0):
+            x = np.vstack([cl, [[L, 0.3 * L, -0.2 * L]], [[-L, L, L]]])
+            m = np.concatenate([rng.uniform(0.5, 1.5, 7) if not errs else m[:7], [1e-30, 1e-30]])
+            t = O.build_tree(x, m, 8)
+            q = O.octree_quad(x, m, t)
+            o = O.octree_oct(x, m, t, q) if order == 3 else None
+            a = O.bh_walk(x, m, t, 10.0, quad=q, oct=o)[7]
+            R = x[7] - x[:7]
+            exact = -(m[:7, None] * R / np.linalg.norm(R, axis=1)[:, None] ** 3).sum(axis=0)
+            errs.append(np.linalg.norm(a - exact))
+        ratios[order] = errs[0] / errs[1]
+    assert 25.0 < ratios[2] < 40.0, ratios   # 2^5
+    assert 50.0 < ratios[3] < 80.0, ratios   # 2^6
+
+
+def test_octupole_walk_beats_quadrupole():
+    x, v, m = ic.two_spheres(4096, 5)
+    t = O.build_tree(x, m, 1)
+    q = O.octree_quad(x, m, t)
+    o = O.octree_oct(x, m, t, q)
+    a_d = O.pairwise(x, m)
+    an = np.linalg.norm(a_d, axis=1)
+    e2 = np.linalg.norm(O.bh_walk(x, m, t, 0.8, quad=q) - a_d, axis=1) / an
+    e3 = np.linalg.norm(O.bh_walk(x, m, t, 0.8, quad=q, oct=o) - a_d, axis=1) / an
+    assert np.median(e3) < 0.8 * np.median(e2) and np.percentile(e3, 99) < 0.9 * np.percentile(e2, 99)
+
+
 # ---------------------------------------------------------------- IAS15
 def _ias_coeffs():
     """The shared coefficient header, read through the oracle library's own
