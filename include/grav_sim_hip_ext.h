@@ -162,10 +162,12 @@ ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs);
 /* ---- learned Barnes-Hut correction (ACCELERATION_METHOD_BARNES_HUT_LEARNED) --
  * A monopole walk at the caller's opening angle (0.8 is the intended coarse
  * value) accumulates per-particle features of the accepted nodes (quadrupole
- * force sum G q, size-weighted sums such as s3 = sum M l^3 / d^5, node
- * counts).  The residual a_direct - a_coarse is predicted as
- * G q + G s3 * tscale * net(F): the accepted nodes' quadrupole terms plus, in
- * units of the next multipole order's scale, what an MLP
+ * and octupole force sums G q and G o from the nodes' second and third
+ * moments, size-weighted sums such as s3 = sum M l^3 / d^5, node counts).
+ * The residual a_direct - a_coarse is predicted as
+ * G (q + o) + G s3 * tscale * net(F): the accepted nodes' quadrupole and
+ * octupole terms plus, in units of the next multipole orders' scale, what an
+ * MLP
  *   16 -> 128 -> 128 -> 128 -> 3, tanh-GELU, BF16 matrix-core products with
  *   FP32 accumulation, FP32 master weights
  * learns from the normalised features F; the prediction is added to the

@@ -27,6 +27,10 @@ tests/test_oracle.py, beyond the compiled reference where it builds):
                  the reference's walk is not a pin (leaf mass dropped and key
                  mis-indexing, src/linear_octree.c:563-570,
                  src/acceleration_barnes_hut.c:119-147).
+* ``octree_quad`` / ``octree_oct``  second / third central moments per node:
+                 == direct long-double sums over each node's particles; the
+                 octupole force's remainder falls as distance^-6 (closed
+                 form of a far cluster), the quadrupole's as distance^-5.
 * ``leapfrog``   bitwise == reference launch_simulation(LEAPFROG) final state;
                  time reversibility.
 * ``energy``     bitwise == reference compute_energy.
