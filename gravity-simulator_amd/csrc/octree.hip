@@ -1138,6 +1138,12 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     const float xr = static_cast<float>(ps.x - gc0), yr = static_cast<float>(ps.y - gc1),
                 zr = static_cast<float>(ps.z - gc2);
     const float eps2 = static_cast<float>(args.eps2);
+    // Particle pairs closer than 2^-7 of the group's width: their FP32
+    // coordinates relative to the group centre carry an error of ~2^-25 of
+    // that width each, so such a pair's separation is formed again in FP64
+    // from the particle records (to about 2^-17 of the separation otherwise).
+    const float gw = static_cast<float>(fmax(fmax(hi0 - lo0, hi1 - lo1), hi2 - lo2));
+    const float close2 = __builtin_fmaf(gw * 0x1p-7f, gw * 0x1p-7f, eps2);
     double A0 = 0.0, A1 = 0.0, A2 = 0.0;
     unsigned cnt = 0;
     bool overflow = false;
@@ -1173,6 +1179,7 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     };
     auto sum_parts = [&](int m) {
         float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        float mn = __builtin_inff();
         unsigned self = 0;
         for (int e = 0; e < m; e++) {
             const float4 q = pl[e];
@@ -1183,10 +1190,39 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
             const float ri = __builtin_amdgcn_rsqf(r2);
             const bool me = pi[e] == s;
             const float w = me ? 0.0f : (q.w * ri) * (ri * ri);
+            mn = fminf(mn, me ? __builtin_inff() : r2);
             self += me ? 1u : 0u;
             f0 = __builtin_fmaf(dx, w, f0);
             f1 = __builtin_fmaf(dy, w, f1);
             f2 = __builtin_fmaf(dz, w, f2);
+        }
+        if (__ballot(valid && mn < close2)) {
+            // a close pair in this batch: sum it again with those separations
+            // taken from the FP64 records
+            f0 = f1 = f2 = 0.0f;
+            for (int e = 0; e < m; e++) {
+                const float4 q = pl[e];
+                const int pe = pi[e];
+                const bool me = pe == s;
+                float dx = q.x - xr, dy = q.y - yr, dz = q.z - zr;
+                float r2 = __builtin_fmaf(dx, dx, eps2);
+                r2 = __builtin_fmaf(dy, dy, r2);
+                r2 = __builtin_fmaf(dz, dz, r2);
+                if (valid && !me && r2 < close2) {
+                    const double4 pp = P[pe];
+                    dx = static_cast<float>(pp.x - ps.x);
+                    dy = static_cast<float>(pp.y - ps.y);
+                    dz = static_cast<float>(pp.z - ps.z);
+                    r2 = __builtin_fmaf(dx, dx, eps2);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                }
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                const float w = me ? 0.0f : (q.w * ri) * (ri * ri);
+                f0 = __builtin_fmaf(dx, w, f0);
+                f1 = __builtin_fmaf(dy, w, f1);
+                f2 = __builtin_fmaf(dz, w, f2);
+            }
         }
         A0 += static_cast<double>(f0);
         A1 += static_cast<double>(f1);

@@ -219,6 +219,37 @@ def test_group_walk_coincident_clump_and_outlier_halo(leaf, quad):
     assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
 
 
+@pytest.mark.parametrize("theta,quad", [(0.0, False), (0.5, False), (0.5, True)])
+def test_group_walk_tight_pairs_without_softening(theta, quad):
+    """Twelve pairs 1e-6 apart inside a Plummer sphere, eps = 0: the pairs'
+    separations are a few 1e-6 of their groups' widths, which FP32
+    coordinates relative to the group centre resolve only to a few per cent;
+    the walk forms them from the FP64 particle records instead, so every
+    force stays within FP32 rounding of the oracle's FP64 group walk (at
+    theta = 0, of the FP64 direct sum)."""
+    print("group walk tight pairs", theta, quad, flush=True)
+    n, k = 4096, 12
+    x, v, m = ic.plummer(n, 43)
+    rng = np.random.default_rng(43)
+    partner = rng.choice(n - k, k, replace=False)
+    u = rng.normal(size=(k, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    x[n - k:] = x[partner] + 1e-6 * u
+    t = O.build_tree(x, m, 1)
+    q = O.octree_quad(x, m, t) if quad else None
+    if theta == 0.0:
+        a_o = O.pairwise(x, m, eps=0.0)
+    else:
+        a_o, cnt = O.bh_walk_group(x, m, t, theta, 64, eps=0.0, quad=q)
+    a_g = L.acceleration(x, m, 21 if quad else 20, eps=0.0, theta=theta, leaf=1)
+    if theta > 0.0:
+        assert L.last_walk_interactions() == int(cnt.sum())
+    e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+    pairs = np.concatenate([partner, np.arange(n - k, n)])
+    assert e[pairs].max() <= 1e-5, e[pairs].max()
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
 @pytest.mark.parametrize("n,leaf", [(1, 1), (2, 1), (5, 8), (63, 1), (777, 2), (5000, 40)])
 def test_group_walk_ragged_and_theta_zero(n, leaf):
     """theta = 0 opens every node: the direct sum with n (n - 1) interactions,
