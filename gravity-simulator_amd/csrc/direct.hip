@@ -783,9 +783,10 @@ hipError_t launch_direct_f32(Context *c, const double *x, const double *m, int n
 }
 
 // FP32 direct sum for a list of targets against all n sources (the learned
-// correction's labels and spot check).  Same kernels as launch_direct_f32
-// (kind 1): the targets' positions are gathered into a compact array and the
-// source range is split so that even a few hundred targets fill the chip.
+// correction's labels and spot check).  The shared-target kernel of
+// launch_direct_f32 (kind 2, 16 wavefronts per 64 targets): the targets'
+// positions are gathered into a compact array and the source range is split
+// so that even a few hundred targets fill the chip.
 __global__ void k_gather_targets(const double *__restrict__ x, const int *__restrict__ idx,
                                  int nt, double *__restrict__ xt)
 {
@@ -802,12 +803,16 @@ hipError_t launch_gather_rows3(Context *c, const double *src, const int *idx, in
     return hipGetLastError();
 }
 
+constexpr int TARGET_WAVES = 16;  // wavefronts per 64 targets in launch_direct_f32_targets
+
+// Source chunks: about 1024 workgroups in all, each wavefront of a chunk's
+// workgroup with at least one tile.
 static int targets_split(int n, int nt)
 {
     const int ntiles = f32s_tiles(n);
-    const int wgs = ((nt + 63) / 64 + 3) / 4;
-    int split = (2048 + wgs - 1) / wgs;
-    if (split > ntiles) split = ntiles;
+    const int wgs = (nt + 63) / 64;
+    int split = (1024 + wgs - 1) / wgs;
+    if (split > ntiles / TARGET_WAVES) split = ntiles / TARGET_WAVES;
     return split < 1 ? 1 : split;
 }
 
@@ -845,9 +850,8 @@ hipError_t launch_direct_f32_targets(Context *c, const double *x, const double *
     hipLaunchKernelGGL(k_f32_sources, dim3((n_pad + 255) / 256), dim3(256), 0, c->stream, x, m, n,
                        n_pad, S, C);
     const float eps2 = static_cast<float>(eps * eps);
-    const int waves = (nt + 63) / 64;
-    dim3 grid((waves + 3) / 4, nsplit);
-    launch_f32s_shape<1>(c, grid, xt, nt, S, C, eps2, eps2 > 0.0f, tpc, ntiles, partial);
+    dim3 grid((nt + 63) / 64, nsplit);
+    launch_f32p_shape<TARGET_WAVES>(c, grid, xt, nt, S, C, eps2, eps2 > 0.0f, tpc, ntiles, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_reduce_partials, dim3((3 * nt + 255) / 256), dim3(256), 0, c->stream,

@@ -54,7 +54,7 @@ enum Slot : int {
     SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_PACKED_P, SLOT_SORTED_P, SLOT_MISC,
     SLOT_GROUP_PARENTS,
     // learned correction
-    SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD, SLOT_MLP_X, SLOT_SPOT,
+    SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD, SLOT_MLP_X, SLOT_SPOT, SLOT_MLP_TS, SLOT_MLP_IMG,
     SLOT_COUNT
 };
 

@@ -186,14 +186,14 @@ int gs_learned_num_params(void);
 /* Trains the network on one particle set: coarse walk at theta_coarse,
  * FP32 direct-sum labels for `samples` particles drawn without replacement
  * with `seed`, feature normalisation and target scale from that sample
- * (targets: the residual left after G q divided by G s3, each sample's norm
- * clipped at the sample's 99.9th percentile, scaled to unit RMS), then
- * `epochs` epochs of Adam (batches of `batch` rows, learning rate lr with
- * cosine decay) from a deterministic initialisation on nine tenths of the
- * sample.  On the held-out tenth the output scale alpha in {0, 1/8, ..., 1}
- * that gives the smallest 99th-percentile relative error is chosen (alpha 0
- * leaves the quadrupole sum alone) and folded into the target scale.
- * final_loss (may be NULL): mean squared error of the last epoch's last
+ * (targets: the residual left after G (q + o) divided by G s3, each
+ * sample's norm clipped at the sample's 99.9th percentile, scaled to unit
+ * RMS), then `epochs` epochs of Adam (batches of `batch` rows, learning rate
+ * lr with cosine decay) from a deterministic initialisation on nine tenths
+ * of the sample.  On the held-out tenth the output scale alpha in
+ * {0, 1/8, ..., 1} that gives the smallest 99th-percentile relative error is
+ * chosen (alpha 0 leaves the quadrupole and octupole sums alone) and folded
+ * into the target scale.  final_loss (may be NULL): mean squared error of the last epoch's last
  * batch. */
 ErrorStatus gs_learned_train(const System *system, double softening_length, double theta_coarse,
                              int max_num_particles_per_leaf, int samples, int epochs, int batch,

@@ -45,7 +45,7 @@ def test_forward_matches_oracle(rows):
     assert _rel(Y, Y64) < 3e-2
 
 
-@pytest.mark.parametrize("rows", [37, 512])
+@pytest.mark.parametrize("rows", [37, 512, 4096, 5000])
 def test_gradients_match_oracle(rows):
     w = NN.random_params(12)
     L.learned_set_params(w.astype(np.float32))
