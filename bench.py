@@ -256,6 +256,7 @@ def run_bh(L, ic, learned=True):
     loss = L.learned_train(xt, mt, eps=EPS, theta=0.8, leaf=1, samples=1000000, epochs=10, batch=4096,
                            lr=3e-3, seed=5)
     train_s = time.perf_counter() - t0
+    train_ms = L.learned_training_times()
     rep = L.learned_training_report()
     L.learned_calibrate(xv, mv, eps=EPS, theta=0.8, leaf=1, samples=1 << 18, seed=11)
     cal = L.learned_training_report()
@@ -265,7 +266,7 @@ def run_bh(L, ic, learned=True):
     L.learned_set_policy(P99_TARGET, 1024, 0.5)
     checked = bh_case(L, sim, n5, 18, 0.8, reps=3, label="learned+spot-check")
     spot_p99, fell_back = L.learned_status()
-    res["learned_theta0.8"] = {"train_s": train_s, "train_loss": loss, "held_out": rep,
+    res["learned_theta0.8"] = {"train_s": train_s, "train_ms": train_ms, "train_loss": loss, "held_out": rep,
                                "calibration_seed8": cal, "without_check": raw,
                                "quadrupole_theta0.8": quad08,
                                "learned_over_quadrupole_p99": raw["p99_rel"] / quad08["p99_rel"],

@@ -3,6 +3,7 @@
 // training-set generation from the library's own direct-sum kernel, and the
 // gs_learned_* entry points.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -167,39 +168,53 @@ static int leaf_or_default(int leaf) { return leaf < 1 ? 1 : leaf; }
 // Relative errors of held-out samples nt .. nt+nh-1 with the network's
 // correction scaled by alpha in {0, 1/8, ..., 1}; keeps the alpha of the
 // smallest 99th percentile (ties: smallest median) and records the held-out
-// statistics with and without the network.
+// statistics with and without the network.  ac: the coarse accelerations of
+// the samples (sample order).
 static ErrorStatus choose_output_scale(const std::vector<double> &ac, const std::vector<double> &ref,
-                                       const std::vector<float> &ws, const std::vector<int> &idx,
-                                       int nt, int nh, const std::vector<float> &Yh, double G,
-                                       double tscale, double *alpha_out)
+                                       const std::vector<float> &ws, int nt, int nh,
+                                       const std::vector<float> &Yh, double G, double tscale,
+                                       double *alpha_out)
 {
     MlpModel &M = mlp_model();
     double best_p99 = 0.0, best_med = 0.0, best_alpha = 0.0;
+    // per sample: the prediction without the network (coarse + G (q + o)), the
+    // network's full-scale term, the reference and its squared norm
+    std::vector<double> base(3 * static_cast<size_t>(nh)), dir(3 * static_cast<size_t>(nh));
+    std::vector<double> rn2(nh);
+    for (int h = 0; h < nh; h++) {
+        const int k = nt + h;
+        const double *ak = &ac[3 * static_cast<size_t>(k)];
+        const float *wk = &ws[static_cast<size_t>(MLP_WALK_FEAT) * k];
+        const double q[3] = {wk[0], wk[1], wk[2]}, v[3] = {wk[3], wk[4], wk[5]};
+        double e1[3], e2[3], e3[3];
+        mlp_local_frame(ak, q, v, e1, e2, e3);
+        const double s = G * static_cast<double>(wk[7]) * tscale;
+        const double o[3] = {wk[10], wk[11], wk[12]};
+        const float *y = &Yh[3 * static_cast<size_t>(h)];
+        double r2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            base[3 * h + d] = ak[d] + G * (q[d] + o[d]);
+            dir[3 * h + d] = s * (y[0] * e1[d] + y[1] * e2[d] + y[2] * e3[d]);
+            const double rf = ref[3 * static_cast<size_t>(k) + d];
+            r2 += rf * rf;
+        }
+        rn2[h] = r2;
+    }
     std::vector<double> err(nh);
     for (int step = 0; step <= 8; step++) {
         const double alpha = step / 8.0;
         for (int h = 0; h < nh; h++) {
             const int k = nt + h;
-            const double *ak = &ac[3 * static_cast<size_t>(idx[k])];
-            const float *wk = &ws[static_cast<size_t>(MLP_WALK_FEAT) * k];
-            const double q[3] = {wk[0], wk[1], wk[2]}, v[3] = {wk[3], wk[4], wk[5]};
-            double e1[3], e2[3], e3[3];
-            mlp_local_frame(ak, q, v, e1, e2, e3);
-            const double s = G * static_cast<double>(wk[7]) * tscale * alpha;
-            const double o[3] = {wk[10], wk[11], wk[12]};
-            const float *y = &Yh[3 * static_cast<size_t>(h)];
-            double dn = 0.0, rn = 0.0;
+            double dn = 0.0;
             for (int d = 0; d < 3; d++) {
-                const double pd = ak[d] + G * (q[d] + o[d]) + s * (y[0] * e1[d] + y[1] * e2[d] + y[2] * e3[d]);
+                const double pd = base[3 * h + d] + alpha * dir[3 * h + d];
                 const double rf = ref[3 * static_cast<size_t>(k) + d];
                 dn += (pd - rf) * (pd - rf);
-                rn += rf * rf;
             }
-            err[h] = rn > 0.0 ? std::sqrt(dn / rn) : 0.0;
+            err[h] = rn2[h] > 0.0 ? std::sqrt(dn / rn2[h]) : 0.0;
         }
-        std::vector<double> e = err;
-        const double p99 = percentile99(e);  // sorts e
-        const double med = e[e.size() / 2];
+        const double p99 = percentile99(err);  // sorts err
+        const double med = err[err.size() / 2];
         if (step == 0) {
             M.hold_p99_skip = p99;
             M.hold_med_skip = med;
@@ -223,8 +238,8 @@ int gs_learned_num_params(void) { return MLP_NPARAM; }
 
 // Training / calibration samples of one particle set: ns particles drawn
 // without replacement (partial Fisher-Yates, 64-bit LCG), their FP32
-// split-precision direct-sum labels, the coarse walk's accelerations of all
-// particles and the raw and walk features of the samples (host copies).
+// split-precision direct-sum labels, their coarse walk accelerations and
+// their raw and walk features (host copies, sample order).
 struct Samples {
     int n = 0, ns = 0;
     std::vector<int> idx;
@@ -261,23 +276,25 @@ static ErrorStatus build_samples(Context *c, const System *system, double eps, d
     const size_t off_ref = up(sizeof(int) * ns);
     const size_t off_feat = off_ref + up(sizeof(double) * 3 * ns);
     const size_t off_wf = off_feat + up(sizeof(float) * MLP_F * static_cast<size_t>(ns));
-    GS_TRY(slot(c, SLOT_SPOT, off_wf + sizeof(float) * MLP_WALK_FEAT * static_cast<size_t>(ns),
-                &sp));
+    const size_t off_ac = off_wf + up(sizeof(float) * MLP_WALK_FEAT * static_cast<size_t>(ns));
+    GS_TRY(slot(c, SLOT_SPOT, off_ac + sizeof(double) * 3 * static_cast<size_t>(ns), &sp));
     int *didx = static_cast<int *>(sp);
     double *dref = reinterpret_cast<double *>(static_cast<char *>(sp) + off_ref);
     float *dfs = reinterpret_cast<float *>(static_cast<char *>(sp) + off_feat);
     float *dwf = reinterpret_cast<float *>(static_cast<char *>(sp) + off_wf);
+    double *dac = reinterpret_cast<double *>(static_cast<char *>(sp) + off_ac);
     GS_HIP(hipMemcpyAsync(didx, S.idx.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
     GS_HIP(launch_direct_f32_targets(c, x, m, n, G, eps, didx, ns, dref, buf, lb));
     GS_HIP(mlp_gather_rows(c, F, didx, ns, MLP_F, dfs));
     GS_HIP(mlp_gather_rows(c, wf, didx, ns, MLP_WALK_FEAT, dwf));
+    GS_HIP(launch_gather_rows3(c, a, didx, ns, dac));
     S.ref.resize(3 * static_cast<size_t>(ns));
-    S.ac.resize(3 * static_cast<size_t>(n));
+    S.ac.resize(3 * static_cast<size_t>(ns));
     S.fs.resize(static_cast<size_t>(MLP_F) * ns);
     S.ws.resize(static_cast<size_t>(MLP_WALK_FEAT) * ns);
     GS_HIP(hipMemcpyAsync(S.ref.data(), dref, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
                           c->stream));
-    GS_HIP(hipMemcpyAsync(S.ac.data(), a, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    GS_HIP(hipMemcpyAsync(S.ac.data(), dac, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipMemcpyAsync(S.fs.data(), dfs, sizeof(float) * MLP_F * ns, hipMemcpyDeviceToHost,
                           c->stream));
     GS_HIP(hipMemcpyAsync(S.ws.data(), dwf, sizeof(float) * MLP_WALK_FEAT * ns,
@@ -326,11 +343,17 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
     GS_TRY(mlp_ensure(c));
     MlpModel &M = mlp_model();
     const double G = system->G;
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t) {
+        return std::chrono::duration<double, std::milli>(clk::now() - t).count();
+    };
+    const clk::time_point t0 = clk::now();
     Samples S;
     GS_TRY(build_samples(c, system, softening_length, theta_coarse, max_num_particles_per_leaf,
                          samples, seed, S));
+    M.time_samples_ms = ms_since(t0);
+    const clk::time_point t1 = clk::now();
     const int ns = S.ns;
-    const std::vector<int> &idx = S.idx;
     const std::vector<double> &ref = S.ref, &ac = S.ac;
     const std::vector<float> &fs = S.fs, &ws = S.ws;
 
@@ -341,7 +364,7 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
     // RMS per component
     std::vector<double> r(3 * static_cast<size_t>(ns)), rn(ns);
     for (int k = 0; k < ns; k++) {
-        const double *ak = &ac[3 * static_cast<size_t>(idx[k])];
+        const double *ak = &ac[3 * static_cast<size_t>(k)];
         const float *wk = &ws[static_cast<size_t>(MLP_WALK_FEAT) * k];
         const double q[3] = {wk[0], wk[1], wk[2]}, v[3] = {wk[3], wk[4], wk[5]};
         double e1[3], e2[3], e3[3];
@@ -401,7 +424,13 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
     const int nh = ns >= 20 ? ns / 10 : 0;
     const int nt = ns - nh;
     float *T = X + static_cast<size_t>(ns) * MLP_F;
+    GS_HIP(hipStreamSynchronize(c->stream));
+    M.time_prep_ms = ms_since(t1);
+    const clk::time_point t2 = clk::now();
     GS_TRY(mlp_train(c, X, T, nt, epochs, std::min(batch, nt), lr, seed, final_loss));
+    GS_HIP(hipStreamSynchronize(c->stream));
+    M.time_loop_ms = ms_since(t2);
+    const clk::time_point t3 = clk::now();
     double alpha = 1.0;
     if (nh > 0) {
         std::vector<float> Yh(3 * static_cast<size_t>(nh));
@@ -410,11 +439,12 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
         GS_TRY(mlp_forward_rows(c, X + static_cast<size_t>(nt) * MLP_F, nh, static_cast<float *>(py)));
         GS_HIP(hipMemcpyAsync(Yh.data(), py, sizeof(float) * 3 * nh, hipMemcpyDeviceToHost, c->stream));
         GS_HIP(hipStreamSynchronize(c->stream));
-        GS_TRY(choose_output_scale(ac, ref, ws, idx, nt, nh, Yh, G, tscale, &alpha));
+        GS_TRY(choose_output_scale(ac, ref, ws, nt, nh, Yh, G, tscale, &alpha));
     }
     M.tscale_base = static_cast<float>(tscale);
     M.tscale = static_cast<float>(tscale * alpha);
     GS_HIP(hipStreamSynchronize(c->stream));
+    M.time_select_ms = ms_since(t3);
     M.trained = true;
     return make_success_error_status();
 }
@@ -440,8 +470,7 @@ ErrorStatus gs_learned_calibrate(const System *system, double softening_length, 
     std::vector<float> Yh;
     GS_TRY(forward_samples(c, S, 0, S.ns, Yh));
     double alpha = 0.0;
-    GS_TRY(choose_output_scale(S.ac, S.ref, S.ws, S.idx, 0, S.ns, Yh, system->G, M.tscale_base,
-                               &alpha));
+    GS_TRY(choose_output_scale(S.ac, S.ref, S.ws, 0, S.ns, Yh, system->G, M.tscale_base, &alpha));
     M.tscale = static_cast<float>(M.tscale_base * alpha);
     return make_success_error_status();
 }
@@ -586,6 +615,19 @@ ErrorStatus gs_learned_training_report(double *alpha, double *p99_net, double *p
     if (p99_without) *p99_without = M.hold_p99_skip;
     if (median_net) *median_net = M.hold_med_net;
     if (median_without) *median_without = M.hold_med_skip;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_learned_training_times(double *samples_ms, double *prep_ms, double *loop_ms,
+                                      double *select_ms)
+{
+    const MlpModel &M = mlp_model();
+    if (M.time_loop_ms < 0.0)
+        return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "No training has run");
+    if (samples_ms) *samples_ms = M.time_samples_ms;
+    if (prep_ms) *prep_ms = M.time_prep_ms;
+    if (loop_ms) *loop_ms = M.time_loop_ms;
+    if (select_ms) *select_ms = M.time_select_ms;
     return make_success_error_status();
 }
 

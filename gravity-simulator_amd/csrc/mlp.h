@@ -89,6 +89,10 @@ struct MlpModel {
     // that scale and without it (quadrupole sum only)
     double hold_alpha = -1.0, hold_p99_net = -1.0, hold_p99_skip = -1.0;
     double hold_med_net = -1.0, hold_med_skip = -1.0;
+    // wall-clock milliseconds of the last training: samples (coarse walk,
+    // direct-sum labels, copies), host preparation of the training set, the
+    // training loop, the held-out choice of the output scale
+    double time_samples_ms = -1.0, time_prep_ms = -1.0, time_loop_ms = -1.0, time_select_ms = -1.0;
     // runtime policy of method BARNES_HUT_LEARNED
     double spot_tol = 1e-3;      // p99 relative error that triggers the fallback
     int spot_samples = 1024;     // particles checked against the FP32 direct sum

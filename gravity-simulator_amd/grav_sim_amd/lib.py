@@ -40,7 +40,8 @@ EXPORTS = [
     "gs_set_fp32_config", "gs_get_fp32_config", "gs_acceleration_noalloc", "gs_copy_calibrate", "gs_whfast_ensemble",
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
     "gs_learned_reset", "gs_learned_forward", "gs_learned_gradients", "gs_learned_features",
-    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "gs_learned_calibrate",
+    "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "gs_learned_training_times",
+    "gs_learned_calibrate",
     "ias15", "whfast",
     "rk_embedded",
     "free_memory_int32", "free_memory_double", "load_built_in_system_python", "keplerian_to_cartesian",
@@ -136,6 +137,7 @@ def load(path=None):
         "gs_learned_set_policy": (ES, [C.c_double, C.c_int, C.c_double]),
         "gs_learned_status": (ES, [PD, PI]),
         "gs_learned_training_report": (ES, [PD, PD, PD, PD, PD]),
+        "gs_learned_training_times": (ES, [PD, PD, PD, PD]),
         "gs_learned_calibrate": (ES, [pS, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint]),
         "ias15": (ES, [pS, C.POINTER(_abi.IntegratorParam), pP, C.POINTER(_abi.OutputParam),
                        C.POINTER(_abi.SimulationStatus), C.POINTER(_abi.Settings), C.c_double]),
@@ -519,6 +521,13 @@ def learned_status():
     f = C.c_int(0)
     check(load().gs_learned_status(C.byref(p), C.byref(f)), "gs_learned_status")
     return p.value, f.value
+
+
+def learned_training_times():
+    """dict(samples_ms, prep_ms, loop_ms, select_ms) of the last training."""
+    v = [C.c_double(0.0) for _ in range(4)]
+    check(load().gs_learned_training_times(*[C.byref(t) for t in v]), "gs_learned_training_times")
+    return dict(zip(("samples_ms", "prep_ms", "loop_ms", "select_ms"), [t.value for t in v]))
 
 
 def learned_training_report():

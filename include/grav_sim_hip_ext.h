@@ -203,6 +203,13 @@ ErrorStatus gs_learned_train(const System *system, double softening_length, doub
  * scale and without it.  Any pointer may be NULL. */
 ErrorStatus gs_learned_training_report(double *alpha, double *p99_net, double *p99_without,
                                        double *median_net, double *median_without);
+/* Wall-clock milliseconds of the last gs_learned_train: the samples (coarse
+ * walk, direct-sum labels, copies to the host), the host's preparation of the
+ * training set (targets, normalisation, upload), the training loop on the GPU
+ * and the held-out choice of the output scale.  Any pointer may be NULL;
+ * GRAV_VALUE_ERROR before the first training. */
+ErrorStatus gs_learned_training_times(double *samples_ms, double *prep_ms, double *loop_ms,
+                                      double *select_ms);
 /* Chooses the output scale again, on `samples` particles of another particle
  * set (a validation realisation): the same candidates and rule as in
  * gs_learned_train, the trained weights unchanged; the report then holds
