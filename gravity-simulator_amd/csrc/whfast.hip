@@ -300,11 +300,10 @@ __device__ void sort_pairs(WfShared &S, int n)
     bitonic_sort(S, n);
 }
 
-// distances from the particle whose id is 0 (src/system.c:1232-1277), then
-// sort and permute (pos, vel, m, ids) through the alternate buffers; only
-// the particles that changed place move.
-__device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, double *m,
-                                 int *ids, double *pos2, double *vel2, double *m2, int *ids2)
+// Distances from the particle whose id is 0 (src/system.c:1232-1277) and
+// their sort: S.idx[i] = the current place of the particle that goes to
+// sorted place i (the identity when there is no particle with id 0).
+__device__ void sort_keys(WfShared &S, int n, const double *pos, const int *ids)
 {
     if (threadIdx.x == 0) {
         int pr = -1;
@@ -320,7 +319,11 @@ __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, d
     }
     __syncthreads();
     const int pr = S.primary;
-    if (pr < 0) return;  // no particle with id 0: the reference reports an error; keep order
+    if (pr < 0) {  // no particle with id 0: the reference reports an error; keep order
+        for (int i = threadIdx.x; i < n; i += WF_BLOCK) S.idx[i] = static_cast<short>(i);
+        __syncthreads();
+        return;
+    }
     const double p0 = pos[3 * pr], p1 = pos[3 * pr + 1], p2 = pos[3 * pr + 2];
     for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
         S.key[i] = i == pr ? 0.0 : norm3(pos[3 * i] - p0, pos[3 * i + 1] - p1, pos[3 * i + 2] - p2);
@@ -328,6 +331,14 @@ __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, d
     }
     __syncthreads();
     sort_pairs(S, n);
+}
+
+// sort_keys, then permute (pos, vel, m, ids) through the alternate buffers;
+// only the particles that changed place move.
+__device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, double *m,
+                                 int *ids, double *pos2, double *vel2, double *m2, int *ids2)
+{
+    sort_keys(S, n, pos, ids);
     for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
         const int s = S.idx[i];
         if (s == i) continue;
@@ -358,15 +369,18 @@ __device__ void sort_by_distance(WfShared &S, int n, double *pos, double *vel, d
 // Massive list (m != 0, or every particle for the pairwise form) and the
 // cumulative masses eta at each massive particle, summed in index order
 // (src/integrator_whfast.c:1266-1279; massless particles add exactly 0).
-__device__ void massive_list(WfShared &S, int n, const double *m, int method)
+__device__ void massive_list(WfShared &S, int n, const double *m, int method,
+                             const short *perm = nullptr)
 {
+    // the mass at sorted place i: m[i], or m[perm[i]] before the arrays are permuted
+    auto mass = [&](int i) { return m[perm ? perm[i] : i]; };
     // ordinals of massive particles, ascending: bit mask, then one wavefront
     // turns the mask words into the list (lane w owns particles 32w..32w+31)
     for (int w = threadIdx.x; w < WF_MAX_N / 32; w += WF_BLOCK) S.mbits[w] = 0u;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += WF_BLOCK)
         // particle 0 (the primary) always anchors the list
-        if (i == 0 || method == 1 || m[i] != 0.0) atomicOr(&S.mbits[i >> 5], 1u << (i & 31));
+        if (i == 0 || method == 1 || mass(i) != 0.0) atomicOr(&S.mbits[i >> 5], 1u << (i & 31));
     __syncthreads();
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
@@ -392,9 +406,9 @@ __device__ void massive_list(WfShared &S, int n, const double *m, int method)
         double e = 0.0;
         for (int q = 0; q < nm; q++) {
             const int i = S.mlist[q];
-            e = (i == 0) ? m[0] : e + m[i];
+            e = (i == 0) ? mass(0) : e + mass(i);
             S.mprefix[q] = e;
-            S.mm[q] = m[i];
+            S.mm[q] = mass(i);
         }
     }
     __syncthreads();
@@ -906,15 +920,19 @@ __global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
     const WfSys sy = A.sys[blockIdx.x];
     const long long o = sy.off;
     int n = A.count[blockIdx.x];
-    double *x = A.x + 3 * o, *jx = A.jx + 3 * o, *jv = A.jv + 3 * o;
-    double *jx2 = A.jx2 + 3 * o, *jv2 = A.jv2 + 3 * o;
+    double *x = A.x + 3 * o;
+    // the state (jx, jv, m, ids) alternates between the two buffer sets: the
+    // drift reads each particle from its old place in one set and writes it
+    // to its sorted place in the other, so the sort's permutation costs no
+    // pass of its own; the other set is scratch for the rest of the step
+    double *jx = A.jx + 3 * o, *jv = A.jv + 3 * o, *jx2 = A.jx2 + 3 * o, *jv2 = A.jv2 + 3 * o;
     double *m = A.m + o, *m2 = A.m2 + o;
     int *ids = A.ids + o, *ids2 = A.ids2 + o;
     const double eps3 = A.eps * A.eps * A.eps;
     for (long long s = A.step0; s < A.step0 + A.nsteps; s++) {
         const double dt = (s == A.total_steps - 1) ? A.dt_last : A.dt;
         const bool last = s == A.step0 + A.nsteps - 1;
-        sort_by_distance(S, n, jx, jv, m, ids, jx2, jv2, m2, ids2);
+        sort_keys(S, n, jx, ids);
         // while no massive particle changes place, the massive list, masses
         // and cumulative masses stand
         int moved = s == A.step0 ? 1 : 0;
@@ -924,7 +942,7 @@ __global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
                 const int j = S.mlist[q];
                 if (S.idx[j] != j) moved = 1;
             }
-        if (__syncthreads_or(moved)) massive_list(S, n, m, A.method);
+        if (__syncthreads_or(moved)) massive_list(S, n, m, A.method, S.idx);
         fill_eta(S, n);
         if (threadIdx.x == 0) {
             S.any_remove = 0;
@@ -932,21 +950,42 @@ __global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
         }
         for (int i = threadIdx.x; i < n; i += WF_BLOCK) flags[i] = 0;
         __syncthreads();
-        const double gm0 = A.G * m[0];
+        const double gm0 = A.G * m[S.idx[0]];
         int bad = 0;
-        for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
-            const double gm = gm0 * S.key[i] / S.key[i - 1];
-            const int f = kepler_drift(&jx[3 * i], &jv[3 * i], gm, dt);
-            if (f & 2) atomicOr(&A.status[blockIdx.x], 1);
-            if ((f & 1) && A.remove_invalid) {
-                flags[i] = 1;
-                S.any_remove = 1;
+        // drift (sorted place i >= 1), reading the old place S.idx[i] of jx,
+        // jv, m, ids and writing place i of the other set
+        for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+            const int from = S.idx[i];
+            double q[3] = {jx[3 * from], jx[3 * from + 1], jx[3 * from + 2]};
+            double v[3] = {jv[3 * from], jv[3 * from + 1], jv[3 * from + 2]};
+            const double mi = m[from];
+            const int id = ids[from];
+            if (i >= 1) {
+                const double gm = gm0 * S.key[i] / S.key[i - 1];
+                const int f = kepler_drift(q, v, gm, dt);
+                if (f & 2) atomicOr(&A.status[blockIdx.x], 1);
+                if ((f & 1) && A.remove_invalid) {
+                    flags[i] = 1;
+                    S.any_remove = 1;
+                }
+                for (int d = 0; d < 3; d++)
+                    if (!isfinite(q[d])) bad |= 1 << d;
             }
-            for (int d = 0; d < 3; d++)
-                if (!isfinite(jx[3 * i + d])) bad |= 1 << d;
+            for (int d = 0; d < 3; d++) {
+                jx2[3 * i + d] = q[d];
+                jv2[3 * i + d] = v[d];
+            }
+            m2[i] = mi;
+            ids2[i] = id;
         }
         if (bad) atomicOr(&S.bad, bad);
         __syncthreads();
+        {
+            double *t = jx; jx = jx2; jx2 = t;
+            t = jv; jv = jv2; jv2 = t;
+            t = m; m = m2; m2 = t;
+            int *u = ids; ids = ids2; ids2 = u;
+        }
         // (S.bad is reused for the velocities below)
         const int pos_bad = S.bad;
         if (S.any_remove) {
@@ -981,6 +1020,18 @@ __global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
             if (threadIdx.x < 3) jv2[threadIdx.x] = jv[threadIdx.x];
             __syncthreads();
             j2c_block(n, jv2, A.v + 3 * o, m, S, X);
+        }
+    }
+    // an odd number of steps leaves the state in the second set: back to the first
+    if (jx != A.jx + 3 * o) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += WF_BLOCK) {
+            for (int d = 0; d < 3; d++) {
+                jx2[3 * i + d] = jx[3 * i + d];
+                jv2[3 * i + d] = jv[3 * i + d];
+            }
+            m2[i] = m[i];
+            ids2[i] = ids[i];
         }
     }
     if (threadIdx.x == 0) A.count[blockIdx.x] = n;

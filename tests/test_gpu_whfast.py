@@ -44,6 +44,21 @@ def test_whfast_matches_oracle_bitwise():
         assert np.array_equal(ids, io) and np.array_equal(x, xo) and np.array_equal(v, vo), s
 
 
+@pytest.mark.parametrize("tf", [2.57, 0.03])
+def test_whfast_odd_step_counts_match_oracle_bitwise(tf):
+    """The state alternates between two buffer sets from step to step (the
+    drift writes each particle to its sorted place in the other set): launches
+    of an odd number of steps (257 = 256 + 1, and 3) end in the second set and
+    are copied back; the result is still the oracle's, bit for bit."""
+    off, X, V, M = ic.ensemble(3, 37, max_test=200)
+    res = L.whfast_ensemble(off, X, V, M, 1.0, 0.01, tf, method=2)
+    for s in range(3):
+        sl = slice(off[s], off[s + 1])
+        io, xo, vo, mo = O.whfast(X[sl], V[sl], M[sl], 1.0, 0.01, tf, method=2)
+        ids, x, v, m = res[s]
+        assert np.array_equal(ids, io) and np.array_equal(x, xo) and np.array_equal(v, vo), s
+
+
 def test_whfast_planar_systems_match_oracle_bitwise():
     """Planar systems: every z is 0, so the z component of the Jacobi ->
     Cartesian recurrence carries c == 0 and takes the serial path, while x and
