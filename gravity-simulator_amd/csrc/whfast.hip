@@ -414,27 +414,33 @@ __device__ void massive_list(WfShared &S, int n, const double *m, int method,
     __syncthreads();
 }
 
-// eta[i]: the cumulative mass up to sorted position i
-__device__ __forceinline__ double eta_at(const WfShared &S, int i)
-{
-    // largest massive ordinal <= i
-    int lo = 0, hi = S.nm - 1, best = 0;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        if (S.mlist[mid] <= i) {
-            best = mid;
-            lo = mid + 1;
-        } else {
-            hi = mid - 1;
-        }
+// The largest massive ordinal g with mlist[g] <= i for the ascending i of
+// one thread: the ordinal and the next massive place stay in registers and
+// advance only when i passes that place (mlist[0] = 0 starts it).
+struct MassiveCursor {
+    int g = 0, next = 0;
+    __device__ __forceinline__ void init(const WfShared &S, int nm)
+    {
+        g = 0;
+        next = nm > 1 ? S.mlist[1] : 0x7fffffff;
     }
-    return S.mprefix[best];
-}
+    __device__ __forceinline__ int at(const WfShared &S, int nm, int i)
+    {
+        while (i >= next) {
+            g++;
+            next = g + 1 < nm ? S.mlist[g + 1] : 0x7fffffff;
+        }
+        return g;
+    }
+};
 
 // eta[i] for every particle, into S.key (free between the sorts).
 __device__ void fill_eta(WfShared &S, int n)
 {
-    for (int i = threadIdx.x; i < n; i += WF_BLOCK) S.key[i] = eta_at(S, i);
+    const int nm = S.nm < WF_MAX_MASSIVE ? S.nm : WF_MAX_MASSIVE;
+    MassiveCursor mc;
+    mc.init(S, nm);
+    for (int i = threadIdx.x; i < n; i += WF_BLOCK) S.key[i] = S.mprefix[mc.at(S, nm, i)];
     __syncthreads();
 }
 
@@ -468,22 +474,6 @@ __device__ void j2c_component(int n, int d, const double *jq, double *q, const d
         c = c2;
     }
     q[d] = c / m[0];
-}
-
-// Largest massive ordinal g with mlist[g] <= i (i >= 0).
-__device__ __forceinline__ int massive_floor(const WfShared &S, int nm, int i)
-{
-    int lo = 0, hi = nm - 1, g = 0;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        if (S.mlist[mid] <= i) {
-            g = mid;
-            lo = mid + 1;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    return g;
 }
 
 // Quotient of the recurrence for the massless particle i in the run above
@@ -567,8 +557,10 @@ __device__ void j2c_fill(int n, const double *jq, double *q, const double *m, co
         else
             q[tid] = X.j.q0[tid];
     }
+    MassiveCursor mc;
+    mc.init(S, nm);
     for (int i = 1 + tid; i < n; i += WF_BLOCK) {
-        const int g = massive_floor(S, nm, i);
+        const int g = mc.at(S, nm, i);
         const bool event = S.mlist[g] == i;
         for (int d = 0; d < 3; d++) {
             if (d == 0 ? f0 : (d == 1 ? f1 : f2)) continue;
@@ -833,8 +825,10 @@ __device__ void kick_from_quotients(WfShared &S, const WfScratch &X, int n, doub
                      jv_sync);
     }
     if (method == 2) {
+        MassiveCursor mc;
+        mc.init(S, nm);
         for (int i = 1 + threadIdx.x; i < n; i += WF_BLOCK) {
-            const int g = massive_floor(S, nm, i);
+            const int g = mc.at(S, nm, i);
             if (S.mlist[g] == i) continue;
             const double jxi[3] = {jx[3 * i], jx[3 * i + 1], jx[3 * i + 2]};
             double xi[3];
