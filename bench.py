@@ -521,6 +521,17 @@ def main():
     # the HBM copy calibration first: the autotuner's FP32 load then runs
     # right before the timed region, so the clocks are up when it starts
     copy_gbs = L.copy_calibrate(1 << 28, 5)
+    # FP32 issue rates (cycles per wave64 instruction, measured at the
+    # device's nominal clock) of the instruction forms the kernels use
+    issue = L.issue_rates()
+    simd_cycles_per_s = 4.0 * info.cu_count * info.clock_khz * 1e3
+    fma_peak = simd_cycles_per_s * 64 * 2 / issue["fma"] / 1e12
+    # the direct-sum interaction (counters: 13.1 VALU wave-instructions per 64
+    # interactions): 6 FMA, 6 MUL/SUB and one v_rsq_f32 per lane
+    mix_cycles = 6 * issue["fma"] + 6 * issue["mul"] + issue["rsq"]
+    mix_ceiling = simd_cycles_per_s * 64 * FLOPS_PER_INTERACTION / mix_cycles / 1e12
+    log(f"issue: fma {issue['fma']:.2f}, mul {issue['mul']:.2f}, rsq {issue['rsq']:.2f} cycles per wave64 "
+        f"instruction; FMA chains {fma_peak:.1f} TFLOP/s; direct-sum mix ceiling {mix_ceiling:.1f} TFLOP/s")
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
@@ -572,7 +583,12 @@ def main():
                                                               tune["source_split"])),
                      "algorithmic_bytes": ALGORITHMIC_F32_BYTES,
                      "kernel": "direct_f32", "kernel_avg_ms": kavg * 1e3,
-                     "flops_per_launch": FLOPS_PER_INTERACTION * inter},
+                     "flops_per_launch": FLOPS_PER_INTERACTION * inter,
+                     "fp32_issue_cycles": issue,
+                     "measured_fma_peak": fma_peak,
+                     "frac_of_measured_fma_peak": achieved / fma_peak,
+                     "instruction_mix_ceiling": mix_ceiling,
+                     "frac_of_instruction_mix_ceiling": achieved / mix_ceiling},
         "accuracy": {"vs": "FP64 GPU direct sum, 4096 sampled targets", "median_rel": med, "p99_rel": p99,
                      "max_rel": mx},
         "hbm_copy_gbs": copy_gbs,

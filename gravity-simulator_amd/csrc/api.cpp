@@ -751,6 +751,44 @@ ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs)
     return make_success_error_status();
 }
 
+ErrorStatus gs_issue_rates(double *fma_cycles, double *mul_cycles, double *rsq_cycles)
+{
+    Context *c;
+    GS_TRY(context(&c));
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int blocks = 8 * 4 * c->cu_count;  // 32 wavefronts per CU, several rounds
+    void *p;
+    GS_TRY(slot(c, SLOT_MISC, sizeof(float) * 256 * static_cast<size_t>(blocks), &p));
+    float *out = static_cast<float *>(p);
+    hipDeviceProp_t prop;
+    GS_HIP(hipGetDeviceProperties(&prop, c->device));
+    const double clock_hz = static_cast<double>(prop.clockRate) * 1e3;
+    const double simds = 4.0 * c->cu_count;
+    double *res[3] = {fma_cycles, mul_cycles, rsq_cycles};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (!e) e = hipEventCreate(&e1);
+    for (int kind = 0; kind < 3 && !e; kind++) {
+        e = launch_issue_probe(c, kind, out, blocks);  // warm-up
+        float best = 0.0f;
+        for (int r = 0; r < 3 && !e; r++) {
+            e = hipEventRecord(e0, c->stream);
+            if (!e) e = launch_issue_probe(c, kind, out, blocks);
+            if (!e) e = hipEventRecord(e1, c->stream);
+            if (!e) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (!e) e = hipEventElapsedTime(&ms, e0, e1);
+            if (!e && (r == 0 || ms < best)) best = ms;
+        }
+        if (!e && res[kind])
+            *res[kind] = best * 1e-3 * clock_hz * simds / issue_probe_instructions(blocks);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e) return hip_error(e, "issue-rate probe", __FILE__, __LINE__, __func__);
+    return make_success_error_status();
+}
+
 // ---------------------------------------------------------------- FP32 autotuner
 // (block, targets per thread): block 4/8/16 = the shared-target
 // scalar-source kernel with that many wavefronts per 64 targets, 0 = the

@@ -859,6 +859,51 @@ hipError_t launch_direct_f32_targets(Context *c, const double *x, const double *
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ issue rates
+// Chains of one FP32 instruction form (8 independent chains per lane, VGPR
+// operands, so the VOP2 encodings v_fmac_f32 / v_mul_f32 and v_rsq_f32):
+// their time gives the cycles per wave64 instruction of each form, from
+// which an FP32 kernel's instruction mix bounds its rate.  Each thread
+// writes one float.
+constexpr int ISSUE_ITERS = 4096;
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_issue(float *__restrict__ out, float a0, float b0)
+{
+    const float a = a0 + threadIdx.x * 1e-9f, b = b0 - threadIdx.x * 1e-9f;
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = static_cast<float>(threadIdx.x + k + 1);
+    for (int i = 0; i < ISSUE_ITERS; i++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (KIND == 0) x[k] = __builtin_fmaf(a, b, x[k]);
+            if (KIND == 1) x[k] = x[k] * a;
+            if (KIND == 2) x[k] = __builtin_amdgcn_rsqf(x[k]);
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += x[k];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t launch_issue_probe(Context *c, int kind, float *out, int blocks)
+{
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_issue<0>, dim3(blocks), dim3(256), 0, c->stream, out, 0.999f, 0.001f); break;
+        case 1: hipLaunchKernelGGL(k_issue<1>, dim3(blocks), dim3(256), 0, c->stream, out, 0.999f, 0.001f); break;
+        case 2: hipLaunchKernelGGL(k_issue<2>, dim3(blocks), dim3(256), 0, c->stream, out, 0.999f, 0.001f); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+double issue_probe_instructions(int blocks)
+{
+    return static_cast<double>(blocks) * 256.0 / 64.0 * 8.0 * ISSUE_ITERS;
+}
+
 // ------------------------------------------------------------------ copy
 // Plain streaming copy (16 B per lane per iteration), used to calibrate the
 // HBM byte counters and to measure achievable copy bandwidth.

@@ -127,6 +127,11 @@ hipError_t launch_direct_f32_targets(Context *c, const double *x, const double *
 size_t direct_f32_targets_bytes(int n, int nt);
 // out[t] = src[idx[t]] for rows of 3 doubles.
 hipError_t launch_gather_rows3(Context *c, const double *src, const int *idx, int nt, double *out);
+// FP32 issue-rate probe: kind 0 v_fmac_f32, 1 v_mul_f32, 2 v_rsq_f32 chains,
+// `blocks` workgroups of 256 threads, one float each into out; and the
+// number of wave64 instructions of that form one launch issues.
+hipError_t launch_issue_probe(Context *c, int kind, float *out, int blocks);
+double issue_probe_instructions(int blocks);
 // FP64 sums for the listed targets split over source chunks (the sampled
 // error reference: same terms, chunked summation order).
 hipError_t launch_direct_f64_sampled(Context *c, const double *x, const double *m, int n, double G,

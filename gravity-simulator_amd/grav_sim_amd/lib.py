@@ -37,7 +37,8 @@ EXPORTS = [
     "gs_sim_create", "gs_sim_destroy", "gs_sim_upload", "gs_sim_download", "gs_sim_acceleration",
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
-    "gs_set_fp32_config", "gs_get_fp32_config", "gs_acceleration_noalloc", "gs_copy_calibrate", "gs_whfast_ensemble",
+    "gs_set_fp32_config", "gs_get_fp32_config", "gs_acceleration_noalloc", "gs_copy_calibrate", "gs_issue_rates",
+    "gs_whfast_ensemble",
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
     "gs_learned_reset", "gs_learned_forward", "gs_learned_gradients", "gs_learned_features",
     "gs_learned_set_policy", "gs_learned_status", "gs_learned_training_report", "gs_learned_training_times",
@@ -123,6 +124,7 @@ def load(path=None):
         "gs_get_fp32_config": (ES, [PI, PI, PI]),
         "gs_acceleration_noalloc": (ES, [C.c_void_p, C.POINTER(System), C.POINTER(AccelerationParam)]),
         "gs_copy_calibrate": (ES, [C.c_size_t, C.c_int, PD]),
+        "gs_issue_rates": (ES, [PD, PD, PD]),
         "gs_whfast_ensemble": (ES, [C.c_int, PI64, PI, PD, PD, PD, C.c_double, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_bool, PI]),
         "gs_learned_num_params": (C.c_int, []),
@@ -290,6 +292,13 @@ def copy_calibrate(nbytes=1 << 28, reps=5):
     g = C.c_double()
     check(load().gs_copy_calibrate(int(nbytes), int(reps), C.byref(g)), "gs_copy_calibrate")
     return g.value
+
+
+def issue_rates():
+    """dict(fma, mul, rsq): cycles per wave64 instruction of FP32 chains."""
+    v = [C.c_double(0.0) for _ in range(3)]
+    check(load().gs_issue_rates(*[C.byref(t) for t in v]), "gs_issue_rates")
+    return dict(zip(("fma", "mul", "rsq"), [t.value for t in v]))
 
 
 def set_fp32_config(block, ipt, split):

@@ -159,6 +159,12 @@ ErrorStatus gs_last_walk_interactions(long long *count);
  * is the reference point for the HBM byte counters. */
 ErrorStatus gs_copy_calibrate(size_t bytes, int reps, double *gbs);
 
+/* Cycles per wave64 instruction (per SIMD, at the device's nominal clock)
+ * of independent chains of v_fmac_f32, v_mul_f32 and v_rsq_f32 over the
+ * whole GPU: the measured issue rates behind the FP32 kernels' instruction
+ * mix ceilings.  Any pointer may be NULL.  No counterpart in the reference. */
+ErrorStatus gs_issue_rates(double *fma_cycles, double *mul_cycles, double *rsq_cycles);
+
 /* ---- learned Barnes-Hut correction (ACCELERATION_METHOD_BARNES_HUT_LEARNED) --
  * A monopole walk at the caller's opening angle (0.8 is the intended coarse
  * value) accumulates per-particle features of the accepted nodes (quadrupole

@@ -127,3 +127,12 @@ def test_autotuner_picks_a_valid_shape_that_meets_the_error_bound():
         L.set_fp32_config(*saved)
     e = rel_err(a32, O.pairwise(x, m, eps=1e-3))
     assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (block, ipt, split, np.median(e), e.max())
+
+
+def test_issue_rate_probe_is_plausible():
+    """The FP32 issue-rate probe behind the bench's instruction-mix ceiling: every
+    form takes at least one cycle per wave64 instruction per SIMD, the FMA at
+    least as long as the MUL, the transcendental v_rsq_f32 longer than the MUL."""
+    r = L.issue_rates()
+    print("issue rates (cycles per wave64 instruction):", r)
+    assert 1.0 <= r["mul"] <= 1.1 * r["fma"] and r["rsq"] > r["mul"]
