@@ -11,18 +11,25 @@ Printed as one JSON line:
   value      directed interactions/s = steps * N (N-1) / wall time
   roofline   dominant kernel (direct_f32): 20 flops per interaction over its
              HIP-event-timed average launch, against the MI355X FP32 vector
-             peak; traffic = HBM bytes per launch from rocprofv3 --pmc
+             peak; traffic = HBM bytes per launch from rocprofv3 --pmc; the
+             measured issue rates of v_fmac_f32 / v_mul_f32 / v_rsq_f32
+             (gs_issue_rates) and the ceiling they put on the kernel's
+             instruction mix (6 FMA + 6 MUL/SUB + 1 rsq per 64 interactions
+             per lane-wave), with the kernel's fraction of it
   accuracy   relative error vs the FP64 GPU direct sum (4096 sampled targets)
   bh         Barnes-Hut, two Plummer spheres (scale radius 1, centres 10
              apart), relative errors vs the FP64 GPU direct sum on 4096
              strided particles:
              n2e20  N = 2^20 (seed 4), theta = 0.5, monopole (the reference's
                     method): particles/s of sort + build + walk
-             n2e21  N = 2^21 (seed 5): "plain" = monopole theta = 0.5;
-                    candidates meeting p99 <= 1e-3 (monopole at the largest
-                    theta that does, quadrupole, learned correction trained on
-                    10^6 samples of seed 7); "chosen" = the fastest of those;
-                    the learned correction at theta = 0.8 with its spot check
+             n2e21  N = 2^21 (seed 5): "plain" = monopole theta = 0.5,
+                    and the group walk at theta = 0.5; candidates meeting
+                    p99 <= 1e-3 (monopole, quadrupole, group walk and group
+                    walk with quadrupoles at the largest theta that does, the
+                    learned correction trained on 10^6 samples of seed 7);
+                    "chosen" = the fastest of those; the learned correction
+                    at theta = 0.8 (training times, calibration on seed 8,
+                    against the quadrupole walk at 0.8, with its spot check)
   whfast     4096 planetary systems (1 star + 8 planets + U{0..2000} test
              particles), WHFast dt = 0.01, 10^4 steps: particle-steps/s of
              the batched kernel
