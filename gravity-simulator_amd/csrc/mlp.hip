@@ -293,7 +293,8 @@ constexpr int TS_ROWS = 128;              // rows per slice (= per workgroup)
 constexpr int TS_WAVES = TS_ROWS / 16;    // one 16-row tile per wavefront
 constexpr int TS_LD = MLP_H + 8;          // LDS row of 128 bf16, padded by 16 B
 constexpr int TS_LD1 = MLP_F + 8;         // W1^T rows: K = 16, padded by 16 B
-constexpr int TS_MAX_SLICES = 32;         // = MLP_SPLIT_MAX: batches of up to 4096 rows
+constexpr int TS_MAX_SLICES = 32;         // batches of up to 4096 rows
+static_assert(TS_MAX_SLICES <= MLP_SPLIT_MAX, "the gradient slices fit the split buffer");
 constexpr size_t TS_HH = static_cast<size_t>(TS_ROWS) * MLP_H;  // one 128 x 128 block
 
 struct alignas(16) TsLds {
