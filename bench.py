@@ -231,19 +231,20 @@ def run_bh(L, ic, learned=True):
             res["plain_at_target"] = r
             cands.append(r)
             break
-    for th in (0.6, 0.55, 0.5, 0.45, 0.4):
+    # opening angles in steps of 0.02 near where each method crosses the target
+    for th in (0.6, 0.56, 0.54, 0.52, 0.5, 0.45, 0.4):
         r = bh_case(L, sim, n5, 19, th, reps=3)
         if r["p99_rel"] <= P99_TARGET:
             res["quadrupole_at_target"] = r
             cands.append(r)
             break
-    for th in (0.45, 0.4, 0.35, 0.3, 0.25):
+    for th in (0.46, 0.44, 0.42, 0.4, 0.38, 0.35, 0.3, 0.25):
         r = bh_case(L, sim, n5, 20, th, reps=3)
         if r["p99_rel"] <= P99_TARGET:
             res["group_at_target"] = r
             cands.append(r)
             break
-    for th in (0.7, 0.65, 0.6, 0.55, 0.5, 0.45):
+    for th in (0.7, 0.65, 0.62, 0.6, 0.58, 0.56, 0.55, 0.5, 0.45):
         r = bh_case(L, sim, n5, 21, th, reps=3)
         if r["p99_rel"] <= P99_TARGET:
             res["group_quadrupole_at_target"] = r
