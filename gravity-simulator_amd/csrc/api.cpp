@@ -18,11 +18,14 @@ ErrorStatus learned_acceleration(Context *c, const double *x, const double *m, i
                                  const AccelerationParam *p, double *a, long long *interactions);
 
 static long long g_last_walk_interactions = 0;
+static long long g_last_walk_far_pairs = 0;
+static double g_group_expansion_ratio = 0.1;
 
 ErrorStatus device_acceleration(Context *c, const double *x, const double *m, int n, double G,
                                 const AccelerationParam *p, double *a)
 {
     const double eps = p->softening_length;
+    g_last_walk_far_pairs = 0;
     switch (p->method) {
         case ACCELERATION_METHOD_PAIRWISE:
         case ACCELERATION_METHOD_MASSLESS:
@@ -57,6 +60,13 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
                                    q ? 2 : 1));
             return make_success_error_status();
         }
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION: {
+            DeviceTree *t;
+            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, false, &t));
+            GS_TRY(tree_walk_group(c, t, G, eps, p->opening_angle, a, &g_last_walk_interactions, 1,
+                                   g_group_expansion_ratio, &g_last_walk_far_pairs));
+            return make_success_error_status();
+        }
         case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
             return learned_acceleration(c, x, m, n, G, p, a, &g_last_walk_interactions);
         default:
@@ -69,7 +79,7 @@ static bool known_method(int method)
 {
     return (method >= ACCELERATION_METHOD_PAIRWISE && method <= ACCELERATION_METHOD_BARNES_HUT) ||
            (method >= ACCELERATION_METHOD_PAIRWISE_FP32 &&
-            method <= ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE);
+            method <= ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION);
 }
 
 // Host-buffer form used by acceleration(): H2D x and m, compute, D2H a.
@@ -276,6 +286,8 @@ static const char *benchmark_method_name(int method)
         case ACCELERATION_METHOD_BARNES_HUT_GROUP: return "Barnes-Hut (group walk)";
         case ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE:
             return "Barnes-Hut (group walk, quadrupole)";
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION:
+            return "Barnes-Hut (group walk, local expansions)";
         default: return nullptr;
     }
 }
@@ -716,6 +728,29 @@ ErrorStatus gs_last_walk_interactions(long long *count)
 {
     if (!count) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "count is NULL");
     *count = g_last_walk_interactions;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_last_walk_expansion_pairs(long long *count)
+{
+    if (!count) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "count is NULL");
+    *count = g_last_walk_far_pairs;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_set_group_expansion_ratio(double phi)
+{
+    if (!(phi > 0.0 && phi < 1.0))
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                    "Group expansion ratio must be in (0, 1). Got: %g", phi);
+    g_group_expansion_ratio = phi;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_get_group_expansion_ratio(double *phi)
+{
+    if (!phi) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "phi is NULL");
+    *phi = g_group_expansion_ratio;
     return make_success_error_status();
 }
 

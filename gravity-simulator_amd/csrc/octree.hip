@@ -703,6 +703,7 @@ struct WalkArgs {
     double theta2;
     double eps2;
     double G;
+    double phi2;  // group walk: squared expansion ratio (0: no local expansions)
 };
 
 __device__ __forceinline__ void pp_force(double xi0, double xi1, double xi2, double px, double py,
@@ -1097,7 +1098,96 @@ __global__ __launch_bounds__(256) void k_group_runs(const NodeRec *__restrict__ 
     if (!WRITE && lane == 0) count_or_off[i] = at;
 }
 
-template <bool QUAD>
+// Moments of the far nodes' fields about the group centre c, for the
+// second-order local expansion.  A source of mass M at y = c - com
+// (s2 = |y|^2 + eps^2, q1 = M s2^-3/2, q2 = 3 M s2^-5/2, q3 = 15 M s2^-7/2)
+// gives the field a(c + d) = a0 + J d + (1/2) H[d, d] + O(|d|^3) with
+//   a0_k = -q1 y_k,  J_kl = q2 y_k y_l - q1 delta_kl,
+//   H_klm = q2 (delta_kl y_m + delta_km y_l + delta_lm y_k) - q3 y_k y_l y_m,
+// so the sums Q = sum q1, A = sum -q1 y, S = sum q2 y, P = sum q2 y y and
+// T = sum q3 y y y (symmetric: 6 and 10 components) determine it.
+struct FarMoments {
+    float Q, A0, A1, A2, S0, S1, S2;
+    float Pxx, Pyy, Pzz, Pxy, Pxz, Pyz;
+    float Txxx, Tyyy, Tzzz, Txxy, Txxz, Txyy, Tyyz, Txzz, Tyzz, Txyz;
+};
+
+__device__ __forceinline__ void far_add(FarMoments &F, float y0, float y1, float y2, float m,
+                                        float eps2)
+{
+    float s2 = __builtin_fmaf(y0, y0, eps2);
+    s2 = __builtin_fmaf(y1, y1, s2);
+    s2 = __builtin_fmaf(y2, y2, s2);
+    const float ri = __builtin_amdgcn_rsqf(s2);
+    const float ri2 = ri * ri;
+    const float q1 = (m * ri) * ri2;
+    const float q2 = 3.0f * q1 * ri2;
+    const float q3 = 5.0f * q2 * ri2;
+    F.Q += q1;
+    F.A0 = __builtin_fmaf(-q1, y0, F.A0);
+    F.A1 = __builtin_fmaf(-q1, y1, F.A1);
+    F.A2 = __builtin_fmaf(-q1, y2, F.A2);
+    const float u0 = q2 * y0, u1 = q2 * y1, u2 = q2 * y2;
+    F.S0 += u0;
+    F.S1 += u1;
+    F.S2 += u2;
+    F.Pxx = __builtin_fmaf(u0, y0, F.Pxx);
+    F.Pyy = __builtin_fmaf(u1, y1, F.Pyy);
+    F.Pzz = __builtin_fmaf(u2, y2, F.Pzz);
+    F.Pxy = __builtin_fmaf(u0, y1, F.Pxy);
+    F.Pxz = __builtin_fmaf(u0, y2, F.Pxz);
+    F.Pyz = __builtin_fmaf(u1, y2, F.Pyz);
+    const float v0 = q3 * y0, v1 = q3 * y1, v2 = q3 * y2;
+    const float v00 = v0 * y0, v01 = v0 * y1, v02 = v0 * y2, v11 = v1 * y1, v12 = v1 * y2,
+                v22 = v2 * y2;
+    F.Txxx = __builtin_fmaf(v00, y0, F.Txxx);
+    F.Txxy = __builtin_fmaf(v00, y1, F.Txxy);
+    F.Txxz = __builtin_fmaf(v00, y2, F.Txxz);
+    F.Txyy = __builtin_fmaf(v01, y1, F.Txyy);
+    F.Txyz = __builtin_fmaf(v01, y2, F.Txyz);
+    F.Txzz = __builtin_fmaf(v02, y2, F.Txzz);
+    F.Tyyy = __builtin_fmaf(v11, y1, F.Tyyy);
+    F.Tyyz = __builtin_fmaf(v11, y2, F.Tyyz);
+    F.Tyzz = __builtin_fmaf(v12, y2, F.Tyzz);
+    F.Tzzz = __builtin_fmaf(v22, y2, F.Tzzz);
+}
+
+__device__ __forceinline__ float wave_sumf(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Wave totals of the moments, then the expansion at offset d from the centre:
+//   a_k = a0_k + (P d)_k - Q d_k + d_k (S.d) + (1/2) S_k |d|^2 - (1/2) (T d d)_k
+__device__ __forceinline__ void far_eval(FarMoments F, float d0, float d1, float d2, float &o0,
+                                         float &o1, float &o2)
+{
+    F.Q = wave_sumf(F.Q);
+    F.A0 = wave_sumf(F.A0), F.A1 = wave_sumf(F.A1), F.A2 = wave_sumf(F.A2);
+    F.S0 = wave_sumf(F.S0), F.S1 = wave_sumf(F.S1), F.S2 = wave_sumf(F.S2);
+    F.Pxx = wave_sumf(F.Pxx), F.Pyy = wave_sumf(F.Pyy), F.Pzz = wave_sumf(F.Pzz);
+    F.Pxy = wave_sumf(F.Pxy), F.Pxz = wave_sumf(F.Pxz), F.Pyz = wave_sumf(F.Pyz);
+    F.Txxx = wave_sumf(F.Txxx), F.Tyyy = wave_sumf(F.Tyyy), F.Tzzz = wave_sumf(F.Tzzz);
+    F.Txxy = wave_sumf(F.Txxy), F.Txxz = wave_sumf(F.Txxz), F.Txyy = wave_sumf(F.Txyy);
+    F.Tyyz = wave_sumf(F.Tyyz), F.Txzz = wave_sumf(F.Txzz), F.Tyzz = wave_sumf(F.Tyzz);
+    F.Txyz = wave_sumf(F.Txyz);
+    const float Sd = __builtin_fmaf(F.S2, d2, __builtin_fmaf(F.S1, d1, F.S0 * d0));
+    const float dd = __builtin_fmaf(d2, d2, __builtin_fmaf(d1, d1, d0 * d0));
+    const float x2 = d0 * d0, y2 = d1 * d1, z2 = d2 * d2;
+    const float xy = 2.0f * d0 * d1, xz = 2.0f * d0 * d2, yz = 2.0f * d1 * d2;
+    const float T0 = F.Txxx * x2 + F.Txyy * y2 + F.Txzz * z2 + F.Txxy * xy + F.Txxz * xz + F.Txyz * yz;
+    const float T1 = F.Txxy * x2 + F.Tyyy * y2 + F.Tyzz * z2 + F.Txyy * xy + F.Txyz * xz + F.Tyyz * yz;
+    const float T2 = F.Txxz * x2 + F.Tyyz * y2 + F.Tzzz * z2 + F.Txyz * xy + F.Txzz * xz + F.Tyzz * yz;
+    const float P0 = F.Pxx * d0 + F.Pxy * d1 + F.Pxz * d2;
+    const float P1 = F.Pxy * d0 + F.Pyy * d1 + F.Pyz * d2;
+    const float P2 = F.Pxz * d0 + F.Pyz * d1 + F.Pzz * d2;
+    o0 = F.A0 + (P0 - F.Q * d0) + d0 * Sd + 0.5f * (F.S0 * dd - T0);
+    o1 = F.A1 + (P1 - F.Q * d1) + d1 * Sd + 0.5f * (F.S1 * dd - T1);
+    o2 = F.A2 + (P2 - F.Q * d2) + d2 * Sd + 0.5f * (F.S2 * dd - T2);
+}
+
+template <bool QUAD, bool LE>
 __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restrict__ nodes,
                                                          const double4 *__restrict__ P,
                                                          const int *__restrict__ sidx, int n,
@@ -1144,6 +1234,12 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     // from the particle records (to about 2^-17 of the separation otherwise).
     const float gw = static_cast<float>(fmax(fmax(hi0 - lo0, hi1 - lo1), hi2 - lo2));
     const float close2 = __builtin_fmaf(gw * 0x1p-7f, gw * 0x1p-7f, eps2);
+    // local expansions: an accepted node is far when rg < phi |com - c|, rg
+    // the half-diagonal of the group's box
+    const double hb0 = 0.5 * (hi0 - lo0), hb1 = 0.5 * (hi1 - lo1), hb2 = 0.5 * (hi2 - lo2);
+    const double rg2 = hb0 * hb0 + hb1 * hb1 + hb2 * hb2;
+    FarMoments F = {};
+    unsigned nfar = 0;
     double A0 = 0.0, A1 = 0.0, A2 = 0.0;
     unsigned cnt = 0;
     bool overflow = false;
@@ -1256,15 +1352,30 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
         __builtin_amdgcn_wave_barrier();
         const NodeRec nd = nodes[node];
         const bool leaf = nd.nchild == 0;
-        const bool single = act && leaf && nd.count == 1;
+        const bool single_leaf = act && leaf && nd.count == 1;
         const bool incl = nd.first < gend && nd.first + nd.count > g0;
         const double e0 = fmax(fmax(lo0 - nd.cx, nd.cx - hi0), 0.0);
         const double e1 = fmax(fmax(lo1 - nd.cy, nd.cy - hi1), 0.0);
         const double e2 = fmax(fmax(lo2 - nd.cz, nd.cz - hi2), 0.0);
         const double d2 = e0 * e0 + e1 * e1 + e2 * e2;
         const bool accept = !incl && nd.l2 < args.theta2 * d2;
-        const bool take_node = act && !single && accept;
-        const bool multi = act && leaf && !single && !accept;
+        bool far = false;
+        if (LE) {
+            // far accepted nodes (single-particle leaves too): their moments
+            // about the group centre, one node per lane
+            const double y0 = gc0 - nd.cx, y1 = gc1 - nd.cy, y2 = gc2 - nd.cz;
+            far = act && accept && rg2 < args.phi2 * (y0 * y0 + y1 * y1 + y2 * y2);
+            if (far)
+                far_add(F, static_cast<float>(y0), static_cast<float>(y1), static_cast<float>(y2),
+                        nd.fmass, eps2);
+            const unsigned nf = __popcll(__ballot(far));
+            nfar += nf;
+            cnt += nf;
+        }
+        // single-particle leaves not expanded: their particle into the particle list
+        const bool single = single_leaf && !far;
+        const bool take_node = act && !single_leaf && accept && !far;
+        const bool multi = act && leaf && !single_leaf && !accept;
         const bool open = act && !leaf && !accept;
         // accepted nodes -> node list
         {
@@ -1356,6 +1467,13 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     }
     if (nn > 0) sum_nodes(nn);
     if (np > 0) sum_parts(np);
+    if (LE && nfar > 0) {
+        float o0, o1, o2;
+        far_eval(F, xr, yr, zr, o0, o1, o2);
+        A0 += static_cast<double>(o0);
+        A1 += static_cast<double>(o1);
+        A2 += static_cast<double>(o2);
+    }
     if (valid) {
         const int i = sidx[s];
         a[3 * i + 0] = args.G * A0;
@@ -1364,7 +1482,10 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     }
     unsigned long long tot = valid ? cnt : 0u;
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    if (lane == 0 && n_int) atomicAdd(n_int, tot + (overflow ? (1ull << 62) : 0ull));
+    if (lane == 0 && n_int) {
+        atomicAdd(n_int, tot + (overflow ? (1ull << 62) : 0ull));
+        if (LE) atomicAdd(n_int + 1, static_cast<unsigned long long>(nfar));
+    }
 }
 
 // ------------------------------------------------------------------ host side
@@ -1698,6 +1819,7 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
     args.theta2 = theta * theta;
     args.eps2 = eps * eps;
     args.G = G;
+    args.phi2 = 0.0;
     void *p;
     GS_TRY(slot(c, SLOT_MISC, 256, &p));
     unsigned long long *cnt = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
@@ -1739,18 +1861,23 @@ static double group_width_cap(double width, int n)
 }
 
 ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, double theta,
-                            double *a, long long *interactions, int order)
+                            double *a, long long *interactions, int order, double phi,
+                            long long *far_pairs)
 {
     if (order == 2 && !t->has_quad)
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Quadrupole walk of a tree built without second moments");
+    if (phi > 0.0 && order != 1)
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Local expansions are built for the monopole group walk only");
     WalkArgs args;
     args.theta2 = theta * theta;
     args.eps2 = eps * eps;
     args.G = G;
+    args.phi2 = phi > 0.0 ? phi * phi : 0.0;
     void *p;
     GS_TRY(slot(c, SLOT_MISC, 256, &p));
+    // [0]: interactions (+ overflow flag), [1]: far (group, node) pairs
     unsigned long long *cnt = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
-    GS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream));
+    GS_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), c->stream));
     // groups: at most one per particle (they partition the particles)
     void *gp;
     GS_TRY(slot(c, SLOT_NODE_TMP, sizeof(int2) * static_cast<size_t>(t->n) + 256, &gp));
@@ -1794,20 +1921,25 @@ ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, dou
     GS_HIP(hipGetLastError());
     const int grid = (static_cast<int>(num_groups) + GW_WAVES - 1) / GW_WAVES;
     if (order >= 2)
-        hipLaunchKernelGGL((k_walk_group<true>), dim3(grid), dim3(GW_BLOCK), 0, c->stream, t->nodes,
-                           t->P, t->sidx, t->n, groups, static_cast<int>(num_groups), args, a, cnt,
-                           t->quadf);
+        hipLaunchKernelGGL((k_walk_group<true, false>), dim3(grid), dim3(GW_BLOCK), 0, c->stream,
+                           t->nodes, t->P, t->sidx, t->n, groups, static_cast<int>(num_groups), args,
+                           a, cnt, t->quadf);
+    else if (args.phi2 > 0.0)
+        hipLaunchKernelGGL((k_walk_group<false, true>), dim3(grid), dim3(GW_BLOCK), 0, c->stream,
+                           t->nodes, t->P, t->sidx, t->n, groups, static_cast<int>(num_groups), args,
+                           a, cnt, t->quadf);
     else
-        hipLaunchKernelGGL((k_walk_group<false>), dim3(grid), dim3(GW_BLOCK), 0, c->stream,
+        hipLaunchKernelGGL((k_walk_group<false, false>), dim3(grid), dim3(GW_BLOCK), 0, c->stream,
                            t->nodes, t->P, t->sidx, t->n, groups, static_cast<int>(num_groups), args,
                            a, cnt, t->quadf);
     timing_end(c, rec);
     GS_HIP(hipGetLastError());
-    unsigned long long h = 0;
-    GS_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long h[2] = {0, 0};
+    GS_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     GS_HIP(hipStreamSynchronize(c->stream));
-    if (h >> 62) return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group walk stack overflow");
-    if (interactions) *interactions = static_cast<long long>(h);
+    if (h[0] >> 62) return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group walk stack overflow");
+    if (interactions) *interactions = static_cast<long long>(h[0]);
+    if (far_pairs) *far_pairs = static_cast<long long>(h[1]);
     return make_success_error_status();
 }
 

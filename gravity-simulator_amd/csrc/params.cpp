@@ -28,7 +28,8 @@ static bool is_bh(int method)
            method == ACCELERATION_METHOD_BARNES_HUT_LEARNED ||
            method == ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE ||
            method == ACCELERATION_METHOD_BARNES_HUT_GROUP ||
-           method == ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE;
+           method == ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE ||
+           method == ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION;
 }
 
 static bool known_method(int method)
@@ -43,6 +44,7 @@ static bool known_method(int method)
         case ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE:
         case ACCELERATION_METHOD_BARNES_HUT_GROUP:
         case ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE:
+        case ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION:
             return true;
         default:
             return false;

@@ -24,6 +24,10 @@ ACCELERATION_METHOD_BARNES_HUT = 3
 ACCELERATION_METHOD_PAIRWISE_FP32 = 16
 ACCELERATION_METHOD_MASSLESS_FP32 = 17
 ACCELERATION_METHOD_BARNES_HUT_LEARNED = 18
+ACCELERATION_METHOD_BARNES_HUT_QUADRUPOLE = 19
+ACCELERATION_METHOD_BARNES_HUT_GROUP = 20
+ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE = 21
+ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION = 22
 
 INTEGRATOR_EULER = 1
 INTEGRATOR_EULER_CROMER = 2

@@ -12,9 +12,10 @@ Differences, all additive:
 
 * ``AccelerationParam.method`` also takes the library's GPU methods
   "pairwise_fp32", "massless_fp32" (FP32 tile sums with Kahan compensation),
-  "barnes_hut_learned", "barnes_hut_quadrupole" (IDs 16-19), "barnes_hut_group"
-  and "barnes_hut_group_quadrupole" (IDs 20, 21: one tree walk per group of
-  at most 64 particles).
+  "barnes_hut_learned", "barnes_hut_quadrupole" (IDs 16-19), "barnes_hut_group",
+  "barnes_hut_group_quadrupole" and "barnes_hut_group_expansion" (IDs 20-22:
+  one tree walk per group of at most 64 particles; 22 adds the far nodes
+  through a second-order expansion about the group's centre).
 * ``GravitySimulatorAPI.launch_ensemble`` runs many WHFast systems in one
   batched GPU call (``gs_whfast_ensemble``).
 * Plotting helpers (grav_sim/plotting.py) are not provided.
@@ -80,11 +81,12 @@ class AccelerationParam:
 
     AVAILABLE_ACCELERATION_METHODS = ["pairwise", "massless", "barnes_hut", "particle_mesh", "pairwise_fp32",
                                       "massless_fp32", "barnes_hut_learned", "barnes_hut_quadrupole",
-                                      "barnes_hut_group", "barnes_hut_group_quadrupole"]
+                                      "barnes_hut_group", "barnes_hut_group_quadrupole",
+                                      "barnes_hut_group_expansion"]
     ACCELERATION_METHODS_TO_ENCODING = {"pairwise": 1, "massless": 2, "barnes_hut": 3, "particle_mesh": 4,
                                         "pairwise_fp32": 16, "massless_fp32": 17, "barnes_hut_learned": 18,
                                         "barnes_hut_quadrupole": 19, "barnes_hut_group": 20,
-                                        "barnes_hut_group_quadrupole": 21}
+                                        "barnes_hut_group_quadrupole": 21, "barnes_hut_group_expansion": 22}
     ACCELERATION_ENCODING_TO_METHODS = {v: k for k, v in ACCELERATION_METHODS_TO_ENCODING.items()}
 
     def __init__(self):

@@ -37,6 +37,7 @@ EXPORTS = [
     "gs_sim_create", "gs_sim_destroy", "gs_sim_upload", "gs_sim_download", "gs_sim_acceleration",
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
+    "gs_last_walk_expansion_pairs", "gs_set_group_expansion_ratio", "gs_get_group_expansion_ratio",
     "gs_set_fp32_config", "gs_get_fp32_config", "gs_acceleration_noalloc", "gs_copy_calibrate", "gs_issue_rates",
     "gs_whfast_ensemble",
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
@@ -119,6 +120,9 @@ def load(path=None):
         "gs_sim_accel_error": (ES, [C.c_void_p, pP, C.c_int, PD, PD, PD]),
         "gs_octree_sort": (ES, [pS, PI64, PI, PD, PD]),
         "gs_last_walk_interactions": (ES, [C.POINTER(C.c_longlong)]),
+        "gs_last_walk_expansion_pairs": (ES, [C.POINTER(C.c_longlong)]),
+        "gs_set_group_expansion_ratio": (ES, [C.c_double]),
+        "gs_get_group_expansion_ratio": (ES, [C.POINTER(C.c_double)]),
         "gs_autotune_fp32": (ES, [C.c_int, PI, PI, PI]),
         "gs_set_fp32_config": (ES, [C.c_int, C.c_int, C.c_int]),
         "gs_get_fp32_config": (ES, [PI, PI, PI]),
@@ -256,6 +260,23 @@ def last_walk_interactions():
     n = C.c_longlong()
     check(load().gs_last_walk_interactions(C.byref(n)))
     return n.value
+
+
+def last_walk_expansion_pairs():
+    """(group, node) pairs of the last walk with local expansions (method 22)."""
+    n = C.c_longlong()
+    check(load().gs_last_walk_expansion_pairs(C.byref(n)))
+    return n.value
+
+
+def set_group_expansion_ratio(phi):
+    check(load().gs_set_group_expansion_ratio(float(phi)))
+
+
+def get_group_expansion_ratio():
+    v = C.c_double()
+    check(load().gs_get_group_expansion_ratio(C.byref(v)))
+    return v.value
 
 
 def device_info():

@@ -47,6 +47,16 @@ extern "C" {
  * relative to the group centre.  Monopoles (20) or with quadrupoles (21). */
 #define ACCELERATION_METHOD_BARNES_HUT_GROUP 20
 #define ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE 21
+/* Group walk (as 20) with local expansions: an accepted node whose centre of
+ * mass lies farther than r_g / phi from the group's box centre c (r_g the
+ * box's half-diagonal; phi from gs_set_group_expansion_ratio, 0.1 by
+ * default) is not summed per particle: its monopole field is expanded about
+ * c to second order (value, gradient and second derivatives; truncation
+ * error of order phi^3 of that node's term), the expansions of all such
+ * nodes are added once per group and evaluated at each particle.  The
+ * accepted set, and so the interaction count, is that of method 20 at the
+ * same theta. */
+#define ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION 22
 
 /* ---- device information --------------------------------------------------- */
 typedef struct GsDeviceInfo
@@ -153,6 +163,17 @@ ErrorStatus gs_octree_sort(const System *system, int64 *sorted_keys, int *sorted
 /* Number of interactions (accepted nodes + leaf particle pairs) of the last
  * Barnes-Hut walk. */
 ErrorStatus gs_last_walk_interactions(long long *count);
+
+/* Number of (group, node) pairs the last group walk with local expansions
+ * (ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION) took through the
+ * expansion; 0 after any other method.  No counterpart in the reference. */
+ErrorStatus gs_last_walk_expansion_pairs(long long *count);
+
+/* Expansion ratio phi of ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION
+ * (0 < phi < 1; default 0.1).  Process-wide.  No counterpart in the
+ * reference. */
+ErrorStatus gs_set_group_expansion_ratio(double phi);
+ErrorStatus gs_get_group_expansion_ratio(double *phi);
 
 /* Streams `bytes` through a plain device copy kernel `reps` times and writes
  * the achieved bandwidth (read + write bytes / time).  Its kernel ("copy")

@@ -478,7 +478,81 @@ typedef struct
     double (*acc)[3];
     int64_t *cnt;
     const int *idx;
+    /* local expansions (phi2 > 0): box centre, squared half-diagonal, the
+     * far nodes' moments and their number */
+    double phi2, c[3], rg2;
+    double *F;
+    int64_t *nfar;
 } GroupWalk;
+
+/* Second-order local expansion of far monopole fields about a centre c (no
+ * counterpart in the reference: the group walk with local expansions is this
+ * library's own method).  A source of mass M at y = c - com, s2 = |y|^2 +
+ * eps^2, has the Plummer-softened field a(c + d) = -M (y + d) / (|y + d|^2 +
+ * eps^2)^(3/2), whose Taylor terms about d = 0 are
+ *   a0_k = -q1 y_k,  J_kl = q2 y_k y_l - q1 delta_kl,
+ *   H_klm = q2 (delta_kl y_m + delta_km y_l + delta_lm y_k) - q3 y_k y_l y_m
+ * with q1 = M s2^-3/2, q2 = 3 M s2^-5/2, q3 = 15 M s2^-7/2.  F holds the sums
+ * over sources of Q = q1, A = -q1 y (3), S = q2 y (3), P = q2 y y (xx, yy,
+ * zz, xy, xz, yz) and T = q3 y y y (xxx, yyy, zzz, xxy, xxz, xyy, yyz, xzz,
+ * yzz, xyz): 23 numbers. */
+void orc_far_add(double *F, double y0, double y1, double y2, double M, double eps2)
+{
+    const double s2 = y0 * y0 + y1 * y1 + y2 * y2 + eps2;
+    const double s = sqrt(s2);
+    const double q1 = M / (s2 * s);
+    const double q2 = 3.0 * q1 / s2;
+    const double q3 = 5.0 * q2 / s2;
+    const double y[3] = {y0, y1, y2};
+    F[0] += q1;
+    for (int k = 0; k < 3; k++)
+    {
+        F[1 + k] -= q1 * y[k];
+        F[4 + k] += q2 * y[k];
+    }
+    static const int P2[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    for (int j = 0; j < 6; j++)
+        F[7 + j] += q2 * y[P2[j][0]] * y[P2[j][1]];
+    static const int T3[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
+                                  {0, 1, 1}, {1, 1, 2}, {0, 2, 2}, {1, 2, 2}, {0, 1, 2}};
+    for (int j = 0; j < 10; j++)
+        F[13 + j] += q3 * y[T3[j][0]] * y[T3[j][1]] * y[T3[j][2]];
+}
+
+/* a(c + d) = a0 + J d + (1/2) H[d, d] from the sums of orc_far_add, with the
+ * symmetric tensors expanded to full index form. */
+void orc_far_eval(const double *F, double d0, double d1, double d2, double out[3])
+{
+    const double d[3] = {d0, d1, d2};
+    double P[3][3], S[3], T[3][3][3];
+    static const int P2[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    static const int T3[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
+                                  {0, 1, 1}, {1, 1, 2}, {0, 2, 2}, {1, 2, 2}, {0, 1, 2}};
+    for (int j = 0; j < 6; j++)
+        P[P2[j][0]][P2[j][1]] = P[P2[j][1]][P2[j][0]] = F[7 + j];
+    for (int j = 0; j < 10; j++)
+    {
+        const int a = T3[j][0], b = T3[j][1], c = T3[j][2];
+        T[a][b][c] = T[a][c][b] = T[b][a][c] = T[b][c][a] = T[c][a][b] = T[c][b][a] = F[13 + j];
+    }
+    for (int k = 0; k < 3; k++)
+        S[k] = F[4 + k];
+    for (int k = 0; k < 3; k++)
+    {
+        double v = F[1 + k];
+        for (int l = 0; l < 3; l++)
+        {
+            const double J = P[k][l] - (k == l ? F[0] : 0.0);
+            v += J * d[l];
+            for (int m = 0; m < 3; m++)
+            {
+                const double H = S[m] * (k == l) + S[l] * (k == m) + S[k] * (l == m) - T[k][l][m];
+                v += 0.5 * H * d[l] * d[m];
+            }
+        }
+        out[k] = v;
+    }
+}
 
 static void group_node(const GroupWalk *g, int node)
 {
@@ -499,6 +573,19 @@ static void group_node(const GroupWalk *g, int node)
         const double l = (w->width * 2.0) / (double)(2LL << d);
         if ((l * l) < w->theta2 * dd2)
         {
+            if (g->phi2 > 0.0)
+            {
+                /* far: rg < phi |com - c| (the same FP64 expressions as the GPU's test) */
+                const double y0 = g->c[0] - com[0], y1 = g->c[1] - com[1], y2 = g->c[2] - com[2];
+                if (g->rg2 < g->phi2 * (y0 * y0 + y1 * y1 + y2 * y2))
+                {
+                    orc_far_add(g->F, y0, y1, y2, w->mass[node], w->eps2);
+                    (*g->nfar)++;
+                    for (int q = 0; q < gn; q++)
+                        g->cnt[q]++;
+                    return;
+                }
+            }
             for (int q = 0; q < gn; q++)
             {
                 const double R0 = g->xs[q][0] - com[0], R1 = g->xs[q][1] - com[1], R2 = g->xs[q][2] - com[2];
@@ -549,6 +636,20 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
                        const double *comx, const double *comy, const double *comz,
                        const double *quad, int group)
 {
+    orc_bh_walk_group_le(a, n_int, n, x, m, G, eps, theta, width, sorted_keys, sorted_idx, depth,
+                         prefix, count, first, nchild, child0, mass, comx, comy, comz, quad, group,
+                         0.0, NULL);
+}
+
+void orc_bh_walk_group_le(double *a, int64_t *n_int, int n, const double *x, const double *m,
+                          double G, double eps, double theta, double width,
+                          const int64_t *sorted_keys, const int32_t *sorted_idx, const int *depth,
+                          const int64_t *prefix, const int *count, const int *first,
+                          const int *nchild, const int *child0, const double *mass,
+                          const double *comx, const double *comy, const double *comz,
+                          const double *quad, int group, double phi, int64_t *far_pairs)
+{
+    int64_t far_total = 0;
     Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
               depth, count, first,  nchild,    child0,        prefix, mass,        comx,
               comy,  comz,  quad,   NULL,      0};
@@ -603,11 +704,16 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
         }
         free(stack);
     }
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : far_total)
     for (int gi = 0; gi < ng; gi++)
     {
         GroupWalk g;
+        double F[23] = {0.0};
+        int64_t nfar = 0;
         g.w = &w;
+        g.F = F;
+        g.nfar = &nfar;
+        g.phi2 = phi > 0.0 ? phi * phi : 0.0;
         g.s0 = gl[gi][0];
         g.s1 = gl[gi][0] + gl[gi][1];
         const int gn = g.s1 - g.s0;
@@ -629,8 +735,24 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
         g.xs = xs;
         g.acc = acc;
         g.cnt = cnt;
+        {
+            const double h0 = 0.5 * (g.hi[0] - g.lo[0]), h1 = 0.5 * (g.hi[1] - g.lo[1]),
+                         h2 = 0.5 * (g.hi[2] - g.lo[2]);
+            g.rg2 = h0 * h0 + h1 * h1 + h2 * h2;
+            for (int k = 0; k < 3; k++)
+                g.c[k] = 0.5 * (g.lo[k] + g.hi[k]);
+        }
         for (int c = child0[0]; c < child0[0] + nchild[0]; c++)
             group_node(&g, c);
+        if (nfar > 0)
+            for (int q = 0; q < gn; q++)
+            {
+                double e[3];
+                orc_far_eval(F, xs[q][0] - g.c[0], xs[q][1] - g.c[1], xs[q][2] - g.c[2], e);
+                for (int k = 0; k < 3; k++)
+                    acc[q][k] += w.G * e[k];
+            }
+        far_total += nfar;
         for (int q = 0; q < gn; q++)
         {
             const int i = sorted_idx[g.s0 + q];
@@ -644,6 +766,8 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
         free(cnt);
     }
     free(gl);
+    if (far_pairs)
+        *far_pairs = far_total;
 }
 
 void orc_octree_quad(int num, const double *x, const double *m, const int32_t *sorted_idx,

@@ -100,6 +100,25 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
                        const double *comx, const double *comy, const double *comz,
                        const double *quad, int group);
 
+/* orc_bh_walk_group with local expansions (phi > 0; monopoles only): an
+ * accepted node whose centre of mass lies farther than rg / phi from the
+ * group's box centre c (rg the box's half-diagonal) is not summed per
+ * particle: its field is expanded about c to second order (orc_far_add) and
+ * the sum of those expansions is evaluated at each particle (orc_far_eval).
+ * Such a node still counts as one interaction of each particle of the group;
+ * *far_pairs (may be NULL) gets the number of (group, node) pairs expanded. */
+void orc_bh_walk_group_le(double *a, int64_t *n_int, int n, const double *x, const double *m,
+                          double G, double eps, double theta, double width,
+                          const int64_t *sorted_keys, const int32_t *sorted_idx, const int *depth,
+                          const int64_t *prefix, const int *count, const int *first,
+                          const int *nchild, const int *child0, const double *mass,
+                          const double *comx, const double *comy, const double *comz,
+                          const double *quad, int group, double phi, int64_t *far_pairs);
+
+/* Moments of one source for the second-order local expansion (23 sums, see
+ * oracle.c) and the expansion's value at offset d from the centre. */
+void orc_far_add(double *F, double y0, double y1, double y2, double M, double eps2);
+void orc_far_eval(const double *F, double d0, double d1, double d2, double out[3]);
 /* Second central moments per node, Q_ab = sum m (x - com)_a (x - com)_b, in
  * the order xx yy zz xy xz yz: leaves over their particles, internal nodes
  * from their children by the parallel-axis shift.  An extension (the

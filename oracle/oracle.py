@@ -75,6 +75,10 @@ def lib():
                                   PD, PD, PD, PD, PD, PD, PD]
         L.orc_bh_walk_group.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI, PI,
                                         PD, PD, PD, PD, PD, i]
+        L.orc_bh_walk_group_le.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI,
+                                           PI, PD, PD, PD, PD, PD, i, d, PI64]
+        L.orc_far_add.argtypes = [PD, d, d, d, d, d]
+        L.orc_far_eval.argtypes = [PD, d, d, d, PD]
         L.orc_octree_quad.argtypes = [i, PD, PD, PI, PI, PI, PI, PI, PD, PD, PD, PD, PD]
         L.orc_octree_oct.argtypes = [i, PD, PD, PI, PI, PI, PI, PI, PD, PD, PD, PD, PD, PD]
         L.orc_leapfrog.argtypes = [PD, PD, i, PD, d, i, d, d, d]
@@ -261,23 +265,50 @@ def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, fea
     return out if len(out) > 1 else out[0]
 
 
-def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None):
+def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None, phi=0.0):
     """Group walk (groups of at most `group` consecutive sorted particles,
     formed from the tree as orc_bh_walk_group describes); returns (a,
-    per-particle interaction counts).  group=1 is bh_walk."""
+    per-particle interaction counts).  group=1 is bh_walk.
+
+    phi > 0 (quad must be None): local expansions of the far nodes
+    (orc_bh_walk_group_le); returns (a, counts, far (group, node) pairs)."""
     x, m = _c(x), _c(m)
     n = len(m)
     a = np.zeros_like(x)
     cnt = np.zeros(n, dtype=np.int64)
     t = tree
     qp = _abi.dptr(_c(quad)) if quad is not None else None
-    lib().orc_bh_walk_group(_abi.dptr(a), cnt.ctypes.data_as(PI64), n, _abi.dptr(x), _abi.dptr(m), G, eps, theta,
-                            t["width"], _c(t["sorted_keys"], np.int64).ctypes.data_as(PI64),
-                            _abi.iptr(_c(t["sorted_idx"], np.int32)), _abi.iptr(t["depth"]),
-                            t["prefix"].ctypes.data_as(PI64), _abi.iptr(t["count"]), _abi.iptr(t["first"]),
-                            _abi.iptr(t["nchild"]), _abi.iptr(t["child0"]), _abi.dptr(t["mass"]),
-                            _abi.dptr(t["comx"]), _abi.dptr(t["comy"]), _abi.dptr(t["comz"]), qp, int(group))
+    args = [_abi.dptr(a), cnt.ctypes.data_as(PI64), n, _abi.dptr(x), _abi.dptr(m), G, eps, theta,
+            t["width"], _c(t["sorted_keys"], np.int64).ctypes.data_as(PI64),
+            _abi.iptr(_c(t["sorted_idx"], np.int32)), _abi.iptr(t["depth"]),
+            t["prefix"].ctypes.data_as(PI64), _abi.iptr(t["count"]), _abi.iptr(t["first"]),
+            _abi.iptr(t["nchild"]), _abi.iptr(t["child0"]), _abi.dptr(t["mass"]),
+            _abi.dptr(t["comx"]), _abi.dptr(t["comy"]), _abi.dptr(t["comz"]), qp, int(group)]
+    if phi > 0.0:
+        if quad is not None:
+            raise ValueError("local expansions are restated for the monopole walk only")
+        far = C.c_int64()
+        lib().orc_bh_walk_group_le(*args, float(phi), C.byref(far))
+        return a, cnt, far.value
+    lib().orc_bh_walk_group(*args)
     return a, cnt
+
+
+def far_moments(y, M, eps=0.0):
+    """Sum of the 23 expansion moments (orc_far_add) of sources of masses M
+    at offsets y = c - com (k, 3)."""
+    F = np.zeros(23)
+    for (y0, y1, y2), mm in zip(np.atleast_2d(y), np.atleast_1d(M)):
+        lib().orc_far_add(_abi.dptr(F), float(y0), float(y1), float(y2), float(mm), float(eps) ** 2)
+    return F
+
+
+def far_eval(F, d):
+    """Second-order expansion from far_moments evaluated at offset d."""
+    out = np.zeros(3)
+    F = _c(F)
+    lib().orc_far_eval(_abi.dptr(F), float(d[0]), float(d[1]), float(d[2]), _abi.dptr(out))
+    return out
 
 
 def leapfrog(x, v, m, G, dt, tf, eps=0.0, method=1):

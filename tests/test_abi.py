@@ -230,3 +230,17 @@ def test_cosmological_system_layout_and_host_helpers():
         st = lib.finalize_cosmological_system(C.byref(t))
         assert st.return_code == _abi.GRAV_VALUE_ERROR, field
         lib.free_traceback(C.byref(st))
+
+
+def test_group_expansion_ratio_setter_checks_its_range():
+    """gs_set_group_expansion_ratio accepts 0 < phi < 1 only (host-side, no
+    device use); the default is 0.1."""
+    assert L.get_group_expansion_ratio() == 0.1
+    for bad in (0.0, 1.0, -0.5, float("nan")):
+        with pytest.raises(RuntimeError, match="Group expansion ratio"):
+            L.set_group_expansion_ratio(bad)
+    L.set_group_expansion_ratio(0.25)
+    try:
+        assert L.get_group_expansion_ratio() == 0.25
+    finally:
+        L.set_group_expansion_ratio(0.1)

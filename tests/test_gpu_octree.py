@@ -267,6 +267,70 @@ def test_group_walk_ragged_and_theta_zero(n, leaf):
         assert np.all(a_g == 0.0)
 
 
+# ------------------------------------------------ local expansions (method 22)
+@pytest.mark.parametrize("n,theta,eps,leaf,phi", [(8192, 0.5, 0.0, 1, 0.3), (8192, 0.8, 1e-3, 3, 0.5),
+                                                  (30000, 0.4, 1e-3, 1, 0.1)])
+def test_group_expansion_matches_oracle(n, theta, eps, leaf, phi):
+    """Method 22 takes the oracle's classification exactly (equal interaction
+    counts and equal numbers of expanded (group, node) pairs) and its forces
+    stay within FP32 rounding of the oracle's FP64 sums (the far moments are
+    summed in FP32 on the GPU)."""
+    print("group expansion case", n, theta, eps, leaf, phi, flush=True)
+    x, v, m = ic.two_spheres(n, 7)
+    t = O.build_tree(x, m, leaf)
+    a_o, cnt, far = O.bh_walk_group(x, m, t, theta, 64, eps=eps, phi=phi)
+    L.set_group_expansion_ratio(phi)
+    try:
+        a_g = L.acceleration(x, m, 22, eps=eps, theta=theta, leaf=leaf)
+    finally:
+        L.set_group_expansion_ratio(0.1)
+    assert far > 0
+    assert L.last_walk_interactions() == int(cnt.sum())
+    assert L.last_walk_expansion_pairs() == far
+    e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
+def test_group_expansion_ragged_and_clump():
+    """Method 22 on a coincident clump with a wide halo (groups of one
+    particle have a zero-size box: every accepted node beyond the softening
+    scale is then expanded) and on a partial group: the oracle's counts and
+    forces to FP32 rounding."""
+    x, v, m = ic.plummer(6000, 41)
+    rng = np.random.default_rng(41)
+    x[100:250] = x[100]
+    x[:40] = rng.uniform(-60.0, 60.0, size=(40, 3))
+    for xs, ms in ((x, m), (x[:63], m[:63])):
+        t = O.build_tree(xs, ms, 1)
+        a_o, cnt, far = O.bh_walk_group(xs, ms, t, 0.6, 64, eps=1e-3, phi=0.2)
+        L.set_group_expansion_ratio(0.2)
+        try:
+            a_g = L.acceleration(xs, ms, 22, eps=1e-3, theta=0.6, leaf=1)
+        finally:
+            L.set_group_expansion_ratio(0.1)
+        assert L.last_walk_interactions() == int(cnt.sum())
+        assert L.last_walk_expansion_pairs() == far
+        e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+        assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (len(ms), np.median(e), e.max())
+
+
+def test_group_expansion_error_matches_group_walk_n2e20():
+    """N=2^20 two spheres, theta = 0.4, phi = 0.1: against the FP64 direct
+    sum, the expansions leave the group walk's median and p99 errors within
+    5 % while taking most accepted nodes out of the per-particle sums."""
+    x, v, m = ic.two_spheres(1 << 20, 4)
+    sim = L.Sim(x, v, m, 1.0)
+    grp = sim.accel_error(L.param(20, theta=0.4, leaf=1, eps=1e-3), samples=4096)
+    n_grp = L.last_walk_interactions()
+    le = sim.accel_error(L.param(22, theta=0.4, leaf=1, eps=1e-3), samples=4096)
+    assert L.last_walk_interactions() == n_grp
+    far = L.last_walk_expansion_pairs()
+    print(f"N=2^20 theta=0.4: group p99 {grp[1]:.3e}, with expansions {le[1]:.3e}; {far} expanded pairs")
+    assert le[0] <= 1.05 * grp[0] and le[1] <= 1.05 * grp[1]
+    assert far > 0
+    sim.close()
+
+
 def test_group_walk_error_below_per_particle_walk_config4():
     """N=2^20 two-sphere at theta=0.5: the group walk's errors against the FP64
     direct sum are below the per-particle walk's (it opens more nodes)."""

@@ -9,6 +9,14 @@ cases that reduce to what is pinned elsewhere:
   it, so no particle has fewer interactions than in the per-particle walk --
   also for the particles of an outlier cloud, whose wide runs are split into
   groups of one particle.
+
+Its local expansions (orc_bh_walk_group_le, orc_far_add / orc_far_eval) are
+pinned by the Taylor property itself -- the second-order expansion of a sum
+of softened point-mass fields differs from the exact field by O(|d|^3), so a
+wrong term of order 0, 1 or 2 would leave an error of order |d|^0..2 --, by
+the limit phi -> 0 (no node expanded: the plain group walk bit for bit) and
+by the accuracy against the FP64 direct sum, which an expansion error would
+raise.
 """
 import numpy as np
 import pytest
@@ -56,3 +64,60 @@ def test_group_interactions_never_below_the_per_particle_walk(theta):
     a1 = O.bh_walk(x, m, t, theta, eps=1e-3)
     err1 = np.linalg.norm(a1 - ad, axis=1) / np.linalg.norm(ad, axis=1)
     assert np.percentile(err, 99) <= np.percentile(err1, 99)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.3])
+def test_far_expansion_error_is_third_order(eps):
+    """Sum of 7 softened sources expanded about the origin: halving the
+    offset divides the error by 8 (7.5-8.5 at these offsets), in every
+    direction; the value at d = 0 is the exact field."""
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=(7, 3))
+    y = 4.0 * y / np.linalg.norm(y, axis=1)[:, None]  # sources 4 away: c - com = y
+    M = rng.uniform(0.5, 2.0, size=7)
+    F = O.far_moments(y, M, eps)
+
+    def exact(d):
+        r = y + d  # evaluation point minus source position
+        s2 = np.einsum("ij,ij->i", r, r) + eps * eps
+        return -(M / s2 ** 1.5) @ r
+
+    assert np.allclose(O.far_eval(F, np.zeros(3)), exact(np.zeros(3)), rtol=1e-14, atol=0)
+    for _ in range(6):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        e = [np.linalg.norm(O.far_eval(F, h * u) - exact(h * u)) for h in (0.2, 0.1, 0.05)]
+        assert 7.5 < e[0] / e[1] < 8.5 and 7.5 < e[1] / e[2] < 8.5, e
+
+
+def test_expansion_ratio_to_zero_is_the_group_walk():
+    """phi -> 0 expands nothing in groups of nonzero width (the plain group
+    walk bit for bit); a group of one particle has a box of zero width, so
+    every node it accepts is expanded, exactly (d = 0: the field itself)."""
+    x, m = _cloud(6000, 4)
+    t = O.build_tree(x, m, 1)
+    a0, c0 = O.bh_walk_group(x, m, t, 0.6, 64, eps=1e-3)
+    a1, c1, far = O.bh_walk_group(x, m, t, 0.6, 64, eps=1e-3, phi=1e-12)
+    assert np.array_equal(c1, c0)
+    # only the particles of singleton groups (split outlier runs) differ
+    differ = np.any(a1 != a0, axis=1)
+    assert 0 < differ.sum() <= 60 and far > 0
+    assert np.abs(a1 - a0).max() <= 1e-13 * np.abs(a0).max()
+
+
+def test_expansions_keep_the_group_walk_accuracy():
+    """Two spheres, N = 2^14, theta = 0.5: with phi = 0.1 a good share of
+    the accepted nodes goes through the expansions, the interaction counts
+    are the group walk's, and the errors against the FP64 direct sum stay
+    within 3 % of it."""
+    x, _, m = ic.two_spheres(1 << 14, 5)
+    t = O.build_tree(x, m, 1)
+    a_d = O.pairwise(x, m, eps=1e-3)
+    a0, c0 = O.bh_walk_group(x, m, t, 0.5, 64, eps=1e-3)
+    a1, c1, far = O.bh_walk_group(x, m, t, 0.5, 64, eps=1e-3, phi=0.1)
+    assert np.array_equal(c1, c0)
+    # (group, node) pairs against interactions / 64 (~ pairs of full groups)
+    assert far > 0.1 * c0.sum() / 64.0
+    e0 = np.linalg.norm(a0 - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
+    e1 = np.linalg.norm(a1 - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
+    assert np.median(e1) <= 1.03 * np.median(e0) and np.quantile(e1, 0.99) <= 1.03 * np.quantile(e0, 0.99)

@@ -125,7 +125,7 @@ def test_parameter_classes_match_reference_encodings():
         assert i._integrator == code and i.integrator == name
     for name, code in (("pairwise", 1), ("massless", 2), ("barnes_hut", 3), ("pairwise_fp32", 16),
                        ("barnes_hut_quadrupole", 19), ("barnes_hut_group", 20),
-                       ("barnes_hut_group_quadrupole", 21)):
+                       ("barnes_hut_group_quadrupole", 21), ("barnes_hut_group_expansion", 22)):
         a.method = f"  {name} "
         assert a._method == code
     o.coordinate_output_dtype = "float"
