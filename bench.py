@@ -23,10 +23,12 @@ Printed as one JSON line:
              n2e20  N = 2^20 (seed 4), theta = 0.5, monopole (the reference's
                     method): particles/s of sort + build + walk
              n2e21  N = 2^21 (seed 5): "plain" = monopole theta = 0.5,
-                    and the group walk at theta = 0.5; candidates meeting
-                    p99 <= 1e-3 (monopole, quadrupole, group walk and group
-                    walk with quadrupoles at the largest theta that does, the
-                    learned correction trained on 10^6 samples of seed 7);
+                    and the group walk at theta = 0.5 without and with local
+                    expansions; candidates meeting p99 <= 1e-3 (monopole,
+                    quadrupole, group walk, group walk with quadrupoles and
+                    group walk with local expansions at the largest theta
+                    that does, the learned correction trained on 10^6
+                    samples of seed 7);
                     "chosen" = the fastest of those; the learned correction
                     at theta = 0.8 (training times, calibration on seed 8,
                     against the quadrupole walk at 0.8, with its spot check)
@@ -193,10 +195,13 @@ def bh_case(L, sim, n, method, theta, reps=5, label=None):
     nint = L.last_walk_interactions()
     med, p99, mx = sim.accel_error(p, samples=4096)
     r = {"method": label or {3: "monopole", 19: "quadrupole", 18: "learned", 20: "group",
-                                 21: "group+quadrupole"}[method], "theta": theta,
+                                 21: "group+quadrupole", 22: "group+expansions"}[method], "theta": theta,
          "particles_per_s": n / wall, "ms_per_eval": wall * 1e3, "interactions": nint,
          "median_rel": med, "p99_rel": p99, "max_rel": mx}
     r.update({k: round(v, 4) for k, v in parts.items()})
+    if method == 22:
+        r["expansion_ratio"] = L.get_group_expansion_ratio()
+        r["expanded_pairs"] = L.last_walk_expansion_pairs()
     if "bh_walk_ms" in parts:
         r["walk_tflops"] = FLOPS_PER_INTERACTION * nint / (parts["bh_walk_ms"] / 1e3) / 1e12
     return r
@@ -222,8 +227,11 @@ def run_bh(L, ic, learned=True):
                        "trained on 10^6 samples of seed 7", "p99_target": P99_TARGET}
     res["plain"] = bh_case(L, sim, n5, 3, 0.5)
     res["group"] = bh_case(L, sim, n5, 20, 0.5)
+    res["group_expansion"] = bh_case(L, sim, n5, 22, 0.5)
     log(f"BH 2^21 plain theta 0.5: p99 {res['plain']['p99_rel']:.2e}; group walk "
-        f"{res['group']['particles_per_s'] / res['plain']['particles_per_s']:.2f}x, p99 {res['group']['p99_rel']:.2e}")
+        f"{res['group']['particles_per_s'] / res['plain']['particles_per_s']:.2f}x, p99 {res['group']['p99_rel']:.2e}; "
+        f"with expansions {res['group_expansion']['particles_per_s'] / res['plain']['particles_per_s']:.2f}x, "
+        f"p99 {res['group_expansion']['p99_rel']:.2e}")
     cands = []
     for th in (0.45, 0.4, 0.35, 0.3, 0.25, 0.2):
         r = bh_case(L, sim, n5, 3, th, reps=3)
@@ -250,7 +258,13 @@ def run_bh(L, ic, learned=True):
             res["group_quadrupole_at_target"] = r
             cands.append(r)
             break
-    log("BH 2^21 monopole / quadrupole / group at target done")
+    for th in (0.44, 0.42, 0.41, 0.4, 0.38, 0.35, 0.3, 0.25):
+        r = bh_case(L, sim, n5, 22, th, reps=3)
+        if r["p99_rel"] <= P99_TARGET:
+            res["group_expansion_at_target"] = r
+            cands.append(r)
+            break
+    log("BH 2^21 monopole / quadrupole / group / group+expansions at target done")
 
     if not learned:
         sim.close()
