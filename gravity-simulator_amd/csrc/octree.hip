@@ -1384,7 +1384,7 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
                 const int at = nn + __popcll(B & ((1ull << lane) - 1));
                 nl[at] = make_float4(static_cast<float>(nd.cx - gc0), static_cast<float>(nd.cy - gc1),
                                      static_cast<float>(nd.cz - gc2), nd.fmass);
-                ni[at] = node;
+                if (QUAD) ni[at] = node;
             }
             nn += __popcll(B);
         }
@@ -1400,17 +1400,21 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
             }
             np += __popcll(B);
         }
-        // opened internal nodes -> children on the stack
+        // opened internal nodes -> children on the stack, in lane order: the
+        // exclusive prefix of the child counts (0..8) from four bit-plane
+        // ballots, without a cross-lane shuffle chain
         {
             const int c = open ? nd.nchild : 0;
-            int inc = c;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(inc, o);
-                if (lane >= o) inc += t;
+            const unsigned long long below = (1ull << lane) - 1;
+            int before = 0, total = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const unsigned long long B = __ballot((c >> b) & 1);
+                before += __popcll(B & below) << b;
+                total += __popcll(B) << b;
             }
-            const int total = __shfl(inc, 63);
             if (top + total <= GW_STACK) {
-                for (int q = 0; q < c; q++) stk[top + inc - c + q] = nd.child0 + q;
+                for (int q = 0; q < c; q++) stk[top + before + q] = nd.child0 + q;
                 top += total;
             } else {
                 overflow = true;
@@ -1422,7 +1426,7 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
         // particle list, 64 slots at a time, every lane loading one particle
         // (slot t belongs to the first multi leaf whose inclusive count
         // prefix exceeds t)
-        {
+        if (__ballot(multi)) {
             const int c = multi ? nd.count : 0;
             int inc = c;
             for (int o = 1; o < 64; o <<= 1) {
@@ -1459,7 +1463,7 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
             __builtin_amdgcn_wave_barrier();
             if (lane < nn - 64) {
                 nl[lane] = nl[64 + lane];
-                ni[lane] = ni[64 + lane];
+                if (QUAD) ni[lane] = ni[64 + lane];
             }
             nn -= 64;
             __builtin_amdgcn_wave_barrier();
