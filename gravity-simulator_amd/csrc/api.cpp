@@ -20,6 +20,7 @@ ErrorStatus learned_acceleration(Context *c, const double *x, const double *m, i
 static long long g_last_walk_interactions = 0;
 static long long g_last_walk_far_pairs = 0;
 static double g_group_expansion_ratio = 0.1;
+static double g_group_expansion_angle_factor = 1.25;
 
 ErrorStatus device_acceleration(Context *c, const double *x, const double *m, int n, double G,
                                 const AccelerationParam *p, double *a)
@@ -62,9 +63,11 @@ ErrorStatus device_acceleration(Context *c, const double *x, const double *m, in
         }
         case ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION: {
             DeviceTree *t;
-            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, false, &t));
+            GS_TRY(tree_build(c, x, m, n, p->max_num_particles_per_leaf, nullptr, -1.0, true, &t));
             GS_TRY(tree_walk_group(c, t, G, eps, p->opening_angle, a, &g_last_walk_interactions, 1,
-                                   g_group_expansion_ratio, &g_last_walk_far_pairs));
+                                   g_group_expansion_ratio,
+                                   g_group_expansion_angle_factor * p->opening_angle,
+                                   &g_last_walk_far_pairs));
             return make_success_error_status();
         }
         case ACCELERATION_METHOD_BARNES_HUT_LEARNED:
@@ -751,6 +754,22 @@ ErrorStatus gs_get_group_expansion_ratio(double *phi)
 {
     if (!phi) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "phi is NULL");
     *phi = g_group_expansion_ratio;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_set_group_expansion_angle_factor(double k)
+{
+    if (!(k >= 1.0 && k <= 4.0))
+        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR,
+                                    "Group expansion angle factor must be in [1, 4]. Got: %g", k);
+    g_group_expansion_angle_factor = k;
+    return make_success_error_status();
+}
+
+ErrorStatus gs_get_group_expansion_angle_factor(double *k)
+{
+    if (!k) return WRAP_RAISE_ERROR(GRAV_POINTER_ERROR, "k is NULL");
+    *k = g_group_expansion_angle_factor;
     return make_success_error_status();
 }
 

@@ -182,13 +182,16 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
                       float *feat = nullptr);
 // Group walk (one wavefront per 64 sorted particles, group opening test);
 // order 1: monopoles, 2: with quadrupole terms (tree built with_quad).
-// phi > 0 (monopole walk only): accepted nodes with rg < phi |com - c| (rg the
-// half-diagonal of the group's box, c its centre) enter a second-order local
-// expansion about c instead of the per-particle node terms; *far_pairs gets
-// the number of such (group, node) pairs.
+// phi > 0 (monopole walk only): nodes with rg < phi |com - c| (rg the
+// half-diagonal of the group's box, c its centre) are far; they are accepted
+// at the opening angle theta_far (>= theta) and enter a second-order local
+// expansion of their monopole fields about c, plus the value of their
+// quadrupole fields at c (the tree must carry second moments), instead of the
+// per-particle node terms; *far_pairs gets the number of such (group, node)
+// pairs.
 ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, double theta,
                             double *a, long long *interactions, int order, double phi = 0.0,
-                            long long *far_pairs = nullptr);
+                            double theta_far = 0.0, long long *far_pairs = nullptr);
 // Host copy of the tree in the reference struct layout.
 ErrorStatus tree_to_host(Context *c, DeviceTree *t, LinearOctree *out);
 

@@ -703,7 +703,8 @@ struct WalkArgs {
     double theta2;
     double eps2;
     double G;
-    double phi2;  // group walk: squared expansion ratio (0: no local expansions)
+    double phi2;     // group walk: squared expansion ratio (0: no local expansions)
+    double thetaf2;  // squared opening angle for far nodes (expanded ones)
 };
 
 __device__ __forceinline__ void pp_force(double xi0, double xi1, double xi2, double px, double py,
@@ -1112,8 +1113,10 @@ struct FarMoments {
     float Txxx, Tyyy, Tzzz, Txxy, Txxz, Txyy, Tyyz, Txzz, Tyzz, Txyz;
 };
 
+// One far node: its monopole moments, and the value of its quadrupole field
+// at the centre (R = c - com = y) added to a0.
 __device__ __forceinline__ void far_add(FarMoments &F, float y0, float y1, float y2, float m,
-                                        float eps2)
+                                        float eps2, const float *__restrict__ quad)
 {
     float s2 = __builtin_fmaf(y0, y0, eps2);
     s2 = __builtin_fmaf(y1, y1, s2);
@@ -1123,10 +1126,12 @@ __device__ __forceinline__ void far_add(FarMoments &F, float y0, float y1, float
     const float q1 = (m * ri) * ri2;
     const float q2 = 3.0f * q1 * ri2;
     const float q3 = 5.0f * q2 * ri2;
+    float o0, o1, o2;
+    quad_termf(quad, y0, y1, y2, ri, o0, o1, o2);
     F.Q += q1;
-    F.A0 = __builtin_fmaf(-q1, y0, F.A0);
-    F.A1 = __builtin_fmaf(-q1, y1, F.A1);
-    F.A2 = __builtin_fmaf(-q1, y2, F.A2);
+    F.A0 = __builtin_fmaf(-q1, y0, F.A0) + o0;
+    F.A1 = __builtin_fmaf(-q1, y1, F.A1) + o1;
+    F.A2 = __builtin_fmaf(-q1, y2, F.A2) + o2;
     const float u0 = q2 * y0, u1 = q2 * y1, u2 = q2 * y2;
     F.S0 += u0;
     F.S1 += u1;
@@ -1358,16 +1363,19 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
         const double e1 = fmax(fmax(lo1 - nd.cy, nd.cy - hi1), 0.0);
         const double e2 = fmax(fmax(lo2 - nd.cz, nd.cz - hi2), 0.0);
         const double d2 = e0 * e0 + e1 * e1 + e2 * e2;
-        const bool accept = !incl && nd.l2 < args.theta2 * d2;
+        bool accept = !incl && nd.l2 < args.theta2 * d2;
         bool far = false;
         if (LE) {
-            // far accepted nodes (single-particle leaves too): their moments
-            // about the group centre, one node per lane
+            // far nodes (single-particle leaves too), accepted at the far
+            // opening angle: their moments about the group centre, one node
+            // per lane
             const double y0 = gc0 - nd.cx, y1 = gc1 - nd.cy, y2 = gc2 - nd.cz;
-            far = act && accept && rg2 < args.phi2 * (y0 * y0 + y1 * y1 + y2 * y2);
+            const bool farz = rg2 < args.phi2 * (y0 * y0 + y1 * y1 + y2 * y2);
+            accept = accept || (!incl && farz && nd.l2 < args.thetaf2 * d2);
+            far = act && accept && farz;
             if (far)
                 far_add(F, static_cast<float>(y0), static_cast<float>(y1), static_cast<float>(y2),
-                        nd.fmass, eps2);
+                        nd.fmass, eps2, quad + 8 * static_cast<size_t>(node));
             const unsigned nf = __popcll(__ballot(far));
             nfar += nf;
             cnt += nf;
@@ -1824,6 +1832,7 @@ ErrorStatus tree_walk(Context *c, DeviceTree *t, const double *x, double G, doub
     args.eps2 = eps * eps;
     args.G = G;
     args.phi2 = 0.0;
+    args.thetaf2 = args.theta2;
     void *p;
     GS_TRY(slot(c, SLOT_MISC, 256, &p));
     unsigned long long *cnt = reinterpret_cast<unsigned long long *>(static_cast<char *>(p) + 64);
@@ -1866,17 +1875,20 @@ static double group_width_cap(double width, int n)
 
 ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, double theta,
                             double *a, long long *interactions, int order, double phi,
-                            long long *far_pairs)
+                            double theta_far, long long *far_pairs)
 {
     if (order == 2 && !t->has_quad)
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Quadrupole walk of a tree built without second moments");
     if (phi > 0.0 && order != 1)
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Local expansions are built for the monopole group walk only");
+    if (phi > 0.0 && !t->has_quad)
+        return WRAP_RAISE_ERROR(GRAV_FAILURE, "Local expansions need a tree with second moments");
     WalkArgs args;
     args.theta2 = theta * theta;
     args.eps2 = eps * eps;
     args.G = G;
     args.phi2 = phi > 0.0 ? phi * phi : 0.0;
+    args.thetaf2 = theta_far > theta ? theta_far * theta_far : args.theta2;
     void *p;
     GS_TRY(slot(c, SLOT_MISC, 256, &p));
     // [0]: interactions (+ overflow flag), [1]: far (group, node) pairs

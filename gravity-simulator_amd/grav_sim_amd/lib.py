@@ -38,6 +38,7 @@ EXPORTS = [
     "gs_sim_leapfrog_begin", "gs_sim_leapfrog_steps", "gs_sim_leapfrog_end", "gs_sim_energy",
     "gs_sim_accel_error", "gs_octree_sort", "gs_last_walk_interactions", "gs_autotune_fp32",
     "gs_last_walk_expansion_pairs", "gs_set_group_expansion_ratio", "gs_get_group_expansion_ratio",
+    "gs_set_group_expansion_angle_factor", "gs_get_group_expansion_angle_factor",
     "gs_set_fp32_config", "gs_get_fp32_config", "gs_acceleration_noalloc", "gs_copy_calibrate", "gs_issue_rates",
     "gs_whfast_ensemble",
     "gs_learned_num_params", "gs_learned_train", "gs_learned_get_params", "gs_learned_set_params",
@@ -123,6 +124,8 @@ def load(path=None):
         "gs_last_walk_expansion_pairs": (ES, [C.POINTER(C.c_longlong)]),
         "gs_set_group_expansion_ratio": (ES, [C.c_double]),
         "gs_get_group_expansion_ratio": (ES, [C.POINTER(C.c_double)]),
+        "gs_set_group_expansion_angle_factor": (ES, [C.c_double]),
+        "gs_get_group_expansion_angle_factor": (ES, [C.POINTER(C.c_double)]),
         "gs_autotune_fp32": (ES, [C.c_int, PI, PI, PI]),
         "gs_set_fp32_config": (ES, [C.c_int, C.c_int, C.c_int]),
         "gs_get_fp32_config": (ES, [PI, PI, PI]),
@@ -276,6 +279,16 @@ def set_group_expansion_ratio(phi):
 def get_group_expansion_ratio():
     v = C.c_double()
     check(load().gs_get_group_expansion_ratio(C.byref(v)))
+    return v.value
+
+
+def set_group_expansion_angle_factor(k):
+    check(load().gs_set_group_expansion_angle_factor(float(k)))
+
+
+def get_group_expansion_angle_factor():
+    v = C.c_double()
+    check(load().gs_get_group_expansion_angle_factor(C.byref(v)))
     return v.value
 
 

@@ -47,15 +47,18 @@ extern "C" {
  * relative to the group centre.  Monopoles (20) or with quadrupoles (21). */
 #define ACCELERATION_METHOD_BARNES_HUT_GROUP 20
 #define ACCELERATION_METHOD_BARNES_HUT_GROUP_QUADRUPOLE 21
-/* Group walk (as 20) with local expansions: an accepted node whose centre of
- * mass lies farther than r_g / phi from the group's box centre c (r_g the
- * box's half-diagonal; phi from gs_set_group_expansion_ratio, 0.1 by
- * default) is not summed per particle: its monopole field is expanded about
- * c to second order (value, gradient and second derivatives; truncation
- * error of order phi^3 of that node's term), the expansions of all such
- * nodes are added once per group and evaluated at each particle.  The
- * accepted set, and so the interaction count, is that of method 20 at the
- * same theta. */
+/* Group walk (as 20) with local expansions: a node whose centre of mass lies
+ * farther than r_g / phi from the group's box centre c (r_g the box's
+ * half-diagonal; phi from gs_set_group_expansion_ratio, 0.1 by default) is
+ * far.  Far nodes are accepted at the opening angle k theta (k from
+ * gs_set_group_expansion_angle_factor, 1.25 by default; near nodes at theta,
+ * as in method 20) and are not summed per particle: the monopole field of
+ * each is expanded about c to second order (value, gradient and second
+ * derivatives; truncation error of order phi^3 of that node's term) and the
+ * value of its quadrupole field at c is added (the tree carries second
+ * moments), the expansions of all far nodes are added once per group and
+ * evaluated at each particle.  With k = 1 the accepted set, and so the
+ * interaction count, is that of method 20 at the same theta. */
 #define ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION 22
 
 /* ---- device information --------------------------------------------------- */
@@ -174,6 +177,12 @@ ErrorStatus gs_last_walk_expansion_pairs(long long *count);
  * reference. */
 ErrorStatus gs_set_group_expansion_ratio(double phi);
 ErrorStatus gs_get_group_expansion_ratio(double *phi);
+
+/* Opening-angle factor k of the far nodes of
+ * ACCELERATION_METHOD_BARNES_HUT_GROUP_EXPANSION (1 <= k <= 4; default 1.25).
+ * Process-wide.  No counterpart in the reference. */
+ErrorStatus gs_set_group_expansion_angle_factor(double k);
+ErrorStatus gs_get_group_expansion_angle_factor(double *k);
 
 /* Streams `bytes` through a plain device copy kernel `reps` times and writes
  * the achieved bandwidth (read + write bytes / time).  Its kernel ("copy")

@@ -479,8 +479,10 @@ typedef struct
     int64_t *cnt;
     const int *idx;
     /* local expansions (phi2 > 0): box centre, squared half-diagonal, the
-     * far nodes' moments and their number */
-    double phi2, c[3], rg2;
+     * far nodes' squared opening angle and quadrupole moments, their moments
+     * and their number */
+    double phi2, c[3], rg2, thf2;
+    const double *quad_far;
     double *F;
     int64_t *nfar;
 } GroupWalk;
@@ -571,20 +573,22 @@ static void group_node(const GroupWalk *g, int node)
             dd2 += e * e;
         }
         const double l = (w->width * 2.0) / (double)(2LL << d);
-        if ((l * l) < w->theta2 * dd2)
+        /* far: rg < phi |com - c| (the same FP64 expressions as the GPU's
+         * test), accepted at the far opening angle */
+        const double y0 = g->c[0] - com[0], y1 = g->c[1] - com[1], y2 = g->c[2] - com[2];
+        const int far = g->phi2 > 0.0 && g->rg2 < g->phi2 * (y0 * y0 + y1 * y1 + y2 * y2);
+        if ((l * l) < w->theta2 * dd2 || (far && (l * l) < g->thf2 * dd2))
         {
-            if (g->phi2 > 0.0)
+            if (far)
             {
-                /* far: rg < phi |com - c| (the same FP64 expressions as the GPU's test) */
-                const double y0 = g->c[0] - com[0], y1 = g->c[1] - com[1], y2 = g->c[2] - com[2];
-                if (g->rg2 < g->phi2 * (y0 * y0 + y1 * y1 + y2 * y2))
-                {
-                    orc_far_add(g->F, y0, y1, y2, w->mass[node], w->eps2);
-                    (*g->nfar)++;
-                    for (int q = 0; q < gn; q++)
-                        g->cnt[q]++;
-                    return;
-                }
+                orc_far_add(g->F, y0, y1, y2, w->mass[node], w->eps2);
+                if (g->quad_far)
+                    quad_accel(&g->F[1], &g->quad_far[6 * node], y0, y1, y2,
+                               y0 * y0 + y1 * y1 + y2 * y2 + w->eps2, 1.0);
+                (*g->nfar)++;
+                for (int q = 0; q < gn; q++)
+                    g->cnt[q]++;
+                return;
             }
             for (int q = 0; q < gn; q++)
             {
@@ -638,7 +642,7 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
 {
     orc_bh_walk_group_le(a, n_int, n, x, m, G, eps, theta, width, sorted_keys, sorted_idx, depth,
                          prefix, count, first, nchild, child0, mass, comx, comy, comz, quad, group,
-                         0.0, NULL);
+                         0.0, 0.0, NULL, NULL);
 }
 
 void orc_bh_walk_group_le(double *a, int64_t *n_int, int n, const double *x, const double *m,
@@ -647,9 +651,11 @@ void orc_bh_walk_group_le(double *a, int64_t *n_int, int n, const double *x, con
                           const int64_t *prefix, const int *count, const int *first,
                           const int *nchild, const int *child0, const double *mass,
                           const double *comx, const double *comy, const double *comz,
-                          const double *quad, int group, double phi, int64_t *far_pairs)
+                          const double *quad, int group, double phi, double theta_far,
+                          const double *quad_far, int64_t *far_pairs)
 {
     int64_t far_total = 0;
+    const double thf2 = theta_far > theta ? theta_far * theta_far : theta * theta;
     Walk w = {x,     m,     G,      eps * eps, theta * theta, width,  sorted_keys, sorted_idx,
               depth, count, first,  nchild,    child0,        prefix, mass,        comx,
               comy,  comz,  quad,   NULL,      0};
@@ -714,6 +720,8 @@ void orc_bh_walk_group_le(double *a, int64_t *n_int, int n, const double *x, con
         g.F = F;
         g.nfar = &nfar;
         g.phi2 = phi > 0.0 ? phi * phi : 0.0;
+        g.thf2 = thf2;
+        g.quad_far = quad_far;
         g.s0 = gl[gi][0];
         g.s1 = gl[gi][0] + gl[gi][1];
         const int gn = g.s1 - g.s0;

@@ -100,20 +100,26 @@ void orc_bh_walk_group(double *a, int64_t *n_int, int n, const double *x, const 
                        const double *comx, const double *comy, const double *comz,
                        const double *quad, int group);
 
-/* orc_bh_walk_group with local expansions (phi > 0; monopoles only): an
- * accepted node whose centre of mass lies farther than rg / phi from the
- * group's box centre c (rg the box's half-diagonal) is not summed per
- * particle: its field is expanded about c to second order (orc_far_add) and
- * the sum of those expansions is evaluated at each particle (orc_far_eval).
- * Such a node still counts as one interaction of each particle of the group;
- * *far_pairs (may be NULL) gets the number of (group, node) pairs expanded. */
+/* orc_bh_walk_group with local expansions (phi > 0; near nodes monopoles
+ * only, quad must be NULL): a node whose centre of mass lies farther than
+ * rg / phi from the group's box centre c (rg the box's half-diagonal) is far;
+ * it is accepted when its side is below theta or, being far, below theta_far
+ * (used when larger than theta) times the distance to the group's box, and
+ * is not summed per particle: its monopole field is expanded about c to
+ * second order (orc_far_add) and, with quad_far (num_nodes x 6 second
+ * moments, may be NULL), the value of its quadrupole field at c is added to
+ * the expansion's constant term; the sum of the expansions is evaluated at
+ * each particle (orc_far_eval).  Such a node counts as one interaction of
+ * each particle of the group; *far_pairs (may be NULL) gets the number of
+ * (group, node) pairs expanded. */
 void orc_bh_walk_group_le(double *a, int64_t *n_int, int n, const double *x, const double *m,
                           double G, double eps, double theta, double width,
                           const int64_t *sorted_keys, const int32_t *sorted_idx, const int *depth,
                           const int64_t *prefix, const int *count, const int *first,
                           const int *nchild, const int *child0, const double *mass,
                           const double *comx, const double *comy, const double *comz,
-                          const double *quad, int group, double phi, int64_t *far_pairs);
+                          const double *quad, int group, double phi, double theta_far,
+                          const double *quad_far, int64_t *far_pairs);
 
 /* Moments of one source for the second-order local expansion (23 sums, see
  * oracle.c) and the expansion's value at offset d from the centre. */

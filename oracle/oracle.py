@@ -76,7 +76,7 @@ def lib():
         L.orc_bh_walk_group.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI, PI,
                                         PD, PD, PD, PD, PD, i]
         L.orc_bh_walk_group_le.argtypes = [PD, PI64, i, PD, PD, d, d, d, d, PI64, PI, PI, PI64, PI, PI, PI,
-                                           PI, PD, PD, PD, PD, PD, i, d, PI64]
+                                           PI, PD, PD, PD, PD, PD, i, d, d, PD, PI64]
         L.orc_far_add.argtypes = [PD, d, d, d, d, d]
         L.orc_far_eval.argtypes = [PD, d, d, d, PD]
         L.orc_octree_quad.argtypes = [i, PD, PD, PI, PI, PI, PI, PI, PD, PD, PD, PD, PD]
@@ -265,12 +265,15 @@ def bh_walk(x, m, tree, theta, G=1.0, eps=0.0, want_counts=False, quad=None, fea
     return out if len(out) > 1 else out[0]
 
 
-def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None, phi=0.0):
+def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None, phi=0.0, theta_far=0.0,
+                  quad_far=None):
     """Group walk (groups of at most `group` consecutive sorted particles,
     formed from the tree as orc_bh_walk_group describes); returns (a,
     per-particle interaction counts).  group=1 is bh_walk.
 
-    phi > 0 (quad must be None): local expansions of the far nodes
+    phi > 0 (quad must be None): local expansions of the far nodes, accepted
+    at theta_far when that is larger than theta, with the quadrupole value of
+    quad_far (num_nodes, 6) at the centre when given
     (orc_bh_walk_group_le); returns (a, counts, far (group, node) pairs)."""
     x, m = _c(x), _c(m)
     n = len(m)
@@ -288,7 +291,8 @@ def bh_walk_group(x, m, tree, theta, group, G=1.0, eps=0.0, quad=None, phi=0.0):
         if quad is not None:
             raise ValueError("local expansions are restated for the monopole walk only")
         far = C.c_int64()
-        lib().orc_bh_walk_group_le(*args, float(phi), C.byref(far))
+        qf = _abi.dptr(_c(quad_far)) if quad_far is not None else None
+        lib().orc_bh_walk_group_le(*args, float(phi), float(theta_far), qf, C.byref(far))
         return a, cnt, far.value
     lib().orc_bh_walk_group(*args)
     return a, cnt

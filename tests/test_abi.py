@@ -244,3 +244,15 @@ def test_group_expansion_ratio_setter_checks_its_range():
         assert L.get_group_expansion_ratio() == 0.25
     finally:
         L.set_group_expansion_ratio(0.1)
+
+
+def test_group_expansion_angle_factor_setter_checks_its_range():
+    assert L.get_group_expansion_angle_factor() == 1.25
+    for bad in (0.99, 4.5, float("nan")):
+        with pytest.raises(RuntimeError, match="Group expansion angle factor"):
+            L.set_group_expansion_angle_factor(bad)
+    L.set_group_expansion_angle_factor(1.0)
+    try:
+        assert L.get_group_expansion_angle_factor() == 1.0
+    finally:
+        L.set_group_expansion_angle_factor(1.25)

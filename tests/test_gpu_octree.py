@@ -268,22 +268,34 @@ def test_group_walk_ragged_and_theta_zero(n, leaf):
 
 
 # ------------------------------------------------ local expansions (method 22)
-@pytest.mark.parametrize("n,theta,eps,leaf,phi", [(8192, 0.5, 0.0, 1, 0.3), (8192, 0.8, 1e-3, 3, 0.5),
-                                                  (30000, 0.4, 1e-3, 1, 0.1)])
-def test_group_expansion_matches_oracle(n, theta, eps, leaf, phi):
-    """Method 22 takes the oracle's classification exactly (equal interaction
-    counts and equal numbers of expanded (group, node) pairs) and its forces
-    stay within FP32 rounding of the oracle's FP64 sums (the far moments are
-    summed in FP32 on the GPU)."""
-    print("group expansion case", n, theta, eps, leaf, phi, flush=True)
-    x, v, m = ic.two_spheres(n, 7)
-    t = O.build_tree(x, m, leaf)
-    a_o, cnt, far = O.bh_walk_group(x, m, t, theta, 64, eps=eps, phi=phi)
+def _expansion_oracle(x, m, t, theta, eps, phi, k):
+    q = O.octree_quad(x, m, t)
+    return O.bh_walk_group(x, m, t, theta, 64, eps=eps, phi=phi, theta_far=k * theta, quad_far=q)
+
+
+def _expansion_gpu(x, m, theta, eps, leaf, phi, k):
     L.set_group_expansion_ratio(phi)
+    L.set_group_expansion_angle_factor(k)
     try:
-        a_g = L.acceleration(x, m, 22, eps=eps, theta=theta, leaf=leaf)
+        return L.acceleration(x, m, 22, eps=eps, theta=theta, leaf=leaf)
     finally:
         L.set_group_expansion_ratio(0.1)
+        L.set_group_expansion_angle_factor(1.25)
+
+
+@pytest.mark.parametrize("n,theta,eps,leaf,phi,k", [(8192, 0.5, 0.0, 1, 0.3, 1.0), (8192, 0.8, 1e-3, 3, 0.5, 1.0),
+                                                    (8192, 0.5, 0.0, 1, 0.3, 1.25), (8192, 0.6, 1e-3, 3, 0.5, 1.5),
+                                                    (30000, 0.4, 1e-3, 1, 0.1, 1.0), (30000, 0.4, 1e-3, 1, 0.1, 1.25)])
+def test_group_expansion_matches_oracle(n, theta, eps, leaf, phi, k):
+    """Method 22 takes the oracle's classification exactly (equal interaction
+    counts and equal numbers of expanded (group, node) pairs, with far nodes
+    accepted at k theta) and its forces stay within FP32 rounding of the
+    oracle's FP64 sums (the far moments are summed in FP32 on the GPU)."""
+    print("group expansion case", n, theta, eps, leaf, phi, k, flush=True)
+    x, v, m = ic.two_spheres(n, 7)
+    t = O.build_tree(x, m, leaf)
+    a_o, cnt, far = _expansion_oracle(x, m, t, theta, eps, phi, k)
+    a_g = _expansion_gpu(x, m, theta, eps, leaf, phi, k)
     assert far > 0
     assert L.last_walk_interactions() == int(cnt.sum())
     assert L.last_walk_expansion_pairs() == far
@@ -302,12 +314,8 @@ def test_group_expansion_ragged_and_clump():
     x[:40] = rng.uniform(-60.0, 60.0, size=(40, 3))
     for xs, ms in ((x, m), (x[:63], m[:63])):
         t = O.build_tree(xs, ms, 1)
-        a_o, cnt, far = O.bh_walk_group(xs, ms, t, 0.6, 64, eps=1e-3, phi=0.2)
-        L.set_group_expansion_ratio(0.2)
-        try:
-            a_g = L.acceleration(xs, ms, 22, eps=1e-3, theta=0.6, leaf=1)
-        finally:
-            L.set_group_expansion_ratio(0.1)
+        a_o, cnt, far = _expansion_oracle(xs, ms, t, 0.6, 1e-3, 0.2, 1.25)
+        a_g = _expansion_gpu(xs, ms, 0.6, 1e-3, 1, 0.2, 1.25)
         assert L.last_walk_interactions() == int(cnt.sum())
         assert L.last_walk_expansion_pairs() == far
         e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
@@ -316,18 +324,27 @@ def test_group_expansion_ragged_and_clump():
 
 def test_group_expansion_error_matches_group_walk_n2e20():
     """N=2^20 two spheres, theta = 0.4, phi = 0.1: against the FP64 direct
-    sum, the expansions leave the group walk's median and p99 errors within
-    5 % while taking most accepted nodes out of the per-particle sums."""
+    sum, the expansions (far nodes at theta, k = 1) leave the group walk's
+    median and p99 errors within 5 % with its interaction set; with the far
+    nodes at 1.25 theta and their quadrupole values (the default) the p99
+    error stays within 5 % of the group walk's with fewer interactions."""
     x, v, m = ic.two_spheres(1 << 20, 4)
     sim = L.Sim(x, v, m, 1.0)
     grp = sim.accel_error(L.param(20, theta=0.4, leaf=1, eps=1e-3), samples=4096)
     n_grp = L.last_walk_interactions()
-    le = sim.accel_error(L.param(22, theta=0.4, leaf=1, eps=1e-3), samples=4096)
+    L.set_group_expansion_angle_factor(1.0)
+    try:
+        le1 = sim.accel_error(L.param(22, theta=0.4, leaf=1, eps=1e-3), samples=4096)
+    finally:
+        L.set_group_expansion_angle_factor(1.25)
     assert L.last_walk_interactions() == n_grp
-    far = L.last_walk_expansion_pairs()
-    print(f"N=2^20 theta=0.4: group p99 {grp[1]:.3e}, with expansions {le[1]:.3e}; {far} expanded pairs")
-    assert le[0] <= 1.05 * grp[0] and le[1] <= 1.05 * grp[1]
-    assert far > 0
+    far1 = L.last_walk_expansion_pairs()
+    le = sim.accel_error(L.param(22, theta=0.4, leaf=1, eps=1e-3), samples=4096)
+    n_le, far = L.last_walk_interactions(), L.last_walk_expansion_pairs()
+    print(f"N=2^20 theta=0.4: group p99 {grp[1]:.3e}, with expansions {le1[1]:.3e} (k=1, {far1} pairs), "
+          f"{le[1]:.3e} (k=1.25, {far} pairs, {n_le / n_grp:.3f} of the interactions)")
+    assert le1[0] <= 1.05 * grp[0] and le1[1] <= 1.05 * grp[1] and far1 > 0
+    assert le[1] <= 1.05 * grp[1] and n_le < n_grp and far > 0
     sim.close()
 
 

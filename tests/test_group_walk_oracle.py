@@ -121,3 +121,20 @@ def test_expansions_keep_the_group_walk_accuracy():
     e0 = np.linalg.norm(a0 - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
     e1 = np.linalg.norm(a1 - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
     assert np.median(e1) <= 1.03 * np.median(e0) and np.quantile(e1, 0.99) <= 1.03 * np.quantile(e0, 0.99)
+
+
+def test_far_opening_angle_with_quadrupoles_keeps_the_accuracy():
+    """Far nodes accepted at 1.25 theta, with their quadrupole values at the
+    group centre: fewer interactions than the group walk at theta, and its
+    median and p99 errors against the FP64 direct sum are kept (within 1 %)."""
+    x, _, m = ic.two_spheres(1 << 14, 5)
+    t = O.build_tree(x, m, 1)
+    q = O.octree_quad(x, m, t)
+    a_d = O.pairwise(x, m, eps=1e-3)
+    for theta in (0.4, 0.5):
+        a0, c0 = O.bh_walk_group(x, m, t, theta, 64, eps=1e-3)
+        a1, c1, far = O.bh_walk_group(x, m, t, theta, 64, eps=1e-3, phi=0.1, theta_far=1.25 * theta, quad_far=q)
+        assert c1.sum() < c0.sum() and np.all(c1 <= c0) and far > 0
+        e0 = np.linalg.norm(a0 - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
+        e1 = np.linalg.norm(a1 - a_d, axis=1) / np.linalg.norm(a_d, axis=1)
+        assert np.median(e1) <= 1.01 * np.median(e0) and np.quantile(e1, 0.99) <= 1.01 * np.quantile(e0, 0.99)
