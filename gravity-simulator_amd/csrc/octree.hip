@@ -432,9 +432,22 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         }
     } else {
         for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
-            const NodeRec &ch = nodes[c];
-            const double d0 = ch.cx - c0, d1 = ch.cy - c1, d2 = ch.cz - c2;
-            const double Mc = mom[c].x;
+            // the child's centre of mass as its record holds it (X / M, or
+            // its first particle when massless), without reading the record
+            const double4 mc = mom[c];
+            double cc0, cc1, cc2;
+            if (mc.x != 0.0) {
+                cc0 = mc.y / mc.x;
+                cc1 = mc.z / mc.x;
+                cc2 = mc.w / mc.x;
+            } else {
+                const double4 p = P[nodes[c].first];
+                cc0 = p.x;
+                cc1 = p.y;
+                cc2 = p.z;
+            }
+            const double d0 = cc0 - c0, d1 = cc1 - c1, d2 = cc2 - c2;
+            const double Mc = mc.x;
             const double *qc = quad + 8 * static_cast<size_t>(c);
             q0 += qc[0] + Mc * d0 * d0;
             q1 += qc[1] + Mc * d1 * d1;
