@@ -547,6 +547,8 @@ constexpr int FB_TILE = FB_BLOCK * FB_PER;  // particles per workgroup
 constexpr int FB_DEPTHS = MAXL + 1;
 constexpr int FB_WAVES = FB_BLOCK / 64;
 constexpr int FB_MAX_LEAF = 16;             // window search cost grows as leaf_cap^2
+constexpr int FB_SCAN = 64;                 // L entries a thread scans before searching keys
+constexpr int FB_PAD = 4;                   // -1 entries after L_n (the scan reads 4 at a time)
 
 __device__ __forceinline__ int shared_digits(long long a, long long b)
 {
@@ -559,8 +561,8 @@ __device__ __forceinline__ int shared_digits(long long a, long long b)
 __global__ void k_fb_digits(const long long *__restrict__ keys, int n, signed char *__restrict__ L)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    L[i] = static_cast<signed char>((i == 0 || i == n) ? -1 : shared_digits(keys[i - 1], keys[i]));
+    if (i > n + FB_PAD) return;
+    L[i] = static_cast<signed char>((i == 0 || i >= n) ? -1 : shared_digits(keys[i - 1], keys[i]));
 }
 
 __global__ void k_fb_leafdepth(const signed char *__restrict__ L, int n, int leaf_cap,
@@ -675,12 +677,37 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
     for (int r = 0; r < FB_PER; r++) {
         const int i = blockIdx.x * FB_TILE + r * FB_BLOCK + threadIdx.x;
         int below = -1;  // index of node (i, d + 1)
+        // Cell ends and child counts from a forward scan of L: the cell of
+        // depth d starting at i ends at the first j > i with L_j < d, and its
+        // children start at i and at the positions before that end with
+        // L_j == d (all positions inside have L_j >= d).  Depths are visited
+        // deepest first, so the scan resumes where the deeper cell ended; past
+        // FB_SCAN positions the keys are searched instead.
+        int j = i + 1;
+        bool scan = true;
         for (int d = FB_DEPTHS - 1; d >= 0; d--) {
             const bool is = fb_node(l[r], dd[r], d);
             const unsigned long long m = __ballot(is);
             if (!is) continue;
             const int idx = static_cast<int>(base[d][r][wave] + __popcll(m & lt));
             const int shift = 3 * (MAXL - d);
+            int end = -1, kids = 1;
+            if (d > 0 && scan) {
+                while (j - i <= FB_SCAN) {
+                    const int v0 = L[j], v1 = L[j + 1], v2 = L[j + 2], v3 = L[j + 3];
+                    if (v0 < d) { end = j; break; }
+                    kids += v0 == d ? 1 : 0;
+                    if (v1 < d) { end = j + 1; break; }
+                    kids += v1 == d ? 1 : 0;
+                    if (v2 < d) { end = j + 2; break; }
+                    kids += v2 == d ? 1 : 0;
+                    if (v3 < d) { end = j + 3; break; }
+                    kids += v3 == d ? 1 : 0;
+                    j += 4;
+                }
+                if (end >= 0) j = end;
+                else scan = false;
+            }
             NodeRec nd;
             nd.cx = nd.cy = nd.cz = 0.0;
             // (2w / (2 << d))^2 == w^2 * 4^-d exactly (power-of-two scaling)
@@ -688,13 +715,16 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
             nd.prefix = keys[i] >> shift;
             nd.depth = d;
             nd.first = i;
-            nd.count = (d == 0 ? n : cell_end(keys, n, i, shift)) - i;
+            nd.count = (d == 0 ? n : (end >= 0 ? end : cell_end(keys, n, i, shift))) - i;
             nd.fmass = 0.0f;
             if (d < dd[r]) {
-                int b[9];
-                octant_bounds(keys, i, nd.count, d + 1, b);
-                int nc = 0;
-                for (int o = 0; o < 8; o++) nc += b[o + 1] > b[o] ? 1 : 0;
+                int nc = kids;
+                if (end < 0) {
+                    int b[9];
+                    octant_bounds(keys, i, nd.count, d + 1, b);
+                    nc = 0;
+                    for (int o = 0; o < 8; o++) nc += b[o + 1] > b[o] ? 1 : 0;
+                }
                 nd.child0 = below;
                 nd.nchild = nc;
             } else {
@@ -1687,7 +1717,7 @@ static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_ca
     if (hn > scan_max_elements())
         return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Too many particles for the octree: %d", n);
     auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
-    const size_t bL = up(static_cast<size_t>(n) + 1), bD = up(n),
+    const size_t bL = up(static_cast<size_t>(n) + 1 + FB_PAD), bD = up(n),
                  bC = up(sizeof(unsigned) * static_cast<size_t>(hn)), bV = up(sizeof(unsigned) * 32);
     void *p;
     GS_TRY(slot(c, SLOT_LEVELS, bL + bD + bC + bV, &p));
@@ -1703,8 +1733,8 @@ static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_ca
     GS_TRY(pinned(c, 256, &hp));
     unsigned *lev_host = static_cast<unsigned *>(hp);
 
-    hipLaunchKernelGGL(k_fb_digits, dim3((n + 1 + 255) / 256), dim3(256), 0, c->stream, t->skeys,
-                       n, Ld);
+    hipLaunchKernelGGL(k_fb_digits, dim3((n + 1 + FB_PAD + 255) / 256), dim3(256), 0, c->stream,
+                       t->skeys, n, Ld);
     hipLaunchKernelGGL(k_fb_leafdepth, dim3((n + 255) / 256), dim3(256), 0, c->stream, Ld, n,
                        leaf_cap, Dd);
     hipLaunchKernelGGL(k_fb_count, dim3(ntiles), dim3(FB_BLOCK), 0, c->stream, Ld, Dd, n, ntiles,

@@ -59,14 +59,14 @@ def test_benchmark_records_every_method():
     try:
         sets = [(1, 0.0, 1e-3, -1, 1), (2, 0.0, 1e-3, -1, 1), (16, 0.0, 1e-3, -1, 2), (17, 0.0, 1e-3, -1, 1),
                 (3, 0.5, 1e-3, 1, 1), (18, 0.8, 1e-3, 1, 1), (19, 0.5, 1e-3, 1, 1), (20, 0.5, 1e-3, 1, 1),
-                (21, 0.5, 1e-3, 1, 1), (16, 0.0, 1e-3, -1, 0)]
+                (21, 0.5, 1e-3, 1, 1), (22, 0.5, 1e-3, 1, 1), (16, 0.0, 1e-3, -1, 0)]
         recs, text = capture_stdout(lambda: L.benchmark_acceleration(x, m, sets))
     finally:
         L.learned_reset()
     names = re.findall(r"Method: (.*)\n", text)
     assert names == ["Pairwise", "Massless", "Pairwise (FP32 split)", "Massless (FP32 split)", "Barnes-Hut",
                      "Barnes-Hut (learned correction)", "Barnes-Hut (quadrupole)", "Barnes-Hut (group walk)",
-                     "Barnes-Hut (group walk, quadrupole)"]
+                     "Barnes-Hut (group walk, quadrupole)", "Barnes-Hut (group walk, local expansions)"]
     n = len(m)
     massive = np.count_nonzero(m)
     info = L.device_info()
