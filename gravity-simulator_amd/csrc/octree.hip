@@ -1157,7 +1157,10 @@ struct FarMoments {
 };
 
 // One far node: its monopole moments, and the value of its quadrupole field
-// at the centre (R = c - com = y) added to a0.
+// at the centre (R = c - com = y) added to a0.  The moments are formed from
+// u = y / s (|u| <= 1): P = 3 q1 u u, S = 3 q1 s^-1 u, T = 15 q1 s^-1 u u u,
+// so no intermediate exceeds M s^-4 (q3 itself, M s^-7, would overflow FP32
+// for heavy nodes a few 1e-6 away without softening).
 __device__ __forceinline__ void far_add(FarMoments &F, float y0, float y1, float y2, float m,
                                         float eps2, const float *__restrict__ quad)
 {
@@ -1165,39 +1168,38 @@ __device__ __forceinline__ void far_add(FarMoments &F, float y0, float y1, float
     s2 = __builtin_fmaf(y1, y1, s2);
     s2 = __builtin_fmaf(y2, y2, s2);
     const float ri = __builtin_amdgcn_rsqf(s2);
-    const float ri2 = ri * ri;
-    const float q1 = (m * ri) * ri2;
-    const float q2 = 3.0f * q1 * ri2;
-    const float q3 = 5.0f * q2 * ri2;
+    const float u0 = y0 * ri, u1 = y1 * ri, u2 = y2 * ri;
+    const float q1 = (m * ri) * (ri * ri);
+    const float c1 = 3.0f * q1, c2 = c1 * ri, c3 = 5.0f * c2;
     float o0, o1, o2;
     quad_termf(quad, y0, y1, y2, ri, o0, o1, o2);
     F.Q += q1;
     F.A0 = __builtin_fmaf(-q1, y0, F.A0) + o0;
     F.A1 = __builtin_fmaf(-q1, y1, F.A1) + o1;
     F.A2 = __builtin_fmaf(-q1, y2, F.A2) + o2;
-    const float u0 = q2 * y0, u1 = q2 * y1, u2 = q2 * y2;
-    F.S0 += u0;
-    F.S1 += u1;
-    F.S2 += u2;
-    F.Pxx = __builtin_fmaf(u0, y0, F.Pxx);
-    F.Pyy = __builtin_fmaf(u1, y1, F.Pyy);
-    F.Pzz = __builtin_fmaf(u2, y2, F.Pzz);
-    F.Pxy = __builtin_fmaf(u0, y1, F.Pxy);
-    F.Pxz = __builtin_fmaf(u0, y2, F.Pxz);
-    F.Pyz = __builtin_fmaf(u1, y2, F.Pyz);
-    const float v0 = q3 * y0, v1 = q3 * y1, v2 = q3 * y2;
-    const float v00 = v0 * y0, v01 = v0 * y1, v02 = v0 * y2, v11 = v1 * y1, v12 = v1 * y2,
-                v22 = v2 * y2;
-    F.Txxx = __builtin_fmaf(v00, y0, F.Txxx);
-    F.Txxy = __builtin_fmaf(v00, y1, F.Txxy);
-    F.Txxz = __builtin_fmaf(v00, y2, F.Txxz);
-    F.Txyy = __builtin_fmaf(v01, y1, F.Txyy);
-    F.Txyz = __builtin_fmaf(v01, y2, F.Txyz);
-    F.Txzz = __builtin_fmaf(v02, y2, F.Txzz);
-    F.Tyyy = __builtin_fmaf(v11, y1, F.Tyyy);
-    F.Tyyz = __builtin_fmaf(v11, y2, F.Tyyz);
-    F.Tyzz = __builtin_fmaf(v12, y2, F.Tyzz);
-    F.Tzzz = __builtin_fmaf(v22, y2, F.Tzzz);
+    F.S0 = __builtin_fmaf(c2, u0, F.S0);
+    F.S1 = __builtin_fmaf(c2, u1, F.S1);
+    F.S2 = __builtin_fmaf(c2, u2, F.S2);
+    const float w0 = c1 * u0, w1 = c1 * u1, w2 = c1 * u2;
+    F.Pxx = __builtin_fmaf(w0, u0, F.Pxx);
+    F.Pyy = __builtin_fmaf(w1, u1, F.Pyy);
+    F.Pzz = __builtin_fmaf(w2, u2, F.Pzz);
+    F.Pxy = __builtin_fmaf(w0, u1, F.Pxy);
+    F.Pxz = __builtin_fmaf(w0, u2, F.Pxz);
+    F.Pyz = __builtin_fmaf(w1, u2, F.Pyz);
+    const float v0 = c3 * u0, v1 = c3 * u1, v2 = c3 * u2;
+    const float v00 = v0 * u0, v01 = v0 * u1, v02 = v0 * u2, v11 = v1 * u1, v12 = v1 * u2,
+                v22 = v2 * u2;
+    F.Txxx = __builtin_fmaf(v00, u0, F.Txxx);
+    F.Txxy = __builtin_fmaf(v00, u1, F.Txxy);
+    F.Txxz = __builtin_fmaf(v00, u2, F.Txxz);
+    F.Txyy = __builtin_fmaf(v01, u1, F.Txyy);
+    F.Txyz = __builtin_fmaf(v01, u2, F.Txyz);
+    F.Txzz = __builtin_fmaf(v02, u2, F.Txzz);
+    F.Tyyy = __builtin_fmaf(v11, u1, F.Tyyy);
+    F.Tyyz = __builtin_fmaf(v11, u2, F.Tyyz);
+    F.Tyzz = __builtin_fmaf(v12, u2, F.Tyzz);
+    F.Tzzz = __builtin_fmaf(v22, u2, F.Tzzz);
 }
 
 __device__ __forceinline__ float wave_sumf(float v)

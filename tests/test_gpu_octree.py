@@ -322,6 +322,24 @@ def test_group_expansion_ragged_and_clump():
         assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (len(ms), np.median(e), e.max())
 
 
+def test_group_expansion_tight_clump_without_softening():
+    """A clump of 200 particles within 1e-5 of each other, eps = 0 (heavy far
+    nodes a short distance from tight groups): every force is finite and
+    within FP32 rounding of the oracle's FP64 expansions."""
+    x, v, m = ic.plummer(6000, 47)
+    rng = np.random.default_rng(47)
+    x[:200] = x[0] + 1e-5 * rng.uniform(-1.0, 1.0, size=(200, 3))
+    m = m.copy()
+    m[:200] *= 20.0  # a heavy clump
+    t = O.build_tree(x, m, 1)
+    a_o, cnt, far = _expansion_oracle(x, m, t, 0.5, 0.0, 0.1, 1.25)
+    a_g = _expansion_gpu(x, m, 0.5, 0.0, 1, 0.1, 1.25)
+    assert np.all(np.isfinite(a_g))
+    assert L.last_walk_interactions() == int(cnt.sum()) and L.last_walk_expansion_pairs() == far
+    e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
 def test_group_expansion_error_matches_group_walk_n2e20():
     """N=2^20 two spheres, theta = 0.4, phi = 0.1: against the FP64 direct
     sum, the expansions (far nodes at theta, k = 1) leave the group walk's
