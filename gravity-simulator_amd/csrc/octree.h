@@ -36,10 +36,11 @@ struct DeviceTree {
     double4 *P = nullptr;           // device: (x, y, z, m) in sorted order
     NodeRec *nodes = nullptr;       // device: level-ordered nodes
     double4 *mom = nullptr;         // device: (M, sum m x, sum m y, sum m z)
-    double *quad = nullptr;         // device: 8 per node (xx yy zz xy xz yz, 0, 0)
-    float *quadf = nullptr;         // device: 8 per node, the walk's FP32 copy
+    double *quad = nullptr;         // device: 8 per node (xx yy zz xy xz yz, 0, 0),
+                                    // FP64, formed only with the third moments
+    float *quadf = nullptr;         // device: 8 per node, the walks' FP32 moments
                                     // (xx yy zz xy xz yz, 1.5 tr, 0)
-    bool has_quad = false;          // quad holds this tree's second moments
+    bool has_quad = false;          // quadf holds this tree's second moments
     double *oct = nullptr;          // device: 12 per node (xxx yyy zzz xxy xxz xyy yyz
                                     // xzz yzz xyz, 0, 0): third central moments
     float *octf = nullptr;          // device: 12 per node, the walk's FP32 copy

@@ -371,7 +371,9 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
 }
 
 // moments: mom[node] = (sum m, sum m x, sum m y, sum m z); with QUAD also
-// the second central moments (only the quadrupole and feature walks read them)
+// the second central moments (only the quadrupole, expansion and feature
+// walks read them): in FP32 from the children's FP32 moments, or with OCT in
+// FP64 (the third moments' shifts use them) and rounded to FP32
 template <bool QUAD, bool OCT>
 __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
                             const double4 *__restrict__ P, double4 *__restrict__ mom,
@@ -414,11 +416,63 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         nd.cz = p.z;
     }
     if (!QUAD) return;
-    // second central moments Q_ab = sum m (x - com)_a (x - com)_b
-    // (xx yy zz xy xz yz): leaves from their particles, internal nodes from
+    const double c0 = nd.cx, c1 = nd.cy, c2 = nd.cz;
+    float4 *qf = reinterpret_cast<float4 *>(quadf + 8 * static_cast<size_t>(node));
+    if (!OCT) {
+        // second central moments in FP32 only, the walks' copy (xx yy zz xy
+        // xz yz, 1.5 tr): leaves from their particles, internal nodes from
+        // their children's FP32 moments shifted to this centre of mass
+        // (parallel axes; the offsets formed in FP64)
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f, f4 = 0.0f, f5 = 0.0f;
+        if (nd.nchild == 0) {
+            for (int s = nd.first; s < nd.first + nd.count; s++) {
+                const double4 p = P[s];
+                const float d0 = static_cast<float>(p.x - c0), d1 = static_cast<float>(p.y - c1),
+                            d2 = static_cast<float>(p.z - c2);
+                const float mp = static_cast<float>(p.w);
+                f0 = __builtin_fmaf(mp * d0, d0, f0);
+                f1 = __builtin_fmaf(mp * d1, d1, f1);
+                f2 = __builtin_fmaf(mp * d2, d2, f2);
+                f3 = __builtin_fmaf(mp * d0, d1, f3);
+                f4 = __builtin_fmaf(mp * d0, d2, f4);
+                f5 = __builtin_fmaf(mp * d1, d2, f5);
+            }
+        } else {
+            for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
+                const double4 mc = mom[c];
+                double cc0, cc1, cc2;
+                if (mc.x != 0.0) {
+                    cc0 = mc.y / mc.x;
+                    cc1 = mc.z / mc.x;
+                    cc2 = mc.w / mc.x;
+                } else {
+                    const double4 p = P[nodes[c].first];
+                    cc0 = p.x;
+                    cc1 = p.y;
+                    cc2 = p.z;
+                }
+                const float d0 = static_cast<float>(cc0 - c0), d1 = static_cast<float>(cc1 - c1),
+                            d2 = static_cast<float>(cc2 - c2);
+                const float Mc = static_cast<float>(mc.x);
+                const float4 qa = reinterpret_cast<const float4 *>(quadf)[2 * static_cast<size_t>(c)];
+                const float4 qb = reinterpret_cast<const float4 *>(quadf)[2 * static_cast<size_t>(c) + 1];
+                f0 += __builtin_fmaf(Mc * d0, d0, qa.x);
+                f1 += __builtin_fmaf(Mc * d1, d1, qa.y);
+                f2 += __builtin_fmaf(Mc * d2, d2, qa.z);
+                f3 += __builtin_fmaf(Mc * d0, d1, qa.w);
+                f4 += __builtin_fmaf(Mc * d0, d2, qb.x);
+                f5 += __builtin_fmaf(Mc * d1, d2, qb.y);
+            }
+        }
+        qf[0] = make_float4(f0, f1, f2, f3);
+        qf[1] = make_float4(f4, f5, 1.5f * (f0 + f1 + f2), 0.0f);
+        return;
+    }
+    // with third moments: the second central moments Q_ab = sum m (x -
+    // com)_a (x - com)_b in FP64 (xx yy zz xy xz yz), which the third
+    // moments' shifts need: leaves from their particles, internal nodes from
     // their children shifted to this centre of mass (parallel axes)
     double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
-    const double c0 = nd.cx, c1 = nd.cy, c2 = nd.cz;
     if (nd.nchild == 0) {
         for (int s = nd.first; s < nd.first + nd.count; s++) {
             const double4 p = P[s];
@@ -468,7 +522,6 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
     qn[7] = 0.0;
     // the walk's FP32 copy: the six moments rounded to FP32 and 1.5 tr(Q)
     // formed from them, as the quadrupole term uses them
-    float4 *qf = reinterpret_cast<float4 *>(quadf + 8 * static_cast<size_t>(node));
     const float fxx = static_cast<float>(q0), fyy = static_cast<float>(q1),
                 fzz = static_cast<float>(q2);
     qf[0] = make_float4(fxx, fyy, fzz, static_cast<float>(q3));

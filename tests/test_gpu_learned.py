@@ -126,7 +126,10 @@ def test_forced_fallback_is_the_group_quadrupole_walk_checked_again():
     """A spot check that fails replaces the result by the group walk with
     quadrupoles at the fallback angle and checks that again: the status
     then reports the fallback's own p99, and 2 when it misses the tolerance
-    too (here a tolerance nothing meets)."""
+    too (here a tolerance nothing meets).  The fallback walks the tree the
+    learned method built (FP32 second moments rounded from its FP64 ones);
+    method 21 forms its FP32 moments directly, so the two agree to FP32
+    rounding of the quadrupole terms."""
     x, v, m = ic.two_spheres(1 << 15, 43)
     L.learned_set_params(NN.random_params(13).astype(np.float32) * 1e-3)
     L.learned_set_policy(1e-12, 256, 0.5)
@@ -134,7 +137,9 @@ def test_forced_fallback_is_the_group_quadrupole_walk_checked_again():
     p99, fell_back = L.learned_status()
     assert fell_back == 2 and p99 > 1e-12
     a_fb = L.acceleration(x, m, 21, theta=0.5, leaf=1)
-    assert np.array_equal(a, a_fb)
+    assert L.last_walk_interactions() > 0
+    e = np.linalg.norm(a - a_fb, axis=1) / np.linalg.norm(a_fb, axis=1)
+    assert np.median(e) <= 1e-7 and e.max() <= 1e-5, (np.median(e), e.max())
     # the reported p99 is the fallback's: the spot check of the group walk itself
     L.learned_set_policy(1.0, 256, 0.5)
     sim = L.Sim(x, v, m, 1.0)
