@@ -52,7 +52,7 @@ enum Slot : int {
     // octree
     SLOT_BBOX, SLOT_KEYS, SLOT_KEYS_ALT, SLOT_IDX, SLOT_IDX_ALT, SLOT_HIST, SLOT_HIST_SCAN,
     SLOT_NODES, SLOT_NODE_TMP, SLOT_LEVELS, SLOT_PACKED_P, SLOT_SORTED_P, SLOT_MISC,
-    SLOT_GROUP_PARENTS,
+    SLOT_GROUP_PARENTS, SLOT_GROUP_FLAGS,
     // learned correction
     SLOT_FEAT, SLOT_MLP_W, SLOT_MLP_ACT, SLOT_MLP_GRAD, SLOT_MLP_X, SLOT_SPOT, SLOT_MLP_TS, SLOT_MLP_IMG,
     SLOT_COUNT

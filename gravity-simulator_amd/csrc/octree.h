@@ -46,6 +46,8 @@ struct DeviceTree {
     float *octf = nullptr;          // device: 12 per node, the walk's FP32 copy
     int oct_cap = 0;                // nodes oct / octf hold room for
     bool has_oct = false;           // oct holds this tree's third moments
+    unsigned *gflag = nullptr;      // device: per node, 1 for the root and the nodes of
+                                    // more than 64 particles (the group walk's parents)
     unsigned *tmp = nullptr;        // device: per-parent child counts
     unsigned *scan_tmp = nullptr;   // device: scan scratch
 };

@@ -374,16 +374,22 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
 // the second central moments (only the quadrupole, expansion and feature
 // walks read them): in FP32 from the children's FP32 moments, or with OCT in
 // FP64 (the third moments' shifts use them) and rounded to FP32
+// Also marks the group walk's parent nodes (the root and every node of more
+// than 64 particles) in gflag, from the record it reads anyway.
 template <bool QUAD, bool OCT>
 __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
                             const double4 *__restrict__ P, double4 *__restrict__ mom,
                             double *__restrict__ quad, float *__restrict__ quadf,
-                            double *__restrict__ oct, float *__restrict__ octf)
+                            double *__restrict__ oct, float *__restrict__ octf,
+                            unsigned *__restrict__ gflag)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= le - lb) return;
     const int node = lb + t;
-    NodeRec &nd = nodes[node];
+    // the record is read whole and written back whole (neighbouring threads
+    // write neighbouring records: full cache lines, no partial-line merges)
+    NodeRec nd = nodes[node];
+    gflag[node] = (node == 0 || nd.count > 64) ? 1u : 0u;
     double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
     if (nd.nchild == 0) {
         for (int s = nd.first; s < nd.first + nd.count; s++) {
@@ -415,6 +421,7 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         nd.cy = p.y;
         nd.cz = p.z;
     }
+    nodes[node] = nd;
     if (!QUAD) return;
     const double c0 = nd.cx, c1 = nd.cy, c2 = nd.cz;
     float4 *qf = reinterpret_cast<float4 *>(quadf + 8 * static_cast<size_t>(node));
@@ -1119,18 +1126,15 @@ __device__ __forceinline__ void quad_termf(const float *__restrict__ q, float r0
 // 64-particle group at the mean density) gives one group per particle --
 // sparse outliers, for which that is the per-particle walk.
 // The nodes that make groups: the root, and every node of more than 64
-// particles (flag pass, scan, then the list at the scanned offsets).
-__global__ void k_group_parent_flags(const NodeRec *__restrict__ nodes, int num_nodes,
-                                     unsigned *__restrict__ flag)
+// particles (flagged by the moments pass; scanned; then the list at the
+// scanned offsets, a node being flagged where its offset and the next differ).
+__global__ void k_group_parent_list(int num_nodes, const unsigned *__restrict__ off,
+                                    const unsigned *__restrict__ total, int *__restrict__ parents)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < num_nodes) flag[p] = (p == 0 || nodes[p].count > 64) ? 1u : 0u;
-}
-__global__ void k_group_parent_list(const NodeRec *__restrict__ nodes, int num_nodes,
-                                    const unsigned *__restrict__ off, int *__restrict__ parents)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < num_nodes && (p == 0 || nodes[p].count > 64)) parents[off[p]] = p;
+    if (p >= num_nodes) return;
+    const unsigned o = off[p], o1 = p + 1 < num_nodes ? off[p + 1] : *total;
+    if (o1 > o) parents[o] = p;
 }
 
 // One wavefront per parent node: its runs are formed in the same order by
@@ -1743,19 +1747,22 @@ static ErrorStatus tree_moments(Context *c, DeviceTree *t, bool with_quad, bool 
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&t->octf), sizeof(float) * 12 * t->node_cap));
         t->oct_cap = t->node_cap;
     }
+    void *gp;
+    GS_TRY(slot(c, SLOT_GROUP_FLAGS, sizeof(unsigned) * static_cast<size_t>(t->num_nodes) + 256, &gp));
+    t->gflag = static_cast<unsigned *>(gp);
     for (int d = t->num_levels - 1; d >= 0; d--) {
         const int lb = t->level_begin[d], le = t->level_end[d];
         if (le <= lb) continue;
         const dim3 grid((le - lb + 255) / 256);
         if (with_oct)
             hipLaunchKernelGGL((k_multipole<true, true>), grid, dim3(256), 0, c->stream, t->nodes, lb,
-                               le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf);
+                               le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf, t->gflag);
         else if (with_quad)
             hipLaunchKernelGGL((k_multipole<true, false>), grid, dim3(256), 0, c->stream, t->nodes,
-                               lb, le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf);
+                               lb, le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf, t->gflag);
         else
             hipLaunchKernelGGL((k_multipole<false, false>), grid, dim3(256), 0, c->stream, t->nodes,
-                               lb, le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf);
+                               lb, le, t->P, t->mom, t->quad, t->quadf, t->oct, t->octf, t->gflag);
     }
     GS_HIP(hipGetLastError());
     return make_success_error_status();
@@ -2009,10 +2016,8 @@ ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, dou
     const int rec = timing_begin(c, "bh_walk");
     const double cap = group_width_cap(t->width, t->n);
     const dim3 ngrid((t->num_nodes + 255) / 256);
-    hipLaunchKernelGGL(k_group_parent_flags, ngrid, dim3(256), 0, c->stream, t->nodes, t->num_nodes, offs);
-    GS_HIP(hipGetLastError());
-    GS_HIP(scan_exclusive(c->stream, offs, offs, t->num_nodes, stmp, ng));
-    hipLaunchKernelGGL(k_group_parent_list, ngrid, dim3(256), 0, c->stream, t->nodes, t->num_nodes, offs,
+    GS_HIP(scan_exclusive(c->stream, t->gflag, offs, t->num_nodes, stmp, ng));
+    hipLaunchKernelGGL(k_group_parent_list, ngrid, dim3(256), 0, c->stream, t->num_nodes, offs, ng,
                        parents);
     GS_HIP(hipGetLastError());
     unsigned num_parents = 0;
