@@ -1098,6 +1098,15 @@ __device__ __forceinline__ double wave_max(double v)
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
     return v;
 }
+// A value every lane holds alike, moved to scalar registers (the compiler
+// cannot see that a cross-lane reduction is uniform).
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b));
+    const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
 
 // Quadrupole term in FP32, R = x_i - com (the same expression as quad_term).
 __device__ __forceinline__ void quad_termf(const float *__restrict__ q, float r0, float r1,
@@ -1329,8 +1338,10 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     int *rbeg = rbeg_all[wave];
 
     const double4 ps = P[valid ? s : g0];
-    const double lo0 = wave_min(ps.x), lo1 = wave_min(ps.y), lo2 = wave_min(ps.z);
-    const double hi0 = wave_max(ps.x), hi1 = wave_max(ps.y), hi2 = wave_max(ps.z);
+    const double lo0 = uniform(wave_min(ps.x)), lo1 = uniform(wave_min(ps.y)),
+                 lo2 = uniform(wave_min(ps.z));
+    const double hi0 = uniform(wave_max(ps.x)), hi1 = uniform(wave_max(ps.y)),
+                 hi2 = uniform(wave_max(ps.z));
     const double gc0 = 0.5 * (lo0 + hi0), gc1 = 0.5 * (lo1 + hi1), gc2 = 0.5 * (lo2 + hi2);
     const float xr = static_cast<float>(ps.x - gc0), yr = static_cast<float>(ps.y - gc1),
                 zr = static_cast<float>(ps.z - gc2);
@@ -1447,17 +1458,15 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
 
     // from the root itself: it holds the group, so it is opened (a root that
     // is a leaf -- n within the leaf capacity -- gives its particles directly)
-    if (lane == 0) stk[0] = 0;
-    __builtin_amdgcn_wave_barrier();
-    while (top > 0) {
-        int k = min(top, 64);
-        if (top + 7 * k > GW_BATCH_LIMIT) k = max(1, (GW_BATCH_LIMIT - top) / 7);
-        k = min(k, top);
+    // Each pass pops the next pass's entries and requests their records as
+    // soon as its own children are on the stack, so that the loads overlap
+    // the list sums.  The first pass is the root alone.
+    top = 0;
+    int k = 1;
+    int node = 0;
+    NodeRec nd = nodes[0];
+    while (k > 0) {
         const bool act = lane < k;
-        const int node = act ? stk[top - k + lane] : 0;
-        top -= k;
-        __builtin_amdgcn_wave_barrier();
-        const NodeRec nd = nodes[node];
         const bool leaf = nd.nchild == 0;
         const bool single_leaf = act && leaf && nd.count == 1;
         const bool incl = nd.first < gend && nd.first + nd.count > g0;
@@ -1531,6 +1540,15 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
             }
         }
         __builtin_amdgcn_wave_barrier();
+        // the next pass: at most 64 entries, fewer when its children could
+        // bring the stack past the batch limit
+        int kn = min(top, 64);
+        if (top + 7 * kn > GW_BATCH_LIMIT) kn = max(1, (GW_BATCH_LIMIT - top) / 7);
+        kn = min(kn, top);
+        const int node_next = lane < kn ? stk[top - kn + lane] : 0;
+        top -= kn;
+        __builtin_amdgcn_wave_barrier();
+        const NodeRec nd_next = nodes[node_next];
         if (np >= 64) flush_parts();
         // opened leaves of several particles: their particles into the
         // particle list, 64 slots at a time, every lane loading one particle
@@ -1578,6 +1596,9 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
             nn -= 64;
             __builtin_amdgcn_wave_barrier();
         }
+        nd = nd_next;
+        node = node_next;
+        k = kn;
     }
     if (nn > 0) sum_nodes(nn);
     if (np > 0) sum_parts(np);
