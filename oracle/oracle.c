@@ -792,6 +792,10 @@ void orc_octree_quad(int num, const double *x, const double *m, const int32_t *s
             for (int t = first[node]; t < first[node] + count[node]; t++)
             {
                 const int p = sorted_idx[t];
+                /* a massless particle adds nothing; skipped, since a massless
+                 * node's centre of mass is 0/0 as in the reference */
+                if (m[p] == 0.0)
+                    continue;
                 const double d0 = x[p * 3 + 0] - c[0], d1 = x[p * 3 + 1] - c[1],
                              d2 = x[p * 3 + 2] - c[2];
                 q[0] += m[p] * d0 * d0;
@@ -806,6 +810,8 @@ void orc_octree_quad(int num, const double *x, const double *m, const int32_t *s
         {
             for (int ch = child0[node]; ch < child0[node] + nchild[node]; ch++)
             {
+                if (mass[ch] == 0.0)
+                    continue; /* massless child: no moments, centre 0/0 */
                 const double d0 = comx[ch] - c[0], d1 = comy[ch] - c[1], d2 = comz[ch] - c[2];
                 const double M = mass[ch];
                 const double *qc = &quad[6 * ch];
@@ -842,6 +848,8 @@ void orc_octree_oct(int num, const double *x, const double *m, const int32_t *so
             for (int t = first[node]; t < first[node] + count[node]; t++)
             {
                 const int p = sorted_idx[t];
+                if (m[p] == 0.0)
+                    continue; /* as in orc_octree_quad */
                 const double d0 = x[p * 3 + 0] - c[0], d1 = x[p * 3 + 1] - c[1],
                              d2 = x[p * 3 + 2] - c[2];
                 const double mp = m[p];
@@ -861,6 +869,8 @@ void orc_octree_oct(int num, const double *x, const double *m, const int32_t *so
         {
             for (int ch = child0[node]; ch < child0[node] + nchild[node]; ch++)
             {
+                if (mass[ch] == 0.0)
+                    continue; /* as in orc_octree_quad */
                 const double d0 = comx[ch] - c[0], d1 = comy[ch] - c[1], d2 = comz[ch] - c[2];
                 const double M = mass[ch];
                 const double *q = &quad[6 * ch]; /* xx yy zz xy xz yz */

@@ -322,6 +322,26 @@ def test_group_expansion_ragged_and_clump():
         assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (len(ms), np.median(e), e.max())
 
 
+def test_group_expansion_with_massless_particles():
+    """A quarter of the particles massless, and whole massless cells: forces
+    within FP32 rounding of the oracle's.  (Interaction counts are not
+    compared: the oracle keeps the reference's 0/0 centre of a massless node,
+    which no opening test accepts, while the GPU places it on the node's
+    first particle, so it may accept such a node; it adds nothing either
+    way.)"""
+    x, v, m = ic.two_spheres(20000, 53)
+    m = m.copy()
+    m[::4] = 0.0
+    m[5000:5300] = 0.0  # whole massless cells
+    t = O.build_tree(x, m, 1)
+    a_o, cnt, far = _expansion_oracle(x, m, t, 0.5, 1e-3, 0.2, 1.25)
+    a_g = _expansion_gpu(x, m, 0.5, 1e-3, 1, 0.2, 1.25)
+    assert far > 0 and L.last_walk_expansion_pairs() > 0
+    assert np.all(np.isfinite(a_o)) and np.all(np.isfinite(a_g))
+    e = np.linalg.norm(a_g - a_o, axis=1) / np.linalg.norm(a_o, axis=1)
+    assert np.median(e) <= 1e-6 and e.max() <= 1e-4, (np.median(e), e.max())
+
+
 def test_group_expansion_tight_clump_without_softening():
     """A clump of 200 particles within 1e-5 of each other, eps = 0 (heavy far
     nodes a short distance from tight groups): every force is finite and
