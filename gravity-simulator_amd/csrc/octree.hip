@@ -370,6 +370,56 @@ __global__ void k_emit_children(NodeRec *__restrict__ nodes, int lb, int le, int
     nd.nchild = c - c0;
 }
 
+// Monopole sums of the sorted particles [first, first + count), in order, and
+// the centre of mass the walks use (X / M, or the first particle's position
+// for a massless node).
+__device__ __forceinline__ double4 leaf_monopole(const double4 *__restrict__ P, int first, int count,
+                                                 NodeRec &nd)
+{
+    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+    for (int s = first; s < first + count; s++) {
+        const double4 p = P[s];
+        M += p.w;
+        X += p.w * p.x;
+        Y += p.w * p.y;
+        Z += p.w * p.z;
+    }
+    nd.fmass = static_cast<float>(M);
+    if (M != 0.0) {
+        nd.cx = X / M;
+        nd.cy = Y / M;
+        nd.cz = Z / M;
+    } else {
+        const double4 p = P[first];
+        nd.cx = p.x;
+        nd.cy = p.y;
+        nd.cz = p.z;
+    }
+    return make_double4(M, X, Y, Z);
+}
+
+// FP32 second central moments of a leaf's particles about (c0, c1, c2), the
+// walks' copy (xx yy zz xy xz yz, 1.5 tr, 0)
+__device__ __forceinline__ void leaf_quadf(const double4 *__restrict__ P, int first, int count,
+                                           double c0, double c1, double c2, float4 *qf)
+{
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f, f4 = 0.0f, f5 = 0.0f;
+    for (int s = first; s < first + count; s++) {
+        const double4 p = P[s];
+        const float d0 = static_cast<float>(p.x - c0), d1 = static_cast<float>(p.y - c1),
+                    d2 = static_cast<float>(p.z - c2);
+        const float mp = static_cast<float>(p.w);
+        f0 = __builtin_fmaf(mp * d0, d0, f0);
+        f1 = __builtin_fmaf(mp * d1, d1, f1);
+        f2 = __builtin_fmaf(mp * d2, d2, f2);
+        f3 = __builtin_fmaf(mp * d0, d1, f3);
+        f4 = __builtin_fmaf(mp * d0, d2, f4);
+        f5 = __builtin_fmaf(mp * d1, d2, f5);
+    }
+    qf[0] = make_float4(f0, f1, f2, f3);
+    qf[1] = make_float4(f4, f5, 1.5f * (f0 + f1 + f2), 0.0f);
+}
+
 // moments: mom[node] = (sum m, sum m x, sum m y, sum m z); with QUAD also
 // the second central moments (only the quadrupole, expansion and feature
 // walks read them): in FP32 from the children's FP32 moments, or with OCT in
@@ -390,37 +440,43 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
     // write neighbouring records: full cache lines, no partial-line merges)
     NodeRec nd = nodes[node];
     gflag[node] = (node == 0 || nd.count > 64) ? 1u : 0u;
-    double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+    double4 mnode;
     if (nd.nchild == 0) {
-        for (int s = nd.first; s < nd.first + nd.count; s++) {
-            const double4 p = P[s];
-            M += p.w;
-            X += p.w * p.x;
-            Y += p.w * p.y;
-            Z += p.w * p.z;
-        }
+        mnode = leaf_monopole(P, nd.first, nd.count, nd);
     } else {
-        for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
-            const double4 q = mom[c];
-            M += q.x;
-            X += q.y;
-            Y += q.z;
-            Z += q.w;
+        // the children's moments are requested before any is summed (loads
+        // of up to 4 at once, clamped to the last child), then added in
+        // child order
+        double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+        const int cl = nd.child0 + nd.nchild - 1;
+        for (int c = nd.child0; c <= cl; c += 4) {
+            double4 q[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = mom[c + k < cl ? c + k : cl];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (c + k <= cl) {
+                    M += q[k].x;
+                    X += q[k].y;
+                    Y += q[k].z;
+                    Z += q[k].w;
+                }
+        }
+        mnode = make_double4(M, X, Y, Z);
+        nd.fmass = static_cast<float>(M);
+        if (M != 0.0) {
+            nd.cx = X / M;
+            nd.cy = Y / M;
+            nd.cz = Z / M;
+        } else {
+            // massless node: place its (zero) monopole on its first particle
+            const double4 p = P[nd.first];
+            nd.cx = p.x;
+            nd.cy = p.y;
+            nd.cz = p.z;
         }
     }
-    mom[node] = make_double4(M, X, Y, Z);
-    nd.fmass = static_cast<float>(M);
-    if (M != 0.0) {
-        nd.cx = X / M;
-        nd.cy = Y / M;
-        nd.cz = Z / M;
-    } else {
-        // massless node: place its (zero) monopole on its first particle
-        const double4 p = P[nd.first];
-        nd.cx = p.x;
-        nd.cy = p.y;
-        nd.cz = p.z;
-    }
+    mom[node] = mnode;
     nodes[node] = nd;
     if (!QUAD) return;
     const double c0 = nd.cx, c1 = nd.cy, c2 = nd.cz;
@@ -430,30 +486,34 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
         // xz yz, 1.5 tr): leaves from their particles, internal nodes from
         // their children's FP32 moments shifted to this centre of mass
         // (parallel axes; the offsets formed in FP64)
-        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f, f4 = 0.0f, f5 = 0.0f;
         if (nd.nchild == 0) {
-            for (int s = nd.first; s < nd.first + nd.count; s++) {
-                const double4 p = P[s];
-                const float d0 = static_cast<float>(p.x - c0), d1 = static_cast<float>(p.y - c1),
-                            d2 = static_cast<float>(p.z - c2);
-                const float mp = static_cast<float>(p.w);
-                f0 = __builtin_fmaf(mp * d0, d0, f0);
-                f1 = __builtin_fmaf(mp * d1, d1, f1);
-                f2 = __builtin_fmaf(mp * d2, d2, f2);
-                f3 = __builtin_fmaf(mp * d0, d1, f3);
-                f4 = __builtin_fmaf(mp * d0, d2, f4);
-                f5 = __builtin_fmaf(mp * d1, d2, f5);
+            leaf_quadf(P, nd.first, nd.count, c0, c1, c2, qf);
+            return;
+        }
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f, f4 = 0.0f, f5 = 0.0f;
+        const float4 *qs = reinterpret_cast<const float4 *>(quadf);
+        const int cl = nd.child0 + nd.nchild - 1;
+        for (int c = nd.child0; c <= cl; c += 4) {
+            double4 mk[4];
+            float4 qa[4], qb[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int ck = c + k < cl ? c + k : cl;
+                mk[k] = mom[ck];
+                qa[k] = qs[2 * static_cast<size_t>(ck)];
+                qb[k] = qs[2 * static_cast<size_t>(ck) + 1];
             }
-        } else {
-            for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
-                const double4 mc = mom[c];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (c + k > cl) break;
+                const double4 mc = mk[k];
                 double cc0, cc1, cc2;
                 if (mc.x != 0.0) {
                     cc0 = mc.y / mc.x;
                     cc1 = mc.z / mc.x;
                     cc2 = mc.w / mc.x;
                 } else {
-                    const double4 p = P[nodes[c].first];
+                    const double4 p = P[nodes[c + k].first];
                     cc0 = p.x;
                     cc1 = p.y;
                     cc2 = p.z;
@@ -461,14 +521,12 @@ __global__ void k_multipole(NodeRec *__restrict__ nodes, int lb, int le,
                 const float d0 = static_cast<float>(cc0 - c0), d1 = static_cast<float>(cc1 - c1),
                             d2 = static_cast<float>(cc2 - c2);
                 const float Mc = static_cast<float>(mc.x);
-                const float4 qa = reinterpret_cast<const float4 *>(quadf)[2 * static_cast<size_t>(c)];
-                const float4 qb = reinterpret_cast<const float4 *>(quadf)[2 * static_cast<size_t>(c) + 1];
-                f0 += __builtin_fmaf(Mc * d0, d0, qa.x);
-                f1 += __builtin_fmaf(Mc * d1, d1, qa.y);
-                f2 += __builtin_fmaf(Mc * d2, d2, qa.z);
-                f3 += __builtin_fmaf(Mc * d0, d1, qa.w);
-                f4 += __builtin_fmaf(Mc * d0, d2, qb.x);
-                f5 += __builtin_fmaf(Mc * d1, d2, qb.y);
+                f0 += __builtin_fmaf(Mc * d0, d0, qa[k].x);
+                f1 += __builtin_fmaf(Mc * d1, d1, qa[k].y);
+                f2 += __builtin_fmaf(Mc * d2, d2, qa[k].z);
+                f3 += __builtin_fmaf(Mc * d0, d1, qa[k].w);
+                f4 += __builtin_fmaf(Mc * d0, d2, qb[k].x);
+                f5 += __builtin_fmaf(Mc * d1, d2, qb[k].y);
             }
         }
         qf[0] = make_float4(f0, f1, f2, f3);
@@ -1088,15 +1146,45 @@ constexpr int GW_LIST = 128;
 // at most 7 per level of the remaining depth
 constexpr int GW_BATCH_LIMIT = GW_STACK - 8 * (MAXL + 2);
 
+// Wave-wide minimum / maximum of a double, every lane active: within each
+// row of 16 lanes by DPP moves (quad swaps, then row rotations by 4 and 8),
+// then the four rows' results read from lanes 0, 16, 32, 48.  No LDS
+// traffic; min and max are exact, so the order does not matter.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+    const int l = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int h = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(h) << 32) | static_cast<unsigned>(l));
+}
+__device__ __forceinline__ double lane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_ROR4 = 0x124;
+constexpr int DPP_ROW_ROR8 = 0x128;
 __device__ __forceinline__ double wave_min(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-    return v;
+    v = fmin(v, dpp_f64<DPP_QUAD_XOR1>(v));
+    v = fmin(v, dpp_f64<DPP_QUAD_XOR2>(v));
+    v = fmin(v, dpp_f64<DPP_ROW_ROR4>(v));
+    v = fmin(v, dpp_f64<DPP_ROW_ROR8>(v));
+    return fmin(fmin(lane_f64(v, 0), lane_f64(v, 16)), fmin(lane_f64(v, 32), lane_f64(v, 48)));
 }
 __device__ __forceinline__ double wave_max(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    v = fmax(v, dpp_f64<DPP_QUAD_XOR1>(v));
+    v = fmax(v, dpp_f64<DPP_QUAD_XOR2>(v));
+    v = fmax(v, dpp_f64<DPP_ROW_ROR4>(v));
+    v = fmax(v, dpp_f64<DPP_ROW_ROR8>(v));
+    return fmax(fmax(lane_f64(v, 0), lane_f64(v, 16)), fmax(lane_f64(v, 32), lane_f64(v, 48)));
 }
 // A value every lane holds alike, moved to scalar registers (the compiler
 // cannot see that a cross-lane reduction is uniform).
@@ -1149,17 +1237,24 @@ __global__ void k_group_parent_list(int num_nodes, const unsigned *__restrict__ 
 // One wavefront per parent node: its runs are formed in the same order by
 // every lane; each run's particles are loaded one per lane and its box is
 // reduced across the wave.  Groups are counted in a first launch (WRITE =
-// false: count[i] := their number) and written in a second from the
-// exclusive scan of the counts (WRITE = true: groups[off[i] ...]).
-template <bool WRITE>
-__device__ void emit_group(const double4 *__restrict__ P, int f, int c, double cap, int lane,
-                           int2 *__restrict__ groups, unsigned &at)
+// false: count[i] := their number, wide[i] := bit r set when run r of an
+// internal node is wider than the cap) and written in a second from the
+// exclusive scan of the counts (WRITE = true: groups[off[i] ...]), which
+// takes the internal nodes' widths from wide[i].
+__device__ __forceinline__ bool run_is_wide(double4 q, double cap)
 {
-    const double4 q = P[f + (lane < c ? lane : 0)];
+    // q: particle f + lane of the run (its first for lanes past the run)
     const double w0 = wave_max(q.x) - wave_min(q.x);
     const double w1 = wave_max(q.y) - wave_min(q.y);
     const double w2 = wave_max(q.z) - wave_min(q.z);
-    if (w0 > cap || w1 > cap || w2 > cap) {
+    return w0 > cap || w1 > cap || w2 > cap;
+}
+
+template <bool WRITE>
+__device__ __forceinline__ void emit_group(bool wide, int f, int c, int lane,
+                                           int2 *__restrict__ groups, unsigned &at)
+{
+    if (wide) {
         if (WRITE && lane < c) groups[at + lane] = make_int2(f + lane, 1);
         at += c;
     } else {
@@ -1173,6 +1268,7 @@ __global__ __launch_bounds__(256) void k_group_runs(const NodeRec *__restrict__ 
                                                     const int *__restrict__ parents, int num_parents,
                                                     int n, const double4 *__restrict__ P, double cap,
                                                     unsigned *__restrict__ count_or_off,
+                                                    unsigned *__restrict__ wide,
                                                     int2 *__restrict__ groups)
 {
     const int lane = threadIdx.x & 63;
@@ -1187,23 +1283,69 @@ __global__ __launch_bounds__(256) void k_group_runs(const NodeRec *__restrict__ 
         at += 1;
     } else if (nd.nchild == 0) {
         // a leaf of more than 64 particles: consecutive runs of 64
-        for (int f = nd.first; f < nd.first + nd.count; f += 64)
-            emit_group<WRITE>(P, f, min(64, nd.first + nd.count - f), cap, lane, groups, at);
+        for (int f = nd.first; f < nd.first + nd.count; f += 64) {
+            const int c = min(64, nd.first + nd.count - f);
+            const bool w = run_is_wide(P[f + (lane < c ? lane : 0)], cap);
+            emit_group<WRITE>(w, f, c, lane, groups, at);
+        }
     } else {
-        int start = -1, cnt = 0;
-        for (int c = nd.child0; c < nd.child0 + nd.nchild; c++) {
-            const int cf = nodes[c].first, cc = nodes[c].count;
-            if (cc > 64 || cnt + cc > 64) {
-                if (cnt > 0) emit_group<WRITE>(P, start, cnt, cap, lane, groups, at);
+        // the children's (first, count), one child per lane; the runs packed
+        // in child order, run r's (first, count) kept by lane r (at most 8
+        // runs: one per child)
+        int cf = 0, cc = 0;
+        if (lane < nd.nchild) {
+            cf = nodes[nd.child0 + lane].first;
+            cc = nodes[nd.child0 + lane].count;
+        }
+        int start = -1, cnt = 0, nr = 0, rf = 0, rn = 0;
+        for (int k = 0; k < nd.nchild; k++) {
+            const int kf = __shfl(cf, k, 64), kc = __shfl(cc, k, 64);
+            if (kc > 64 || cnt + kc > 64) {
+                if (cnt > 0) {
+                    if (lane == nr) {
+                        rf = start;
+                        rn = cnt;
+                    }
+                    nr++;
+                }
                 start = -1;
                 cnt = 0;
             }
-            if (cc <= 64) {
-                if (cnt == 0) start = cf;
-                cnt += cc;
+            if (kc <= 64) {
+                if (cnt == 0) start = kf;
+                cnt += kc;
             }
         }
-        if (cnt > 0) emit_group<WRITE>(P, start, cnt, cap, lane, groups, at);
+        if (cnt > 0) {
+            if (lane == nr) {
+                rf = start;
+                rn = cnt;
+            }
+            nr++;
+        }
+        if (WRITE) {
+            const unsigned mask = wide[i];
+            for (int r = 0; r < nr; r++)
+                emit_group<WRITE>((mask >> r) & 1u, __shfl(rf, r, 64), __shfl(rn, r, 64), lane,
+                                  groups, at);
+        } else {
+            // every run's particles requested before any is reduced
+            double4 q[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int f = __shfl(rf, r, 64), c = __shfl(rn, r, 64);
+                q[r] = r < nr ? P[f + (lane < c ? lane : 0)] : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            unsigned mask = 0;
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                if (r < nr) {
+                    const bool w = run_is_wide(q[r], cap);
+                    mask |= w ? 1u << r : 0u;
+                    emit_group<WRITE>(w, __shfl(rf, r, 64), __shfl(rn, r, 64), lane, groups, at);
+                }
+            if (lane == 0) wide[i] = mask;
+        }
     }
     if (!WRITE && lane == 0) count_or_off[i] = at;
 }
@@ -2032,8 +2174,10 @@ ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, dou
     unsigned *offs = static_cast<unsigned *>(hp);
     GS_TRY(slot(c, SLOT_HIST_SCAN, scan_temp_bytes(t->num_nodes) + 64, &hp));
     unsigned *stmp = static_cast<unsigned *>(hp);
-    GS_TRY(slot(c, SLOT_GROUP_PARENTS, sizeof(int) * static_cast<size_t>(t->num_nodes) + 256, &hp));
+    // (and after it, each parent's wide-run bits)
+    GS_TRY(slot(c, SLOT_GROUP_PARENTS, 2 * sizeof(int) * static_cast<size_t>(t->num_nodes) + 256, &hp));
     int *parents = static_cast<int *>(hp);
+    unsigned *wide = reinterpret_cast<unsigned *>(parents + t->num_nodes);
     const int rec = timing_begin(c, "bh_walk");
     const double cap = group_width_cap(t->width, t->n);
     const dim3 ngrid((t->num_nodes + 255) / 256);
@@ -2048,7 +2192,7 @@ ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, dou
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group parents beyond the nodes");
     const dim3 pgrid((num_parents + 3) / 4);  // 4 wavefronts per workgroup
     hipLaunchKernelGGL(k_group_runs<false>, pgrid, dim3(256), 0, c->stream, t->nodes, parents,
-                       static_cast<int>(num_parents), t->n, t->P, cap, offs, groups);
+                       static_cast<int>(num_parents), t->n, t->P, cap, offs, wide, groups);
     GS_HIP(hipGetLastError());
     GS_HIP(scan_exclusive(c->stream, offs, offs, num_parents, stmp, ng));
     unsigned num_groups = 0;
@@ -2057,7 +2201,7 @@ ErrorStatus tree_walk_group(Context *c, DeviceTree *t, double G, double eps, dou
     if (num_groups > static_cast<unsigned>(t->n))
         return WRAP_RAISE_ERROR(GRAV_FAILURE, "Barnes-Hut group list larger than the particles");
     hipLaunchKernelGGL(k_group_runs<true>, pgrid, dim3(256), 0, c->stream, t->nodes, parents,
-                       static_cast<int>(num_parents), t->n, t->P, cap, offs, groups);
+                       static_cast<int>(num_parents), t->n, t->P, cap, offs, wide, groups);
     GS_HIP(hipGetLastError());
     const int grid = (static_cast<int>(num_groups) + GW_WAVES - 1) / GW_WAVES;
     if (order >= 2)
