@@ -665,7 +665,7 @@ constexpr int FB_TILE = FB_BLOCK * FB_PER;  // particles per workgroup
 constexpr int FB_DEPTHS = MAXL + 1;
 constexpr int FB_WAVES = FB_BLOCK / 64;
 constexpr int FB_MAX_LEAF = 16;             // window search cost grows as leaf_cap^2
-constexpr int FB_SCAN = 64;                 // L entries a thread scans before searching keys
+constexpr int FB_SCAN = 64;                 // L entries a thread scans before leaving a cell open
 constexpr int FB_PAD = 4;                   // -1 entries after L_n (the scan reads 4 at a time)
 
 __device__ __forceinline__ int shared_digits(long long a, long long b)
@@ -763,7 +763,8 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
                                                       const signed char *__restrict__ D, int n,
                                                       int ntiles,
                                                       const unsigned *__restrict__ scanned,
-                                                      double w2, NodeRec *__restrict__ nodes)
+                                                      double w2, NodeRec *__restrict__ nodes,
+                                                      unsigned char *__restrict__ fix)
 {
     __shared__ unsigned base[FB_DEPTHS][FB_PER][FB_WAVES];
     const int lane = threadIdx.x & 63;
@@ -800,7 +801,7 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
         // children start at i and at the positions before that end with
         // L_j == d (all positions inside have L_j >= d).  Depths are visited
         // deepest first, so the scan resumes where the deeper cell ended; past
-        // FB_SCAN positions the keys are searched instead.
+        // FB_SCAN positions the cell is left to k_fb_fix.
         int j = i + 1;
         bool scan = true;
         for (int d = FB_DEPTHS - 1; d >= 0; d--) {
@@ -833,18 +834,15 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
             nd.prefix = keys[i] >> shift;
             nd.depth = d;
             nd.first = i;
-            nd.count = (d == 0 ? n : (end >= 0 ? end : cell_end(keys, n, i, shift))) - i;
+            // a cell the scan did not close gets its count and child count
+            // from searches on the keys in k_fb_fix, so that the few threads
+            // with long searches do not hold their workgroups here
+            nd.count = d == 0 ? n : (end >= 0 ? end - i : 0);
             nd.fmass = 0.0f;
+            fix[idx] = end < 0 ? 1 : 0;
             if (d < dd[r]) {
-                int nc = kids;
-                if (end < 0) {
-                    int b[9];
-                    octant_bounds(keys, i, nd.count, d + 1, b);
-                    nc = 0;
-                    for (int o = 0; o < 8; o++) nc += b[o + 1] > b[o] ? 1 : 0;
-                }
                 nd.child0 = below;
-                nd.nchild = nc;
+                nd.nchild = end < 0 ? 0 : kids;
             } else {
                 nd.child0 = -1;
                 nd.nchild = 0;
@@ -852,6 +850,28 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
             nodes[idx] = nd;
             below = idx;
         }
+    }
+}
+
+// The cells k_fb_emit left open (fix[node] != 0: the root, and cells whose
+// end lies more than FB_SCAN positions on): count from a galloping search for
+// the end of the cell's key prefix, internal nodes' child counts from the
+// octant bounds of the next digit.
+__global__ void k_fb_fix(NodeRec *__restrict__ nodes, int nn, const long long *__restrict__ keys,
+                         int n, const unsigned char *__restrict__ fix)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn || !fix[node]) return;
+    NodeRec &nd = nodes[node];
+    const int i = nd.first, d = nd.depth;
+    const int count = (d == 0 ? n : cell_end(keys, n, i, 3 * (MAXL - d))) - i;
+    nd.count = count;
+    if (nd.child0 >= 0) {
+        int b[9];
+        octant_bounds(keys, i, count, d + 1, b);
+        int nc = 0;
+        for (int o = 0; o < 8; o++) nc += b[o + 1] > b[o] ? 1 : 0;
+        nd.nchild = nc;
     }
 }
 
@@ -1972,8 +1992,13 @@ static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_ca
     GS_HIP(hipStreamSynchronize(c->stream));
     const int nn = static_cast<int>(lev_host[FB_DEPTHS]);
     GS_TRY(ensure_nodes(c, t, nn));
+    // per-node flags of the cells left open, in the child-count scratch (which
+    // only the level-by-level build uses; cap + 1 words >= nn bytes)
+    unsigned char *fix = reinterpret_cast<unsigned char *>(t->tmp);
     hipLaunchKernelGGL(k_fb_emit, dim3(ntiles), dim3(FB_BLOCK), 0, c->stream, t->skeys, Ld, Dd, n,
-                       ntiles, cnt, t->width * t->width, t->nodes);
+                       ntiles, cnt, t->width * t->width, t->nodes, fix);
+    hipLaunchKernelGGL(k_fb_fix, dim3((nn + 255) / 256), dim3(256), 0, c->stream, t->nodes, nn,
+                       t->skeys, n, fix);
     GS_HIP(hipGetLastError());
     int levels = 0;
     for (int d = 0; d < FB_DEPTHS; d++) {
