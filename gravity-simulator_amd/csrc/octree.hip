@@ -853,24 +853,76 @@ __global__ __launch_bounds__(FB_BLOCK) void k_fb_emit(const long long *__restric
     }
 }
 
+// First position after i whose depth-d prefix (key >> shift) differs from
+// i's, or n: eight independent probes per step, first at i + 8^k (k = 0..7),
+// then splitting the bracket nine ways, so that a cell of C particles costs
+// about 1 + log9 C dependent loads instead of the 2 log2 C of a galloping
+// binary search.  The prefix is constant on [i, end) (sorted keys).
+__device__ int cell_end8(const long long *__restrict__ keys, int n, int i, int shift)
+{
+    const long long pre = keys[i] >> shift;
+    int lo = i + 1, hi = n;  // invariant: the end lies in [lo, hi]
+    {
+        bool same[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const long long q = static_cast<long long>(i) + (1ll << (3 * k));
+            same[k] = q < n && (keys[q] >> shift) == pre;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const long long q = static_cast<long long>(i) + (1ll << (3 * k));
+            if (same[k]) lo = static_cast<int>(q) + 1;
+            else if (q < hi) hi = static_cast<int>(q);
+        }
+        if (lo > hi) lo = hi;  // (cannot happen for sorted keys)
+    }
+    while (lo < hi) {
+        const long long len = hi - lo;
+        int q[8];
+        bool same[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            q[k] = lo + static_cast<int>(len * (k + 1) / 9);  // in [lo, hi)
+            same[k] = (keys[q[k]] >> shift) == pre;
+        }
+        int nlo = lo, nhi = hi;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (same[k]) nlo = q[k] + 1 > nlo ? q[k] + 1 : nlo;
+            else nhi = q[k] < nhi ? q[k] : nhi;
+        }
+        lo = nlo < nhi ? nlo : nhi;
+        hi = nhi;
+    }
+    return lo;
+}
+
 // The cells k_fb_emit left open (fix[node] != 0: the root, and cells whose
-// end lies more than FB_SCAN positions on): count from a galloping search for
-// the end of the cell's key prefix, internal nodes' child counts from the
-// octant bounds of the next digit.
+// end lies more than FB_SCAN positions on): the count from the end of the
+// cell's key prefix; for an internal node the child count from the next
+// level's records: its children are the depth-(d + 1) nodes whose first
+// particle lies in the cell, contiguous from child0 (at most 8) and ending
+// with the level (lev[d + 2]: first node of depth d + 2, or the node count).
 __global__ void k_fb_fix(NodeRec *__restrict__ nodes, int nn, const long long *__restrict__ keys,
-                         int n, const unsigned char *__restrict__ fix)
+                         int n, const unsigned char *__restrict__ fix,
+                         const unsigned *__restrict__ lev)
 {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= nn || !fix[node]) return;
     NodeRec &nd = nodes[node];
     const int i = nd.first, d = nd.depth;
-    const int count = (d == 0 ? n : cell_end(keys, n, i, 3 * (MAXL - d))) - i;
-    nd.count = count;
+    const int end = d == 0 ? n : cell_end8(keys, n, i, 3 * (MAXL - d));
+    nd.count = end - i;
     if (nd.child0 >= 0) {
-        int b[9];
-        octant_bounds(keys, i, count, d + 1, b);
+        const int c0 = nd.child0;
+        const int le = static_cast<int>(lev[d + 2]);
+        int first[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) first[k] = c0 + k < le ? nodes[c0 + k].first : end;
         int nc = 0;
-        for (int o = 0; o < 8; o++) nc += b[o + 1] > b[o] ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) nc += first[k] < end ? 1 : 0;
         nd.nchild = nc;
     }
 }
@@ -1998,7 +2050,7 @@ static ErrorStatus tree_build_flat(Context *c, DeviceTree *t, int n, int leaf_ca
     hipLaunchKernelGGL(k_fb_emit, dim3(ntiles), dim3(FB_BLOCK), 0, c->stream, t->skeys, Ld, Dd, n,
                        ntiles, cnt, t->width * t->width, t->nodes, fix);
     hipLaunchKernelGGL(k_fb_fix, dim3((nn + 255) / 256), dim3(256), 0, c->stream, t->nodes, nn,
-                       t->skeys, n, fix);
+                       t->skeys, n, fix, lev);
     GS_HIP(hipGetLastError());
     int levels = 0;
     for (int d = 0; d < FB_DEPTHS; d++) {
