@@ -54,11 +54,10 @@ struct DeviceTree {
 
 // Stable LSD radix sort of n (key, index) pairs on the low 8 * passes bits of
 // the (non-negative) keys, alternating between (k0, i0) and (k1, i1); the
-// sorted pairs end in (*ko, *io).  hist holds radix_hist_elements(n) counters
-// and stmp scan_temp_bytes(radix_hist_elements(n)) bytes.
+// sorted pairs end in (*ko, *io).  hist holds radix_hist_elements(n) counters.
 size_t radix_hist_elements(int n);
 ErrorStatus radix_sort_pairs(Context *c, long long *k0, int *i0, long long *k1, int *i1, int n,
-                             int passes, unsigned *hist, unsigned *stmp, long long **ko, int **io);
+                             int passes, unsigned *hist, long long **ko, int **io);
 // Bounding box, Morton keys and the stable sort.  With m and packed not
 // NULL the key kernel also writes packed[i] = (x_i, y_i, z_i, m_i) in the
 // input order, for the gather into sorted order.

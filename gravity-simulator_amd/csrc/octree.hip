@@ -173,6 +173,52 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const long long *__restric
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Exclusive scan of each digit's row of tile counts (digit-major layout: a
+// digit's counts for every tile are contiguous), one workgroup per digit;
+// dtot[d] := the digit's total, whose exclusive scan over the digits the
+// scatter adds.  A thread takes RS_ROW consecutive tiles per chunk of
+// RS_BLOCK * RS_ROW, so one chunk covers up to 2^22 keys.
+constexpr int RS_ROW = 8;
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_rowscan(unsigned *__restrict__ hist, int ntiles,
+                                                         unsigned *__restrict__ dtot)
+{
+    __shared__ unsigned wsum[RS_WAVES];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    unsigned *row = hist + static_cast<size_t>(blockIdx.x) * ntiles;
+    unsigned carry = 0;
+    for (int base = 0; base < ntiles; base += RS_BLOCK * RS_ROW) {
+        const int q0 = base + threadIdx.x * RS_ROW;
+        unsigned v[RS_ROW];
+#pragma unroll
+        for (int k = 0; k < RS_ROW; k++) v[k] = q0 + k < ntiles ? row[q0 + k] : 0u;
+        unsigned mine = 0;
+#pragma unroll
+        for (int k = 0; k < RS_ROW; k++) mine += v[k];
+        unsigned incl = mine;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        unsigned run = carry, all = 0;
+        for (int w = 0; w < RS_WAVES; w++) {
+            if (w < wave) run += wsum[w];
+            all += wsum[w];
+        }
+        run += incl - mine;
+#pragma unroll
+        for (int k = 0; k < RS_ROW; k++) {
+            if (q0 + k < ntiles) row[q0 + k] = run;
+            run += v[k];
+        }
+        carry += all;
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
 // Stable scatter: wave w owns tile positions [w*512, (w+1)*512), visited in
 // 8 rounds of 64 consecutive keys; within a round, peers with equal digits
 // are found with 8 ballots and ranked by lane order.  The tile is then
@@ -183,6 +229,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__rest
                                                          const int *__restrict__ iin, int n,
                                                          int shift,
                                                          const unsigned *__restrict__ offs,
+                                                         const unsigned *__restrict__ dtot,
                                                          int ntiles,
                                                          long long *__restrict__ kout,
                                                          int *__restrict__ iout)
@@ -191,6 +238,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__rest
     __shared__ unsigned toff[RS_BINS];   // first tile position of each digit
     __shared__ unsigned goff[RS_BINS];   // first output position of each digit
     __shared__ unsigned wsum[RS_WAVES];
+    __shared__ unsigned dsum[RS_WAVES];
     __shared__ long long skey[RS_TILE];
     __shared__ int sval[RS_TILE];
     const int lane = threadIdx.x & 63;
@@ -228,22 +276,36 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const long long *__rest
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // tile offsets of the digits: exclusive scan of the tile histogram
+    // tile offsets of the digits: exclusive scan of the tile histogram; the
+    // digits' first output positions: exclusive scan of the digit totals
+    // plus this tile's offset within its digit's row
     {
         const int d = threadIdx.x;
         unsigned tot = 0;
         for (int w = 0; w < RS_WAVES; w++) tot += wcount[w][d];
-        unsigned incl = tot;
+        const unsigned dt = dtot[d];
+        unsigned incl = tot, dincl = dt;
         for (int off = 1; off < 64; off <<= 1) {
             const unsigned u = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += u;
+            const unsigned du = __shfl_up(dincl, off, 64);
+            if (lane >= off) {
+                incl += u;
+                dincl += du;
+            }
         }
-        if (lane == 63) wsum[wave] = incl;
-        goff[d] = offs[d * ntiles + blockIdx.x];
+        if (lane == 63) {
+            wsum[wave] = incl;
+            dsum[wave] = dincl;
+        }
+        const unsigned row = offs[d * ntiles + blockIdx.x];
         __syncthreads();
-        unsigned add = 0;
-        for (int w = 0; w < wave; w++) add += wsum[w];
+        unsigned add = 0, dadd = 0;
+        for (int w = 0; w < wave; w++) {
+            add += wsum[w];
+            dadd += dsum[w];
+        }
         toff[d] = add + incl - tot;
+        goff[d] = dadd + dincl - dt + row;
     }
     __syncthreads();
     for (int r = 0; r < RS_ROUNDS; r++) {
@@ -1879,23 +1941,23 @@ static ErrorStatus ensure_nodes(Context *c, DeviceTree *t, int need)
 
 size_t radix_hist_elements(int n)
 {
-    return static_cast<size_t>((n + RS_TILE - 1) / RS_TILE) * RS_BINS;
+    // the tile counts, then the digit totals
+    return static_cast<size_t>((n + RS_TILE - 1) / RS_TILE) * RS_BINS + RS_BINS;
 }
 
 ErrorStatus radix_sort_pairs(Context *c, long long *k0, int *i0, long long *k1, int *i1, int n,
-                             int passes, unsigned *hist, unsigned *stmp, long long **ko, int **io)
+                             int passes, unsigned *hist, long long **ko, int **io)
 {
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
-    const size_t hn = radix_hist_elements(n);
-    if (static_cast<long long>(hn) > scan_max_elements())
-        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Too many elements for the radix sort: %d", n);
+    unsigned *dtot = hist + static_cast<size_t>(ntiles) * RS_BINS;
     for (int pass = 0; pass < passes; pass++) {
         const int shift = pass * RS_BITS;
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, n, shift,
                            hist, ntiles);
-        GS_HIP(scan_exclusive(c->stream, hist, hist, static_cast<long long>(hn), stmp, nullptr));
+        hipLaunchKernelGGL(k_rs_rowscan, dim3(RS_BINS), dim3(RS_BLOCK), 0, c->stream, hist, ntiles,
+                           dtot);
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, k0, i0, n,
-                           shift, hist, ntiles, k1, i1);
+                           shift, hist, dtot, ntiles, k1, i1);
         GS_HIP(hipGetLastError());
         long long *tk = k0;
         k0 = k1;
@@ -1914,9 +1976,6 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
 {
     if (!m) packed = nullptr;
     if (n <= 0) return WRAP_RAISE_ERROR(GRAV_VALUE_ERROR, "Number of particles must be positive");
-    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
-    if (static_cast<long long>(ntiles) * RS_BINS > scan_max_elements())
-        return WRAP_RAISE_ERROR_FMT(GRAV_VALUE_ERROR, "Too many particles for the radix sort: %d", n);
     void *p;
     GS_TRY(slot(c, SLOT_BBOX, 64 * sizeof(double) + 6 * 1024 * sizeof(double), &p));
     double *box = static_cast<double *>(p);
@@ -1929,11 +1988,8 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
     int *i0 = static_cast<int *>(p);
     GS_TRY(slot(c, SLOT_IDX_ALT, sizeof(int) * n, &p));
     int *i1 = static_cast<int *>(p);
-    const size_t hn = static_cast<size_t>(ntiles) * RS_BINS;
-    GS_TRY(slot(c, SLOT_HIST, sizeof(unsigned) * hn, &p));
+    GS_TRY(slot(c, SLOT_HIST, sizeof(unsigned) * radix_hist_elements(n), &p));
     unsigned *hist = static_cast<unsigned *>(p);
-    GS_TRY(slot(c, SLOT_HIST_SCAN, scan_temp_bytes(hn) + 64, &p));
-    unsigned *stmp = static_cast<unsigned *>(p);
 
     const int rec = timing_begin(c, "tree_sort");
     if (box_center && box_width > 0.0) {
@@ -1950,7 +2006,7 @@ ErrorStatus tree_sort(Context *c, const double *x, int n, const double *box_cent
     GS_HIP(hipGetLastError());
     long long *ks;
     int *is;
-    GS_TRY(radix_sort_pairs(c, k0, i0, k1, i1, n, RS_PASSES, hist, stmp, &ks, &is));
+    GS_TRY(radix_sort_pairs(c, k0, i0, k1, i1, n, RS_PASSES, hist, &ks, &is));
     k0 = ks;
     i0 = is;
     timing_end(c, rec);

@@ -351,7 +351,7 @@ struct PmWork {
     long long *key[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
     double4 *wm = nullptr;    // (wx, wy, wz, m) in sorted order
-    unsigned *hist = nullptr, *stmp = nullptr;
+    unsigned *hist = nullptr;
     hipfftHandle r2c = 0, c2r = 0;
     bool have_r2c = false, have_c2r = false;
 
@@ -364,9 +364,8 @@ struct PmWork {
             idx[k] = nullptr;
         }
         if (hist) (void)hipFree(hist);
-        if (stmp) (void)hipFree(stmp);
         if (wm) (void)hipFree(wm);
-        hist = stmp = nullptr;
+        hist = nullptr;
         wm = nullptr;
         cap = 0;
     }
@@ -383,7 +382,6 @@ struct PmWork {
         }
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&wm), sizeof(double4) * n));
         GS_HIP(hipMalloc(reinterpret_cast<void **>(&hist), sizeof(unsigned) * hn));
-        GS_HIP(hipMalloc(reinterpret_cast<void **>(&stmp), scan_temp_bytes(static_cast<long long>(hn)) + 64));
         cap = n;
         return make_success_error_status();
     }
@@ -467,7 +465,7 @@ ErrorStatus pm_acceleration(PmWork &W, const double *x, const double *m, int n,
     GS_HIP(hipGetLastError());
     long long *skey;
     int *sidx;
-    GS_TRY(radix_sort_pairs(c, W.key[0], W.idx[0], W.key[1], W.idx[1], n, W.passes, W.hist, W.stmp,
+    GS_TRY(radix_sort_pairs(c, W.key[0], W.idx[0], W.key[1], W.idx[1], n, W.passes, W.hist,
                             &skey, &sidx));
     hipLaunchKernelGGL(k_pm_clear_ranges, grid_for(n3g), dim3(PM_BLOCK), 0, c->stream, W.range, n3g);
     hipLaunchKernelGGL(k_pm_cell_ranges, grid_for(n), dim3(PM_BLOCK), 0, c->stream, skey, n, W.range);
