@@ -1670,7 +1670,6 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
     auto sum_parts = [&](int m) {
         float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
         float mn = __builtin_inff();
-        unsigned self = 0;
         for (int e = 0; e < m; e++) {
             const float4 q = pl[e];
             const float dx = q.x - xr, dy = q.y - yr, dz = q.z - zr;
@@ -1681,7 +1680,6 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
             const bool me = pi[e] == s;
             const float w = me ? 0.0f : (q.w * ri) * (ri * ri);
             mn = fminf(mn, me ? __builtin_inff() : r2);
-            self += me ? 1u : 0u;
             f0 = __builtin_fmaf(dx, w, f0);
             f1 = __builtin_fmaf(dy, w, f1);
             f2 = __builtin_fmaf(dz, w, f2);
@@ -1717,7 +1715,7 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
         A0 += static_cast<double>(f0);
         A1 += static_cast<double>(f1);
         A2 += static_cast<double>(f2);
-        cnt += m - self;
+        cnt += m;
     };
 
     // sum the first 64 particle-list entries, move the rest (< 64) to the front
@@ -1891,7 +1889,9 @@ __global__ __launch_bounds__(GW_BLOCK) void k_walk_group(const NodeRec *__restri
         a[3 * i + 1] = args.G * A1;
         a[3 * i + 2] = args.G * A2;
     }
-    unsigned long long tot = valid ? cnt : 0u;
+    // (a particle's own entry is in its group's particle list exactly once:
+    // every node holding it is opened, down to its leaf)
+    unsigned long long tot = valid ? cnt - 1u : 0u;
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
     if (lane == 0 && n_int) {
         atomicAdd(n_int, tot + (overflow ? (1ull << 62) : 0ull));
